@@ -1,0 +1,762 @@
+// Host side of the clexmc C ABI: owns device memory, prepares per-supercell
+// padded geometry (halo copy lists, neighbour offsets, independent-set
+// classes) and launches the gfx950 kernels on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/clexmc.h"
+#include "clexmc_device.h"
+#include "clexmc_kernels.h"
+
+using namespace clexmc;
+
+namespace {
+
+struct HostSupercell {
+  DevSupercell dev;  // device pointers filled in
+  std::vector<void *> allocs;
+};
+
+}  // namespace
+
+struct clexmc_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // system (host copies)
+  int n_sublat = 0, n_occ_max = 0, n_func_max = 0, n_orbits = 0, n_corr = 0, binary = 0;
+  std::vector<int> n_occ, orbit_k, orbit_mult, orbit_nfunc, orbit_func_begin;
+  std::vector<int> flower_begin, entry_k, entry_site_begin, entry_sites, entry_table;
+  std::vector<int> entry_home;  // home sublattice of each entry
+  std::vector<double> eci;
+  int halo = 0;
+  DevSystem dsys{};
+  DevSystem *d_sys = nullptr;
+  std::vector<void *> sys_allocs;
+  // supercells
+  std::vector<HostSupercell> scs;
+  DevSupercell *d_scs = nullptr;
+  bool scs_dirty = true;
+  // chains
+  int n_chains = 0;
+  std::vector<int> chain_sc, chain_off;
+  long long occ_total = 0;
+  DevChain *d_chains = nullptr;
+  int8_t *d_occ = nullptr;
+  long long *d_binC = nullptr;
+  double *d_sampE = nullptr;
+  int *d_sampN = nullptr;
+  int *d_sample_count = nullptr;
+  int sample_cap = 0;
+  int block_size = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+};
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      h->err = std::string(#expr) + ": " + hipGetErrorString(_e);                  \
+      return CLEXMC_ERR_HIP;                                                       \
+    }                                                                              \
+  } while (0)
+
+#define ARG_CHECK(cond, msg)   \
+  do {                         \
+    if (!(cond)) {             \
+      h->err = (msg);          \
+      return CLEXMC_ERR_ARG;   \
+    }                          \
+  } while (0)
+
+template <typename T>
+static hipError_t upload(clexmc_handle *h, std::vector<void *> &allocs, const T *src, size_t n, T **dst) {
+  *dst = nullptr;
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  allocs.push_back(p);
+  if (src) {
+    e = hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  } else {
+    e = hipMemset(p, 0, n * sizeof(T));
+    if (e != hipSuccess) return e;
+  }
+  *dst = (T *)p;
+  return hipSuccess;
+}
+
+static void free_all(std::vector<void *> &v) {
+  for (void *p : v) hipFree(p);
+  v.clear();
+}
+
+extern "C" {
+
+const char *clexmc_version(void) { return "clexmc 0.1 (gfx950)"; }
+
+const char *clexmc_last_error(clexmc_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_handle **out) {
+  if (!out || !s) return CLEXMC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CLEXMC_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return CLEXMC_ERR_ARG;
+  clexmc_handle *h = new clexmc_handle();
+  h->device = device;
+  *out = h;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&h->ev0));
+  HIP_TRY(hipEventCreate(&h->ev1));
+
+  ARG_CHECK(s->n_sublat >= 1 && s->n_sublat <= kMaxSublat, "n_sublat out of range (1..8)");
+  ARG_CHECK(s->n_occ_max >= 1 && s->n_occ_max <= kMaxOcc, "n_occ_max out of range (1..4)");
+  ARG_CHECK(s->n_func_max >= 0 && s->n_func_max <= kMaxFunc, "n_func_max out of range (0..3)");
+  ARG_CHECK(s->n_orbits >= 1 && s->n_corr >= 1, "empty cluster expansion");
+  h->n_sublat = s->n_sublat;
+  h->n_occ_max = s->n_occ_max;
+  h->n_func_max = s->n_func_max;
+  h->n_orbits = s->n_orbits;
+  h->n_corr = s->n_corr;
+  h->binary = s->binary && s->n_orbits <= kMaxOrbits;
+  h->n_occ.assign(s->n_occ, s->n_occ + s->n_sublat);
+  h->orbit_k.assign(s->orbit_k, s->orbit_k + s->n_orbits);
+  h->orbit_mult.assign(s->orbit_mult, s->orbit_mult + s->n_orbits);
+  h->orbit_nfunc.assign(s->orbit_nfunc, s->orbit_nfunc + s->n_orbits);
+  h->orbit_func_begin.assign(s->orbit_func_begin, s->orbit_func_begin + s->n_orbits);
+  h->flower_begin.assign(s->flower_begin, s->flower_begin + s->n_sublat + 1);
+  h->entry_k.assign(s->entry_k, s->entry_k + s->n_entries);
+  h->entry_site_begin.assign(s->entry_site_begin, s->entry_site_begin + s->n_entries + 1);
+  h->entry_sites.assign(s->entry_sites, s->entry_sites + 4 * s->n_entry_sites);
+  h->entry_table.assign(s->entry_table, s->entry_table + s->n_entries);
+  h->eci.assign(s->eci, s->eci + s->n_corr);
+  ARG_CHECK(h->flower_begin[s->n_sublat] == s->n_entries, "flower_begin inconsistent with n_entries");
+  ARG_CHECK(h->entry_site_begin[s->n_entries] == s->n_entry_sites, "entry_site_begin inconsistent");
+
+  DevSystem &d = h->dsys;
+  d.n_sublat = s->n_sublat;
+  d.n_occ_max = s->n_occ_max;
+  d.n_func_max = s->n_func_max;
+  d.n_orbits = s->n_orbits;
+  d.n_corr = s->n_corr;
+  d.binary = h->binary;
+  for (int b = 0; b < s->n_sublat; ++b) {
+    d.n_occ[b] = s->n_occ[b];
+    ARG_CHECK(s->n_occ[b] >= 1 && s->n_occ[b] <= kMaxOcc, "n_occ out of range");
+    d.n_func[b] = s->n_occ[b] - 1;
+    for (int f = 0; f < kMaxFunc; ++f)
+      for (int o = 0; o < kMaxOcc; ++o) {
+        float v = 0.f;
+        if (f < s->n_func_max && o < s->n_occ_max)
+          v = (float)s->site_func[(b * s->n_func_max + f) * s->n_occ_max + o];
+        d.site_func[b][f][o] = v;
+      }
+  }
+  // home sublattice and radix multipliers of entry sites; halo width
+  h->entry_home.assign(s->n_entries, 0);
+  std::vector<int> entry_mul(std::max(1, s->n_entry_sites), 1);
+  int halo = 0;
+  for (int b = 0; b < s->n_sublat; ++b)
+    for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+      h->entry_home[e] = b;
+      int mul = 1;
+      for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q) {
+        const int bq = h->entry_sites[4 * q];
+        ARG_CHECK(bq >= 0 && bq < s->n_sublat, "entry site sublattice out of range");
+        entry_mul[q] = mul;
+        mul *= s->n_occ[bq];
+        for (int c = 1; c < 4; ++c) halo = std::max(halo, std::abs(h->entry_sites[4 * q + c]));
+      }
+    }
+  h->halo = halo;
+  // flower tables (FP32)
+  std::vector<float> ft(std::max(1, s->n_flower_tables));
+  for (int i = 0; i < s->n_flower_tables; ++i) ft[i] = (float)s->flower_tables[i];
+  auto &A = h->sys_allocs;
+  HIP_TRY(upload(h, A, h->flower_begin.data(), h->flower_begin.size(), (int **)&d.flower_begin));
+  HIP_TRY(upload(h, A, h->entry_k.data(), h->entry_k.size(), (int **)&d.entry_k));
+  HIP_TRY(upload(h, A, h->entry_site_begin.data(), h->entry_site_begin.size(), (int **)&d.entry_site_begin));
+  HIP_TRY(upload(h, A, (const int4 *)h->entry_sites.data(), (size_t)s->n_entry_sites, (int4 **)&d.entry_sites));
+  HIP_TRY(upload(h, A, entry_mul.data(), entry_mul.size(), (int **)&d.entry_mul));
+  HIP_TRY(upload(h, A, h->entry_table.data(), h->entry_table.size(), (int **)&d.entry_table));
+  HIP_TRY(upload(h, A, ft.data(), ft.size(), (float **)&d.flower_tables));
+  // binary ranges
+  if (h->binary) {
+    std::vector<int2> rng((size_t)s->n_sublat * s->n_orbits, make_int2(0, 0));
+    for (int b = 0; b < s->n_sublat; ++b)
+      for (int g = s->bin_seg_begin[b]; g < s->bin_seg_begin[b + 1]; ++g) {
+        const int o = s->bin_segs[3 * g], e0 = s->bin_segs[3 * g + 1], e1 = s->bin_segs[3 * g + 2];
+        ARG_CHECK(o >= 0 && o < s->n_orbits, "binary segment orbit out of range");
+        ARG_CHECK(rng[b * s->n_orbits + o].y == 0, "binary segments must be contiguous per orbit");
+        rng[b * s->n_orbits + o] = make_int2(e0, e1);
+      }
+    for (int a = 0; a < s->n_orbits; ++a) {
+      d.bin_w[a] = (float)s->bin_orbit_weight[a];
+      d.bin_w64[a] = s->bin_orbit_weight[a];
+      d.orbit_k[a] = s->orbit_k[a];
+      ARG_CHECK(s->orbit_nfunc[a] == 1, "binary path requires one function per orbit");
+    }
+    HIP_TRY(upload(h, A, rng.data(), rng.size(), (int2 **)&d.bin_range));
+  }
+  // correlations
+  std::vector<int> func_orbit(s->n_corr, 0);
+  for (int a = 0; a < s->n_orbits; ++a)
+    for (int f = 0; f < s->orbit_nfunc[a]; ++f) func_orbit[s->orbit_func_begin[a] + f] = a;
+  HIP_TRY(upload(h, A, s->orbit_k, (size_t)s->n_orbits, (int **)&d.orbit_k_all));
+  HIP_TRY(upload(h, A, s->orbit_mult, (size_t)s->n_orbits, (int **)&d.orbit_mult));
+  HIP_TRY(upload(h, A, s->orbit_nfunc, (size_t)s->n_orbits, (int **)&d.orbit_nfunc));
+  HIP_TRY(upload(h, A, s->orbit_func_begin, (size_t)s->n_orbits, (int **)&d.orbit_func_begin));
+  HIP_TRY(upload(h, A, s->corr_orbit_begin, (size_t)s->n_orbits + 1, (int **)&d.corr_orbit_begin));
+  HIP_TRY(upload(h, A, s->corr_elem_begin, (size_t)s->n_orbits + 1, (int **)&d.corr_elem_begin));
+  HIP_TRY(upload(h, A, (const int4 *)s->corr_elem_sites, (size_t)s->n_corr_elem_sites, (int4 **)&d.corr_elem_sites));
+  HIP_TRY(upload(h, A, s->orbit_table_offset, (size_t)s->n_orbits, (int **)&d.orbit_table_offset));
+  HIP_TRY(upload(h, A, s->orbit_tables, (size_t)s->n_orbit_tables, (double **)&d.orbit_tables));
+  HIP_TRY(upload(h, A, func_orbit.data(), func_orbit.size(), (int **)&d.func_orbit));
+  HIP_TRY(upload(h, A, &h->dsys, 1, &h->d_sys));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc *sc, int *id_out) {
+  if (!h || !sc || !id_out) return CLEXMC_ERR_ARG;
+  HIP_TRY(hipSetDevice(h->device));
+  const int L1 = sc->L[0], L2 = sc->L[1], L3 = sc->L[2];
+  ARG_CHECK(L1 >= 1 && L2 >= 1 && L3 >= 1 && L1 <= 4096 && L2 <= 4096 && L3 <= 4096, "bad supercell dims");
+  HostSupercell hs;
+  DevSupercell &d = hs.dev;
+  const int hw = h->halo;
+  d.L1 = L1;
+  d.L2 = L2;
+  d.L3 = L3;
+  d.h = hw;
+  d.P1 = L1 + 2 * hw;
+  d.P2 = L2 + 2 * hw;
+  d.P3 = L3 + 2 * hw;
+  const long long S = (long long)d.P1 * d.P2 * d.P3;
+  const long long pad = S * h->n_sublat;
+  ARG_CHECK(pad < (1LL << 30), "padded image too large");
+  d.S = (int)S;
+  d.pad_bytes = (int)pad;
+  d.n_cells = L1 * L2 * L3;
+  d.n_sites = d.n_cells * h->n_sublat;
+  const int N = d.n_sites;
+  auto pad_addr = [&](int b, int i, int j, int k) {
+    return (int)(b * S + (long long)(k + hw) * d.P1 * d.P2 + (long long)(j + hw) * d.P1 + (i + hw));
+  };
+  auto decode = [&](int l, int &b, int &i, int &j, int &k) {
+    b = l / d.n_cells;
+    int c = l - b * d.n_cells;
+    i = c % L1;
+    j = (c / L1) % L2;
+    k = c / (L1 * L2);
+  };
+  // all padded images of site l other than its main image
+  auto images = [&](int l, std::vector<int> &out) {
+    int b, i, j, k;
+    decode(l, b, i, j, k);
+    out.clear();
+    std::vector<int> pis, pjs, pks;
+    for (int m = -(hw / L1 + 2); m <= hw / L1 + 2; ++m) {
+      int p = i + hw + m * L1;
+      if (p >= 0 && p < d.P1) pis.push_back(p);
+    }
+    for (int m = -(hw / L2 + 2); m <= hw / L2 + 2; ++m) {
+      int p = j + hw + m * L2;
+      if (p >= 0 && p < d.P2) pjs.push_back(p);
+    }
+    for (int m = -(hw / L3 + 2); m <= hw / L3 + 2; ++m) {
+      int p = k + hw + m * L3;
+      if (p >= 0 && p < d.P3) pks.push_back(p);
+    }
+    const int main = pad_addr(b, i, j, k);
+    for (int pk : pks)
+      for (int pj : pjs)
+        for (int pi : pis) {
+          int a = (int)(b * S + (long long)pk * d.P1 * d.P2 + (long long)pj * d.P1 + pi);
+          if (a != main) out.push_back(a);
+        }
+  };
+  // classes
+  ARG_CHECK(sc->n_classes >= 1, "no independent-set classes");
+  std::vector<int> cb(sc->class_begin, sc->class_begin + sc->n_classes + 1);
+  ARG_CHECK(cb[0] == 0, "class_begin[0] must be 0");
+  std::vector<int> class_addr(std::max(1, cb.back())), class_lin(std::max(1, cb.back())),
+      class_sub(sc->n_classes);
+  std::vector<int> halo_begin(1, 0);
+  std::vector<int2> halo;
+  std::vector<int> img;
+  std::vector<char> seen(N, 0);
+  int max_class = 1;
+  for (int c = 0; c < sc->n_classes; ++c) {
+    ARG_CHECK(cb[c + 1] >= cb[c], "class_begin not monotone");
+    max_class = std::max(max_class, cb[c + 1] - cb[c]);
+    int b0 = -1;
+    for (int t = cb[c]; t < cb[c + 1]; ++t) {
+      const int l = sc->class_sites[t];
+      ARG_CHECK(l >= 0 && l < N, "class site out of range");
+      ARG_CHECK(!seen[l], "site listed in two classes");
+      seen[l] = 1;
+      int b, i, j, k;
+      decode(l, b, i, j, k);
+      ARG_CHECK(h->n_occ[b] >= 2, "class contains a fixed site");
+      if (b0 < 0) b0 = b;
+      ARG_CHECK(b == b0, "a class must hold sites of one sublattice");
+      class_addr[t] = pad_addr(b, i, j, k);
+      class_lin[t] = l;
+      images(l, img);
+      for (int a : img) halo.push_back(make_int2(class_addr[t], a));
+    }
+    class_sub[c] = b0 < 0 ? 0 : b0;
+    halo_begin.push_back((int)halo.size());
+  }
+  for (int l = 0; l < N; ++l) {
+    int b = l / d.n_cells;
+    if (h->n_occ[b] >= 2) ARG_CHECK(seen[l], "a variable site is in no class");
+  }
+  // entry offsets
+  std::vector<int> entry_off(std::max<size_t>(1, h->entry_sites.size() / 4));
+  for (size_t e = 0; e < h->entry_k.size(); ++e) {
+    const int b = h->entry_home[e];
+    for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q) {
+      const int bq = h->entry_sites[4 * q], di = h->entry_sites[4 * q + 1], dj = h->entry_sites[4 * q + 2],
+                dk = h->entry_sites[4 * q + 3];
+      entry_off[q] = (int)((long long)(bq - b) * S + (long long)dk * d.P1 * d.P2 + (long long)dj * d.P1 + di);
+    }
+  }
+  // bounds validation of every LDS neighbour access from every class site
+  for (int c = 0; c < sc->n_classes; ++c)
+    for (int t = cb[c]; t < cb[c + 1]; ++t) {
+      const int b = class_sub[c];
+      for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e)
+        for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q) {
+          long long a = (long long)class_addr[t] + entry_off[q];
+          ARG_CHECK(a >= 0 && a < pad, "neighbour access outside the padded image");
+        }
+    }
+  // pair phases (canonical cross-class mixing)
+  std::vector<int> pair_begin(1, 0), pair_sub, pair_halo_begin(1, 0);
+  std::vector<int2> pair_addr, pair_halo;
+  for (int ph = 0; ph < sc->n_pair_phases; ++ph) {
+    const int p0 = sc->pair_phase_begin[ph], p1 = sc->pair_phase_begin[ph + 1];
+    ARG_CHECK(p1 >= p0, "pair_phase_begin not monotone");
+    for (int t = p0; t < p1; ++t) {
+      const int ls = sc->pair_sites[2 * t], lt = sc->pair_sites[2 * t + 1];
+      ARG_CHECK(ls >= 0 && ls < N && lt >= 0 && lt < N && ls != lt, "bad pair site");
+      int b, i, j, k, b2, i2, j2, k2;
+      decode(ls, b, i, j, k);
+      decode(lt, b2, i2, j2, k2);
+      ARG_CHECK(b == b2, "pair sites must share a sublattice");
+      pair_addr.push_back(make_int2(pad_addr(b, i, j, k), pad_addr(b2, i2, j2, k2)));
+      pair_sub.push_back(b);
+      images(ls, img);
+      for (int a : img) pair_halo.push_back(make_int2(pad_addr(b, i, j, k), a));
+      images(lt, img);
+      for (int a : img) pair_halo.push_back(make_int2(pad_addr(b2, i2, j2, k2), a));
+    }
+    pair_begin.push_back((int)pair_addr.size());
+    pair_halo_begin.push_back((int)pair_halo.size());
+  }
+  d.n_classes = sc->n_classes;
+  d.max_class = max_class;
+  d.n_pair_phases = sc->n_pair_phases;
+  auto &A = hs.allocs;
+  HIP_TRY(upload(h, A, cb.data(), cb.size(), (int **)&d.class_begin));
+  HIP_TRY(upload(h, A, class_sub.data(), class_sub.size(), (int **)&d.class_sublat));
+  HIP_TRY(upload(h, A, class_addr.data(), class_addr.size(), (int **)&d.class_addr));
+  HIP_TRY(upload(h, A, class_lin.data(), class_lin.size(), (int **)&d.class_lin));
+  HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
+  HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
+  HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
+  HIP_TRY(upload(h, A, pair_begin.data(), pair_begin.size(), (int **)&d.pair_begin));
+  HIP_TRY(upload(h, A, pair_addr.data(), pair_addr.size(), (int2 **)&d.pair_addr));
+  HIP_TRY(upload(h, A, pair_sub.data(), pair_sub.size(), (int **)&d.pair_sublat));
+  HIP_TRY(upload(h, A, pair_halo_begin.data(), pair_halo_begin.size(), (int **)&d.pair_halo_begin));
+  HIP_TRY(upload(h, A, pair_halo.data(), pair_halo.size(), (int2 **)&d.pair_halo));
+  h->scs.push_back(std::move(hs));
+  h->scs_dirty = true;
+  *id_out = (int)h->scs.size() - 1;
+  return CLEXMC_OK;
+}
+
+static clexmc_status sync_supercells(clexmc_handle *h) {
+  if (!h->scs_dirty) return CLEXMC_OK;
+  if (h->d_scs) hipFree(h->d_scs);
+  h->d_scs = nullptr;
+  std::vector<DevSupercell> v;
+  for (auto &s : h->scs) v.push_back(s.dev);
+  HIP_TRY(hipMalloc(&h->d_scs, sizeof(DevSupercell) * std::max<size_t>(1, v.size())));
+  HIP_TRY(hipMemcpy(h->d_scs, v.data(), sizeof(DevSupercell) * v.size(), hipMemcpyHostToDevice));
+  h->scs_dirty = false;
+  return CLEXMC_OK;
+}
+
+static void free_chains(clexmc_handle *h) {
+  hipFree(h->d_chains);
+  hipFree(h->d_occ);
+  hipFree(h->d_binC);
+  hipFree(h->d_sampE);
+  hipFree(h->d_sampN);
+  hipFree(h->d_sample_count);
+  h->d_chains = nullptr;
+  h->d_occ = nullptr;
+  h->d_binC = nullptr;
+  h->d_sampE = nullptr;
+  h->d_sampN = nullptr;
+  h->d_sample_count = nullptr;
+  h->sample_cap = 0;
+  h->n_chains = 0;
+}
+
+static clexmc_status run_corr(clexmc_handle *h, int chain, double *out_host);
+
+clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supercell_id, const int8_t *occ,
+                                 const double *temperature, const double *chem, const uint64_t *seeds,
+                                 double kb) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_chains >= 1 && supercell_id && occ && temperature && seeds, "bad load_chains arguments");
+  ARG_CHECK(kb > 0.0, "kb must be positive");
+  HIP_TRY(hipSetDevice(h->device));
+  clexmc_status st = sync_supercells(h);
+  if (st != CLEXMC_OK) return st;
+  free_chains(h);
+  std::vector<DevChain> chains(n_chains);
+  h->chain_sc.assign(n_chains, 0);
+  h->chain_off.assign(n_chains, 0);
+  long long off = 0;
+  for (int c = 0; c < n_chains; ++c) {
+    const int id = supercell_id[c];
+    ARG_CHECK(id >= 0 && id < (int)h->scs.size(), "bad supercell id");
+    const DevSupercell &d = h->scs[id].dev;
+    DevChain &ch = chains[c];
+    memset(&ch, 0, sizeof(ch));
+    ch.sc = id;
+    ch.occ_offset = (int)off;
+    h->chain_sc[c] = id;
+    h->chain_off[c] = (int)off;
+    ARG_CHECK(temperature[c] >= 0.0, "negative temperature");
+    const double beta = temperature[c] > 0.0 ? 1.0 / (kb * temperature[c]) : std::numeric_limits<double>::infinity();
+    ch.beta64 = beta;
+    ch.beta = std::isfinite(beta) ? (float)beta : std::numeric_limits<float>::infinity();
+    for (int b = 0; b < h->n_sublat; ++b)
+      for (int o = 0; o < kMaxOcc; ++o)
+        ch.chem[b * kMaxOcc + o] =
+            (chem && o < h->n_occ_max) ? (float)chem[(long long)c * h->n_sublat * h->n_occ_max + b * h->n_occ_max + o] : 0.f;
+    ch.key0 = (uint32_t)(seeds[c] & 0xffffffffu);
+    ch.key1 = (uint32_t)(seeds[c] >> 32);
+    // validate occupation and count occupants on variable sites
+    for (int l = 0; l < d.n_sites; ++l) {
+      const int b = l / d.n_cells;
+      const int v = occ[off + l];
+      ARG_CHECK(v >= 0 && v < h->n_occ[b], "occupant index out of range");
+      if (h->n_occ[b] >= 2) ch.counts[v] += 1;
+    }
+    off += d.n_sites;
+  }
+  ARG_CHECK(off < (1LL << 31), "total occupation too large");
+  h->occ_total = off;
+  HIP_TRY(hipMalloc(&h->d_chains, sizeof(DevChain) * n_chains));
+  HIP_TRY(hipMalloc(&h->d_occ, (size_t)off));
+  HIP_TRY(hipMemcpy(h->d_occ, occ, (size_t)off, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&h->d_binC, sizeof(long long) * (size_t)n_chains * h->n_orbits));
+  HIP_TRY(hipMalloc(&h->d_sample_count, sizeof(int) * n_chains));
+  HIP_TRY(hipMemset(h->d_sample_count, 0, sizeof(int) * n_chains));
+  HIP_TRY(hipMemcpy(h->d_chains, chains.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+  h->n_chains = n_chains;
+  // initial energies (and exact binary counts) from full correlations
+  std::vector<double> corr(h->n_corr);
+  std::vector<long long> C(h->n_orbits);
+  for (int c = 0; c < n_chains; ++c) {
+    st = run_corr(h, c, corr.data());
+    if (st != CLEXMC_OK) return st;
+    double E = 0.0;
+    for (int i = 0; i < h->n_corr; ++i) E += h->eci[i] * corr[i];
+    chains[c].E = E;
+    if (h->binary) {
+      for (int a = 0; a < h->n_orbits; ++a) C[a] = llround(corr[h->orbit_func_begin[a]] * h->orbit_mult[a]);
+      HIP_TRY(hipMemcpy(h->d_binC + (size_t)c * h->n_orbits, C.data(), sizeof(long long) * h->n_orbits,
+                        hipMemcpyHostToDevice));
+      double Eb = 0.0;
+      for (int a = 0; a < h->n_orbits; ++a) Eb += h->dsys.bin_w64[a] * (double)C[a];
+      chains[c].E = Eb;
+    }
+  }
+  HIP_TRY(hipMemcpy(h->d_chains, chains.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+  return CLEXMC_OK;
+}
+
+static clexmc_status run_corr(clexmc_handle *h, int chain, double *out_host) {
+  double *d_out = nullptr;
+  HIP_TRY(hipMalloc(&d_out, sizeof(double) * h->n_corr));
+  CorrArgs a;
+  a.sys = h->d_sys;
+  a.scs = h->d_scs;
+  a.chains = h->d_chains;
+  a.occ = h->d_occ;
+  a.chain = chain;
+  a.out = d_out;
+  hipError_t e = launch_corr(a, h->n_corr, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_host, d_out, sizeof(double) * h->n_corr, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d_out);
+  if (e != hipSuccess) {
+    h->err = std::string("correlations: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out) {
+  if (!h || !corr_out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(chain >= 0 && chain < h->n_chains, "chain out of range");
+  HIP_TRY(hipSetDevice(h->device));
+  return run_corr(h, chain, corr_out);
+}
+
+clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
+                                        const int32_t *new_occ, int sites_per_prop, int include_chem,
+                                        float *dE_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_prop >= 0, "n_prop < 0");
+  if (n_prop == 0) return CLEXMC_OK;
+  ARG_CHECK(chain && sites && new_occ && dE_out, "null argument");
+  ARG_CHECK(sites_per_prop >= 1 && sites_per_prop <= kMaxSitesPerProp, "sites_per_prop out of range (1..4)");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  for (int p = 0; p < n_prop; ++p) {
+    ARG_CHECK(chain[p] >= 0 && chain[p] < h->n_chains, "proposal chain out of range");
+    const DevSupercell &d = h->scs[h->chain_sc[chain[p]]].dev;
+    for (int i = 0; i < sites_per_prop; ++i) {
+      const int l = sites[p * sites_per_prop + i];
+      ARG_CHECK(l >= 0 && l < d.n_sites, "proposal site out of range");
+      const int b = l / d.n_cells;
+      const int v = new_occ[p * sites_per_prop + i];
+      ARG_CHECK(v >= 0 && v < h->n_occ[b], "proposal occupant out of range");
+    }
+  }
+  const size_t ns = (size_t)n_prop * sites_per_prop;
+  int *d_chain = nullptr, *d_sites = nullptr, *d_new = nullptr;
+  float *d_dE = nullptr;
+  HIP_TRY(hipMalloc(&d_chain, sizeof(int) * n_prop));
+  HIP_TRY(hipMalloc(&d_sites, sizeof(int) * ns));
+  HIP_TRY(hipMalloc(&d_new, sizeof(int) * ns));
+  HIP_TRY(hipMalloc(&d_dE, sizeof(float) * n_prop));
+  hipError_t e = hipMemcpyAsync(d_chain, chain, sizeof(int) * n_prop, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sites, sites, sizeof(int) * ns, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_new, new_occ, sizeof(int) * ns, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    DeltaArgs a;
+    a.sys = h->d_sys;
+    a.scs = h->d_scs;
+    a.chains = h->d_chains;
+    a.occ = h->d_occ;
+    a.n_prop = n_prop;
+    a.sites_per_prop = sites_per_prop;
+    a.include_chem = include_chem;
+    a.chain = d_chain;
+    a.sites = d_sites;
+    a.new_occ = d_new;
+    a.dE = d_dE;
+    e = launch_delta_batch(a, h->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(dE_out, d_dE, sizeof(float) * n_prop, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d_chain);
+  hipFree(d_sites);
+  hipFree(d_new);
+  hipFree(d_dE);
+  if (e != hipSuccess) {
+    h->err = std::string("delta_energy_batch: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(threads == 0 || (threads >= 64 && threads <= 512 && threads % 64 == 0),
+            "block size must be 0 or a multiple of 64 in [64, 512]");
+  h->block_size = threads;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
+  ARG_CHECK(n_passes >= 0 && sample_period >= 1, "bad n_passes / sample_period");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  if (n_passes == 0) return CLEXMC_OK;
+  HIP_TRY(hipSetDevice(h->device));
+  int max_pad = 0, max_class = 1;
+  for (int c = 0; c < h->n_chains; ++c) {
+    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
+    max_pad = std::max(max_pad, d.pad_bytes);
+    max_class = std::max(max_class, d.max_class);
+  }
+  int threads = h->block_size;
+  if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+  size_t lds = metropolis_lds_bytes(max_pad, h->n_orbits, h->binary != 0, threads, max_class);
+  int max_lds = 0;
+  HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  if ((long long)lds > (long long)max_lds - 64) {
+    h->err = "chain image does not fit in LDS (" + std::to_string(lds) + " > " + std::to_string(max_lds) + " bytes)";
+    return CLEXMC_ERR_UNSUPPORTED;
+  }
+  // sample capacity (append)
+  std::vector<int> cnt(h->n_chains);
+  HIP_TRY(hipMemcpy(cnt.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
+  int used = 0;
+  for (int v : cnt) used = std::max(used, v);
+  const int need = used + n_passes / sample_period + 2;
+  if (need > h->sample_cap) {
+    const int cap = std::max(need, 2 * h->sample_cap);
+    double *nE = nullptr;
+    int *nN = nullptr;
+    HIP_TRY(hipMalloc(&nE, sizeof(double) * (size_t)h->n_chains * cap));
+    HIP_TRY(hipMalloc(&nN, sizeof(int) * (size_t)h->n_chains * cap * kMaxOcc));
+    if (h->sample_cap > 0) {
+      HIP_TRY(hipMemcpy2D(nE, sizeof(double) * cap, h->d_sampE, sizeof(double) * h->sample_cap,
+                          sizeof(double) * h->sample_cap, h->n_chains, hipMemcpyDeviceToDevice));
+      HIP_TRY(hipMemcpy2D(nN, sizeof(int) * kMaxOcc * cap, h->d_sampN, sizeof(int) * kMaxOcc * h->sample_cap,
+                          sizeof(int) * kMaxOcc * h->sample_cap, h->n_chains, hipMemcpyDeviceToDevice));
+    }
+    hipFree(h->d_sampE);
+    hipFree(h->d_sampN);
+    h->d_sampE = nE;
+    h->d_sampN = nN;
+    h->sample_cap = cap;
+  }
+  RunArgs a;
+  a.sys = h->d_sys;
+  a.scs = h->d_scs;
+  a.chains = h->d_chains;
+  a.occ = h->d_occ;
+  a.binC = h->d_binC;
+  a.sampE = h->d_sampE;
+  a.sampN = h->d_sampN;
+  a.sample_count = h->d_sample_count;
+  a.sample_cap = h->sample_cap;
+  a.n_passes = n_passes;
+  a.sample_period = sample_period;
+  a.rounds = std::max(1, canonical_rounds);
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms) {
+  if (!h || !ms) return CLEXMC_ERR_ARG;
+  *ms = h->last_ms;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_read_samples(clexmc_handle *h, int *n_samples, double *E, int32_t *counts) {
+  if (!h || !n_samples) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  std::vector<int> cnt(h->n_chains);
+  HIP_TRY(hipMemcpy(cnt.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
+  int ns = 0;
+  for (int v : cnt) ns = std::max(ns, v);
+  const int want = *n_samples;
+  *n_samples = ns;
+  if (!E && !counts) return CLEXMC_OK;
+  ARG_CHECK(want >= ns, "output arrays too small for the recorded samples");
+  if (ns == 0) return CLEXMC_OK;
+  if (E)
+    HIP_TRY(hipMemcpy2D(E, sizeof(double) * ns, h->d_sampE, sizeof(double) * h->sample_cap, sizeof(double) * ns,
+                        h->n_chains, hipMemcpyDeviceToHost));
+  if (counts)
+    HIP_TRY(hipMemcpy2D(counts, sizeof(int) * kMaxOcc * ns, h->d_sampN, sizeof(int) * kMaxOcc * h->sample_cap,
+                        sizeof(int) * kMaxOcc * ns, h->n_chains, hipMemcpyDeviceToHost));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_clear_samples(clexmc_handle *h) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemset(h->d_sample_count, 0, sizeof(int) * h->n_chains));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_read_state(clexmc_handle *h, int chain, int8_t *occ, double *E_running, long long *n_attempt,
+                                long long *n_accept) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(chain >= 0 && chain < h->n_chains, "chain out of range");
+  HIP_TRY(hipSetDevice(h->device));
+  DevChain ch;
+  HIP_TRY(hipMemcpy(&ch, h->d_chains + chain, sizeof(DevChain), hipMemcpyDeviceToHost));
+  const DevSupercell &d = h->scs[h->chain_sc[chain]].dev;
+  if (occ) HIP_TRY(hipMemcpy(occ, h->d_occ + h->chain_off[chain], (size_t)d.n_sites, hipMemcpyDeviceToHost));
+  if (E_running) *E_running = ch.E;
+  if (n_attempt) *n_attempt = ch.n_attempt;
+  if (n_accept) *n_accept = ch.n_accept;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_read_binary_counts(clexmc_handle *h, int chain, long long *counts_out) {
+  if (!h || !counts_out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->binary, "binary path not active");
+  ARG_CHECK(chain >= 0 && chain < h->n_chains, "chain out of range");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpy(counts_out, h->d_binC + (size_t)chain * h->n_orbits, sizeof(long long) * h->n_orbits,
+                    hipMemcpyDeviceToHost));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_philox_device(clexmc_handle *h, int n, const uint32_t *ctr, const uint32_t *key,
+                                   uint32_t *out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n >= 1 && ctr && key && out, "bad philox arguments");
+  HIP_TRY(hipSetDevice(h->device));
+  uint4 *dc = nullptr, *dout = nullptr;
+  uint2 *dk = nullptr;
+  HIP_TRY(hipMalloc(&dc, sizeof(uint4) * n));
+  HIP_TRY(hipMalloc(&dk, sizeof(uint2) * n));
+  HIP_TRY(hipMalloc(&dout, sizeof(uint4) * n));
+  hipError_t e = hipMemcpy(dc, ctr, sizeof(uint4) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dk, key, sizeof(uint2) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_philox(n, dc, dk, dout, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(uint4) * n, hipMemcpyDeviceToHost);
+  hipFree(dc);
+  hipFree(dk);
+  hipFree(dout);
+  if (e != hipSuccess) {
+    h->err = std::string("philox: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  return CLEXMC_OK;
+}
+
+void clexmc_destroy(clexmc_handle *h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  free_chains(h);
+  for (auto &s : h->scs) free_all(s.allocs);
+  free_all(h->sys_allocs);
+  if (h->d_scs) hipFree(h->d_scs);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"

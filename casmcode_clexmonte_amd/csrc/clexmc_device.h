@@ -1,0 +1,120 @@
+// Device-side data structures and helpers shared by the clexmc kernels.
+//
+// Layout in device memory (HBM):
+//  * per chain: int8 occupation (N sites, linear index l = b*n_cells + cell);
+//  * per supercell: padded image geometry, independent-set classes as padded
+//    LDS addresses, halo copy lists, per-entry padded neighbour offsets;
+//  * system: flower tables (FP32 for the kernels, FP64 for correlations).
+// In LDS (one workgroup per chain): the padded occupation image
+//   [b][k + h][j + h][i + h] (int8), so every neighbour of every site is a
+//   constant offset away; h is the largest cell offset of any cluster.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clexmc {
+
+constexpr int kMaxSublat = 8;
+constexpr int kMaxOcc = 4;
+constexpr int kMaxFunc = 3;
+constexpr int kMaxOrbits = 16;   // binary integer path: orbits held in registers
+constexpr int kMaxSitesPerProp = 4;
+
+struct DevSystem {
+  int n_sublat, n_occ_max, n_func_max, n_orbits, n_corr, binary;
+  int n_occ[kMaxSublat];
+  int n_func[kMaxSublat];
+  float site_func[kMaxSublat][kMaxFunc][kMaxOcc];
+  // flower (general table path)
+  const int *flower_begin;      // [n_sublat+1]
+  const int *entry_k;           // [n_entries]
+  const int *entry_site_begin;  // [n_entries+1]
+  const int4 *entry_sites;      // [n_entry_sites] (b, di, dj, dk)
+  const int *entry_mul;         // [n_entry_sites] radix multiplier of each other site
+  const int *entry_table;       // [n_entries] value offset into flower_tables
+  const float *flower_tables;
+  // binary path: per (sublattice, orbit) entry range
+  const int2 *bin_range;        // [n_sublat][n_orbits] (entry_begin, entry_end)
+  float bin_w[kMaxOrbits];      // J_a / m_a
+  double bin_w64[kMaxOrbits];
+  int orbit_k[kMaxOrbits];
+  // correlations
+  const int *orbit_k_all;       // [n_orbits]
+  const int *orbit_mult;
+  const int *orbit_nfunc;
+  const int *orbit_func_begin;
+  const int *corr_orbit_begin;  // [n_orbits+1]
+  const int *corr_elem_begin;   // [n_orbits+1]
+  const int4 *corr_elem_sites;
+  const int *orbit_table_offset;
+  const double *orbit_tables;
+  const int *func_orbit;        // [n_corr] orbit of each function
+};
+
+struct DevSupercell {
+  int L1, L2, L3, h;
+  int P1, P2, P3, S;      // padded dims, S = P1*P2*P3 bytes per sublattice
+  int n_cells, n_sites, pad_bytes;
+  int n_classes;
+  const int *class_begin;  // [n_classes+1]
+  const int *class_sublat; // [n_classes]
+  const int *class_addr;   // padded LDS address of each class site
+  const int *class_lin;    // linear index of each class site
+  const int *halo_begin;   // [n_classes+1]
+  const int2 *halo;        // (src, dst) padded addresses
+  const int *entry_off;    // [n_entry_sites] padded relative offset
+  int n_pair_phases;
+  const int *pair_begin;   // [n_pair_phases+1]
+  const int2 *pair_addr;   // padded addresses of both sites
+  const int *pair_sublat;  // [n_pairs] sublattice of both (equal)
+  const int *pair_halo_begin; // [n_pair_phases+1]
+  const int2 *pair_halo;
+  int max_class;           // largest class size
+};
+
+struct DevChain {
+  int sc;
+  int occ_offset;          // into the occupation buffer
+  float beta;
+  int pad0;
+  double beta64;
+  float chem[kMaxSublat * kMaxOcc];
+  uint32_t key0, key1;
+  unsigned long long draw_base;
+  unsigned long long pass_base;
+  double E;                // running formation energy per supercell (FP64)
+  long long n_attempt, n_accept;
+  long long counts[kMaxOcc];
+  // FP64 observable accumulators over recorded samples
+  double sumE, sumE2, sumN1, sumN12, sumEN1;
+  long long n_samples_acc;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with the
+// published multipliers and Weyl key increments.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t hi0 = __umulhi(m0, c.x), lo0 = m0 * c.x;
+    const uint32_t hi1 = __umulhi(m1, c.z), lo1 = m1 * c.z;
+#else
+    const uint64_t p0 = (uint64_t)m0 * c.x, p1 = (uint64_t)m1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+#endif
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 24 random bits -> float in [0,1)
+__device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+}  // namespace clexmc

@@ -1,0 +1,86 @@
+"""Synthetic cluster expansions and chain batches for the measured workloads.
+
+* FCC binary (A, B), 1 site per primitive cell, Chebyshev basis, clusters:
+  empty, point, pairs to the 4th neighbour shell, triplets with every edge
+  within the 2nd shell (8 correlation functions).  ECIs are drawn
+  ``N(0, 0.05 eV) / (1 + L)`` with ``L`` the cluster's maximum length in
+  nearest-neighbour units, from a recorded seed.
+* ``fcc_canonical_12``:  12x12x12 supercell (1,728 sites), canonical swaps at
+  x_B = 0.5, T = 600 K, one chain, 1,000 + 5,000 passes.
+* ``fcc_sgc_24_grid``:   24x24x24 supercell (13,824 sites), semi-grand
+  canonical flips, 256 chains on a 16 T (200..1500 K) x 16 mu (-0.3..0.3 eV)
+  grid, 2,000 passes.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+
+from .system import (BasisSet, ClusterExpansion, FlatCE, Prim, Supercell, fcc_prim, KB)
+
+FCC_ECI_SEED = 20240611
+FCC_NN = 1.0 / math.sqrt(2.0)
+
+
+def fcc_binary_basis() -> BasisSet:
+    prim = fcc_prim(1.0, ("A", "B"))
+    # 4th-neighbour pair shell at sqrt(2) a, 5th at sqrt(5/2) a; 2nd shell at a
+    return BasisSet(prim, {2: 1.5, 3: 1.05}, "chebychev")
+
+
+def seeded_eci(bs: BasisSet, seed: int, sigma: float = 0.05, nn: float = FCC_NN) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    eci = np.zeros(bs.n_functions)
+    for orb in bs.orbits:
+        L = orb.max_length / nn
+        for fi in orb.functions:
+            eci[fi] = rng.normal(0.0, sigma) / (1.0 + L)
+    return eci
+
+
+def fcc_binary_ce(seed: int = FCC_ECI_SEED) -> ClusterExpansion:
+    bs = fcc_binary_basis()
+    return ClusterExpansion(bs, seeded_eci(bs, seed))
+
+
+def random_occupation(n_sites: int, x_b: float, seed: int, exact: bool = True) -> np.ndarray:
+    """Binary occupation with a fraction x_b of B (occupant index 1);
+    exact=True shuffles exactly round(x_b*N) B atoms (Fisher-Yates)."""
+    rng = np.random.default_rng(seed)
+    if exact:
+        occ = np.zeros(n_sites, dtype=np.int8)
+        occ[: int(round(x_b * n_sites))] = 1
+        rng.shuffle(occ)
+        return occ
+    return (rng.random(n_sites) < x_b).astype(np.int8)
+
+
+@dataclass
+class ChainBatch:
+    temperatures: np.ndarray
+    chem: np.ndarray        # [n_chains][n_sublat * n_occ_max]
+    occ: List[np.ndarray]
+    seeds: np.ndarray
+    mu: np.ndarray
+
+
+def fcc_sgc_grid(sc: Supercell, n_T: int = 16, n_mu: int = 16, T_range=(200.0, 1500.0),
+                 mu_range=(-0.3, 0.3), seed: int = 7) -> ChainBatch:
+    Ts = np.linspace(T_range[0], T_range[1], n_T)
+    mus = np.linspace(mu_range[0], mu_range[1], n_mu)
+    temps, mu_list, chem, occs, seeds = [], [], [], [], []
+    c = 0
+    for T in Ts:
+        for mu in mus:
+            temps.append(T)
+            mu_list.append(mu)
+            # Omega = E - mu * N_B  ->  weight of occupant B is mu
+            chem.append([0.0, mu])
+            occs.append(random_occupation(sc.n_sites, 0.5, seed * 100003 + c, exact=False))
+            seeds.append((seed << 32) + c)
+            c += 1
+    return ChainBatch(np.array(temps), np.array(chem), occs, np.array(seeds, dtype=np.uint64),
+                      np.array(mu_list))

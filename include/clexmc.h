@@ -1,0 +1,192 @@
+/* clexmc: C ABI of the MI355X (gfx950) Metropolis hot path for cluster-expansion
+ * Monte Carlo.
+ *
+ * The entry points replace the per-step arithmetic the reference calls through
+ * C++ objects.  Each declaration names the reference interface it stands in
+ * for (paths relative to the reference repository root):
+ *
+ *  - clexmc_create / clexmc_add_supercell: the System + SupercellSystemData
+ *    construction that builds clusters, neighbour lists and ECIs
+ *    (src/casm/clexmonte/system/System.cc:41-72, get_clex :633-641,
+ *    StateData::StateData src/casm/clexmonte/monte_calculator/StateData.cc:9-79);
+ *  - clexmc_load_chains: MonteCalculator::set_state_and_potential + the
+ *    OccLocation initialisation of one or many states
+ *    (include/casm/clexmonte/monte_calculator/MonteCalculator.hh:127-135,161-186);
+ *  - clexmc_delta_energy_batch: BaseMontePotential::occ_delta_per_supercell
+ *    (include/casm/clexmonte/monte_calculator/BaseMonteCalculator.hh:38-43;
+ *    CanonicalCalculator.cc:137-140, SemiGrandCanonicalCalculator.cc:186-213);
+ *  - clexmc_correlations / clexmc_energy: Correlations::per_supercell and
+ *    ClusterExpansion::per_supercell used by the "corr.<key>" and
+ *    "clex.<key>" samplers (src/casm/clexmonte/monte_calculator/
+ *    sampling_functions.cc:121-151);
+ *  - clexmc_run_passes: the Metropolis main loop occupation_metropolis_v2
+ *    (include/casm/clexmonte/methods/occupation_metropolis.hh:72-123) driven
+ *    by CanonicalCalculator::run (CanonicalCalculator.cc:413-462) and
+ *    SemiGrandCanonicalCalculator::run (SemiGrandCanonicalCalculator.cc:423-466),
+ *    and, for many chains at once, the experimental multi-state run
+ *    (BaseMonteCalculator.hh:245-247);
+ *  - clexmc_read_samples / clexmc_read_state: the RunManager sample storage
+ *    read by the analysis functions (analysis_functions.cc:17-173).
+ *
+ * Conventions: plain C types only; no exceptions cross this boundary; every
+ * call returns a clexmc_status and a message is kept for
+ * clexmc_last_error().  A handle owns its device memory and one HIP stream;
+ * it is thread-compatible (one handle per host thread), not thread-safe.
+ * Linear site index l = b * n_cells + i + L1 * (j + L2 * k).
+ */
+#ifndef CLEXMC_H
+#define CLEXMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CLEXMC_OK = 0,
+  CLEXMC_ERR_ARG = 1,
+  CLEXMC_ERR_HIP = 2,
+  CLEXMC_ERR_UNSUPPORTED = 3,
+  CLEXMC_ERR_STATE = 4,
+  CLEXMC_ERR_NO_DEVICE = 5
+} clexmc_status;
+
+typedef struct clexmc_handle clexmc_handle;
+
+/* Flattened cluster expansion.  Produced by the host system builder
+ * (casmcode_clexmonte_amd/system.py FlatCE); all arrays are copied. */
+typedef struct {
+  int n_sublat;
+  const int *n_occ;            /* [n_sublat] */
+  int n_occ_max;               /* <= 4 */
+  int n_func_max;              /* site functions per site, <= 3 */
+  const double *site_func;     /* [n_sublat][n_func_max][n_occ_max] */
+  /* correlations: orbit element lists and orbit tables */
+  int n_orbits;
+  const int *orbit_k;
+  const int *orbit_mult;
+  const int *orbit_nfunc;
+  const int *orbit_func_begin;
+  const int *corr_orbit_begin;   /* [n_orbits+1] */
+  const int *corr_elem_begin;    /* [n_orbits+1] */
+  const int *corr_elem_sites;    /* [n_elem_sites][4] b,i,j,k */
+  int n_corr_elem_sites;
+  const int *orbit_table_offset; /* [n_orbits] */
+  const double *orbit_tables;
+  int n_orbit_tables;            /* doubles */
+  int n_corr;
+  const double *eci;             /* [n_corr] */
+  /* flower: cluster instances containing a site of sublattice b */
+  const int *flower_begin;       /* [n_sublat+1] */
+  int n_entries;
+  const int *entry_orbit;        /* [n_entries] */
+  const int *entry_k;            /* other sites per entry */
+  const int *entry_site_begin;   /* [n_entries+1] */
+  const int *entry_sites;        /* [n_other_sites][4] b,di,dj,dk */
+  int n_entry_sites;
+  const int *entry_table;        /* [n_entries] offset (values) into flower_tables */
+  const double *flower_tables;   /* V[tau_others][f] */
+  int n_flower_tables;
+  /* binary (+-1 Chebyshev) integer path; binary == 0 disables it */
+  int binary;
+  const int *bin_seg_begin;      /* [n_sublat+1] */
+  const int *bin_segs;           /* [n_segs][3]: orbit, entry_begin, entry_end */
+  int n_bin_segs;
+  const double *bin_orbit_weight;/* [n_orbits] J_a / m_a */
+} clexmc_system_desc;
+
+/* Supercell (diagonal, L1 x L2 x L3 primitive cells) with its independent-set
+ * partition: sites of one class share no cluster. */
+typedef struct {
+  int L[3];
+  int n_classes;
+  const int *class_begin;   /* [n_classes+1] */
+  const int *class_sites;   /* linear site indices */
+  /* canonical cross-class mixing: phases of disjoint, mutually
+   * non-interacting site pairs (optional; n_pair_phases may be 0) */
+  int n_pair_phases;
+  const int *pair_phase_begin; /* [n_pair_phases+1] in pairs */
+  const int *pair_sites;       /* [n_pairs][2] */
+} clexmc_supercell_desc;
+
+/* Creates a handle on HIP device `device`.  Replaces System construction
+ * (System.cc:41-72).  Fails with CLEXMC_ERR_NO_DEVICE without a GPU. */
+clexmc_status clexmc_create(const clexmc_system_desc *sys, int device, clexmc_handle **out);
+
+/* Registers a supercell; returns its id.  (SupercellSystemData, System.cc:62-72) */
+clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc *sc, int *id_out);
+
+/* Loads n_chains states (replaces set_state_and_potential + OccLocation
+ * initialisation, MonteCalculator.hh:127-135,161-186).
+ * occ: concatenated per-chain occupations (int8 occupant index per site).
+ * temperature [n_chains] (K); chem [n_chains][n_sublat*n_occ_max] per
+ * site-occupant potential weights such that dOmega = dE - (chem[new]-chem[old])
+ * (SemiGrandCanonicalCalculator.cc:186-213 with exchange potentials folded
+ * in), or NULL; seeds [n_chains] Philox keys.  kb: Boltzmann constant (eV/K). */
+clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supercell_id,
+                                 const int8_t *occ, const double *temperature, const double *chem,
+                                 const uint64_t *seeds, double kb);
+
+/* Per-proposal energy change (per supercell, eV) evaluated on the GPU in FP32
+ * with sequential application of the sites of a proposal (a swap is two
+ * changes, the second seeing the first), at each chain's current occupation.
+ * Replaces BaseMontePotential::occ_delta_per_supercell
+ * (BaseMonteCalculator.hh:38-43; ClusterExpansion::occ_delta_value call site
+ * CanonicalCalculator.cc:137-140).  Host arrays: chain[n_prop],
+ * sites[n_prop][sites_per_prop], new_occ[n_prop][sites_per_prop];
+ * sites_per_prop <= 4.  If include_chem != 0 the chain's chem weights are
+ * subtracted (semi-grand canonical potential). */
+clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int32_t *chain,
+                                        const int32_t *sites, const int32_t *new_occ,
+                                        int sites_per_prop, int include_chem, float *dE_out);
+
+/* Full per-supercell correlations of one chain in FP64 (Correlations::
+ * per_supercell, sampling_functions.cc:121-135).  corr_out[n_corr]. */
+clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out);
+
+/* Runs n_passes Metropolis passes on every loaded chain in one persistent
+ * launch (occupation_metropolis.hh:72-123).  kind: 0 canonical swaps,
+ * 1 semi-grand canonical changes.  A sample (formation energy per
+ * supercell, occupant counts) is recorded every sample_period passes.
+ * canonical_rounds: random matchings per independent set and pass
+ * (canonical only; >= 1). */
+clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period,
+                                int canonical_rounds);
+
+/* Per-chain totals of the last run and the samples recorded so far.
+ * E [n_chains][*n_samples] (formation energy per supercell),
+ * counts [n_chains][*n_samples][n_occ_max]; pass NULL to query *n_samples. */
+clexmc_status clexmc_read_samples(clexmc_handle *h, int *n_samples, double *E, int32_t *counts);
+clexmc_status clexmc_clear_samples(clexmc_handle *h);
+
+/* Current occupation (N sites of the chain's supercell), running formation
+ * energy (FP64) and attempt / accept counters of one chain. */
+clexmc_status clexmc_read_state(clexmc_handle *h, int chain, int8_t *occ, double *E_running,
+                                long long *n_attempt, long long *n_accept);
+
+/* Exact integer cluster-product sums C_a = m_a * corr_a of the binary path
+ * (tracked incrementally, int64 [n_orbits]). */
+clexmc_status clexmc_read_binary_counts(clexmc_handle *h, int chain, long long *counts_out);
+
+/* Kernel timing of the last clexmc_run_passes (HIP events on the handle's
+ * stream, milliseconds). */
+clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
+
+/* Tuning knobs (block size of the Metropolis kernel; 0 = automatic). */
+clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
+
+/* Philox4x32-10 evaluated on the device (known-answer checks). */
+clexmc_status clexmc_philox_device(clexmc_handle *h, int n, const uint32_t *ctr /*[n][4]*/,
+                                   const uint32_t *key /*[n][2]*/, uint32_t *out /*[n][4]*/);
+
+const char *clexmc_last_error(clexmc_handle *h);
+void clexmc_destroy(clexmc_handle *h);
+
+/* Library identification (compile-time target and version). */
+const char *clexmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLEXMC_H */
