@@ -33,7 +33,10 @@ struct clexmc_handle {
   // system (host copies)
   int n_sublat = 0, n_occ_max = 0, n_func_max = 0, n_orbits = 0, n_corr = 0, binary = 0;
   std::vector<int> n_occ, orbit_k, orbit_mult, orbit_nfunc, orbit_func_begin;
-  std::vector<int> flower_begin, entry_k, entry_site_begin, entry_sites, entry_table;
+  std::vector<int> flower_begin, entry_k, entry_site_begin, entry_sites, entry_table, entry_orbit;
+  std::vector<int> entry_mul;
+  std::vector<double> bin_w;
+  int n_flower_tables = 0;
   std::vector<int> entry_home;  // home sublattice of each entry
   std::vector<double> eci;
   int halo = 0;
@@ -48,7 +51,8 @@ struct clexmc_handle {
   int n_chains = 0;
   std::vector<int> chain_sc, chain_off;
   long long occ_total = 0;
-  DevChain *d_chains = nullptr;
+  ChainParams *d_params = nullptr;
+  ChainState *d_state = nullptr;
   int8_t *d_occ = nullptr;
   long long *d_binC = nullptr;
   double *d_sampE = nullptr;
@@ -130,7 +134,12 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   h->n_func_max = s->n_func_max;
   h->n_orbits = s->n_orbits;
   h->n_corr = s->n_corr;
-  h->binary = s->binary && s->n_orbits <= kMaxOrbits;
+  {
+    int n_slots = 0;
+    for (int a = 0; a < s->n_orbits; ++a)
+      if (s->orbit_k[a] >= 1) n_slots++;
+    h->binary = s->binary && s->n_orbits <= kMaxOrbits && n_slots <= kMaxSlots;
+  }
   h->n_occ.assign(s->n_occ, s->n_occ + s->n_sublat);
   h->orbit_k.assign(s->orbit_k, s->orbit_k + s->n_orbits);
   h->orbit_mult.assign(s->orbit_mult, s->orbit_mult + s->n_orbits);
@@ -141,6 +150,8 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   h->entry_site_begin.assign(s->entry_site_begin, s->entry_site_begin + s->n_entries + 1);
   h->entry_sites.assign(s->entry_sites, s->entry_sites + 4 * s->n_entry_sites);
   h->entry_table.assign(s->entry_table, s->entry_table + s->n_entries);
+  h->entry_orbit.assign(s->entry_orbit, s->entry_orbit + s->n_entries);
+  h->n_flower_tables = s->n_flower_tables;
   h->eci.assign(s->eci, s->eci + s->n_corr);
   ARG_CHECK(h->flower_begin[s->n_sublat] == s->n_entries, "flower_begin inconsistent with n_entries");
   ARG_CHECK(h->entry_site_begin[s->n_entries] == s->n_entry_sites, "entry_site_begin inconsistent");
@@ -166,7 +177,8 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   }
   // home sublattice and radix multipliers of entry sites; halo width
   h->entry_home.assign(s->n_entries, 0);
-  std::vector<int> entry_mul(std::max(1, s->n_entry_sites), 1);
+  std::vector<int> &entry_mul = h->entry_mul;
+  entry_mul.assign(std::max(1, s->n_entry_sites), 1);
   int halo = 0;
   for (int b = 0; b < s->n_sublat; ++b)
     for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
@@ -192,23 +204,16 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   HIP_TRY(upload(h, A, entry_mul.data(), entry_mul.size(), (int **)&d.entry_mul));
   HIP_TRY(upload(h, A, h->entry_table.data(), h->entry_table.size(), (int **)&d.entry_table));
   HIP_TRY(upload(h, A, ft.data(), ft.size(), (float **)&d.flower_tables));
-  // binary ranges
+  // binary slots: orbits with >= 1 site, in orbit order
   if (h->binary) {
-    std::vector<int2> rng((size_t)s->n_sublat * s->n_orbits, make_int2(0, 0));
-    for (int b = 0; b < s->n_sublat; ++b)
-      for (int g = s->bin_seg_begin[b]; g < s->bin_seg_begin[b + 1]; ++g) {
-        const int o = s->bin_segs[3 * g], e0 = s->bin_segs[3 * g + 1], e1 = s->bin_segs[3 * g + 2];
-        ARG_CHECK(o >= 0 && o < s->n_orbits, "binary segment orbit out of range");
-        ARG_CHECK(rng[b * s->n_orbits + o].y == 0, "binary segments must be contiguous per orbit");
-        rng[b * s->n_orbits + o] = make_int2(e0, e1);
-      }
+    d.n_slots = 0;
+    h->bin_w.assign(s->n_orbits, 0.0);
     for (int a = 0; a < s->n_orbits; ++a) {
-      d.bin_w[a] = (float)s->bin_orbit_weight[a];
-      d.bin_w64[a] = s->bin_orbit_weight[a];
-      d.orbit_k[a] = s->orbit_k[a];
       ARG_CHECK(s->orbit_nfunc[a] == 1, "binary path requires one function per orbit");
+      d.bin_w64[a] = s->bin_orbit_weight[a];
+      h->bin_w[a] = s->bin_orbit_weight[a];
+      if (s->orbit_k[a] >= 1) d.slot_orbit[d.n_slots++] = a;
     }
-    HIP_TRY(upload(h, A, rng.data(), rng.size(), (int2 **)&d.bin_range));
   }
   // correlations
   std::vector<int> func_orbit(s->n_corr, 0);
@@ -344,6 +349,63 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
           ARG_CHECK(a >= 0 && a < pad, "neighbour access outside the padded image");
         }
     }
+  // binary site programs: per sublattice, per slot (orbit with >= 1 site), the
+  // padded offsets of the other sites of every cluster containing the site,
+  // padded to a multiple of 4 clusters with all-zero (self) offsets
+  std::vector<BinSub> bin_sub(h->n_sublat);
+  std::vector<int> bin_off;
+  if (h->binary) {
+    for (int b = 0; b < h->n_sublat; ++b) {
+      BinSub &B = bin_sub[b];
+      memset(&B, 0, sizeof(B));
+      for (int j = 0; j < h->dsys.n_slots; ++j) {
+        const int a = h->dsys.slot_orbit[j];
+        const int ko = h->orbit_k[a] - 1;
+        B.slot_k[j] = ko;
+        B.slot_w[j] = (float)h->bin_w[a];
+        B.slot_begin[j] = (int)bin_off.size();
+        int n = 0;
+        for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+          if (h->entry_orbit[e] != a) continue;
+          ARG_CHECK(h->entry_site_begin[e + 1] - h->entry_site_begin[e] == ko, "entry size mismatch");
+          for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q) bin_off.push_back(entry_off[q]);
+          n++;
+        }
+        B.slot_n[j] = n;
+        if (ko == 0) {
+          B.slot_len[j] = 0;
+          B.slot_dummy[j] = 0;
+          ARG_CHECK(n <= 1, "point orbit with several clusters per site");
+          continue;
+        }
+        const int len = (n + 3) / 4 * 4;
+        for (int e = n; e < len; ++e)
+          for (int q = 0; q < ko; ++q) bin_off.push_back(0);
+        B.slot_len[j] = len;
+        B.slot_dummy[j] = len - n;
+        ARG_CHECK(n <= 255, "too many clusters per site for the binary path");
+      }
+    }
+  }
+  // general site programs: per cluster k, table offset, then (offset, radix)
+  std::vector<GenSub> gen_sub(h->n_sublat);
+  std::vector<int> gen_prog;
+  for (int b = 0; b < h->n_sublat; ++b) {
+    GenSub &G = gen_sub[b];
+    G.begin = (int)gen_prog.size();
+    G.n_entries = h->flower_begin[b + 1] - h->flower_begin[b];
+    G.nf = std::max(0, h->n_occ[b] - 1);
+    G.pad = 0;
+    for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+      const int k = h->entry_site_begin[e + 1] - h->entry_site_begin[e];
+      gen_prog.push_back(k);
+      gen_prog.push_back(h->entry_table[e]);
+      for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q) {
+        gen_prog.push_back(entry_off[q]);
+        gen_prog.push_back(h->entry_mul[q]);
+      }
+    }
+  }
   // pair phases (canonical cross-class mixing)
   std::vector<int> pair_begin(1, 0), pair_sub, pair_halo_begin(1, 0);
   std::vector<int2> pair_addr, pair_halo;
@@ -374,7 +436,10 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, cb.data(), cb.size(), (int **)&d.class_begin));
   HIP_TRY(upload(h, A, class_sub.data(), class_sub.size(), (int **)&d.class_sublat));
   HIP_TRY(upload(h, A, class_addr.data(), class_addr.size(), (int **)&d.class_addr));
-  HIP_TRY(upload(h, A, class_lin.data(), class_lin.size(), (int **)&d.class_lin));
+  HIP_TRY(upload(h, A, bin_sub.data(), bin_sub.size(), (BinSub **)&d.bin_sub));
+  HIP_TRY(upload(h, A, bin_off.data(), bin_off.size(), (int **)&d.bin_off));
+  HIP_TRY(upload(h, A, gen_sub.data(), gen_sub.size(), (GenSub **)&d.gen_sub));
+  HIP_TRY(upload(h, A, gen_prog.data(), gen_prog.size(), (int **)&d.gen_prog));
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
   HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
@@ -402,13 +467,15 @@ static clexmc_status sync_supercells(clexmc_handle *h) {
 }
 
 static void free_chains(clexmc_handle *h) {
-  hipFree(h->d_chains);
+  hipFree(h->d_params);
+  hipFree(h->d_state);
   hipFree(h->d_occ);
   hipFree(h->d_binC);
   hipFree(h->d_sampE);
   hipFree(h->d_sampN);
   hipFree(h->d_sample_count);
-  h->d_chains = nullptr;
+  h->d_params = nullptr;
+  h->d_state = nullptr;
   h->d_occ = nullptr;
   h->d_binC = nullptr;
   h->d_sampE = nullptr;
@@ -418,7 +485,7 @@ static void free_chains(clexmc_handle *h) {
   h->n_chains = 0;
 }
 
-static clexmc_status run_corr(clexmc_handle *h, int chain, double *out_host);
+static clexmc_status run_corr(clexmc_handle *h, int chain0, int n, double *out_host);
 
 clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supercell_id, const int8_t *occ,
                                  const double *temperature, const double *chem, const uint64_t *seeds,
@@ -430,7 +497,8 @@ clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supe
   clexmc_status st = sync_supercells(h);
   if (st != CLEXMC_OK) return st;
   free_chains(h);
-  std::vector<DevChain> chains(n_chains);
+  std::vector<ChainParams> params(n_chains);
+  std::vector<ChainState> state(n_chains);
   h->chain_sc.assign(n_chains, 0);
   h->chain_off.assign(n_chains, 0);
   long long off = 0;
@@ -438,75 +506,83 @@ clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supe
     const int id = supercell_id[c];
     ARG_CHECK(id >= 0 && id < (int)h->scs.size(), "bad supercell id");
     const DevSupercell &d = h->scs[id].dev;
-    DevChain &ch = chains[c];
-    memset(&ch, 0, sizeof(ch));
-    ch.sc = id;
-    ch.occ_offset = (int)off;
+    ChainParams &cp = params[c];
+    ChainState &cs = state[c];
+    memset(&cp, 0, sizeof(cp));
+    memset(&cs, 0, sizeof(cs));
+    cp.sc = id;
+    cp.occ_offset = (int)off;
     h->chain_sc[c] = id;
     h->chain_off[c] = (int)off;
     ARG_CHECK(temperature[c] >= 0.0, "negative temperature");
     const double beta = temperature[c] > 0.0 ? 1.0 / (kb * temperature[c]) : std::numeric_limits<double>::infinity();
-    ch.beta64 = beta;
-    ch.beta = std::isfinite(beta) ? (float)beta : std::numeric_limits<float>::infinity();
+    cp.beta64 = beta;
+    cp.beta = std::isfinite(beta) ? (float)beta : std::numeric_limits<float>::infinity();
     for (int b = 0; b < h->n_sublat; ++b)
       for (int o = 0; o < kMaxOcc; ++o)
-        ch.chem[b * kMaxOcc + o] =
+        cp.chem[b * kMaxOcc + o] =
             (chem && o < h->n_occ_max) ? (float)chem[(long long)c * h->n_sublat * h->n_occ_max + b * h->n_occ_max + o] : 0.f;
-    ch.key0 = (uint32_t)(seeds[c] & 0xffffffffu);
-    ch.key1 = (uint32_t)(seeds[c] >> 32);
+    cp.key0 = (uint32_t)(seeds[c] & 0xffffffffu);
+    cp.key1 = (uint32_t)(seeds[c] >> 32);
     // validate occupation and count occupants on variable sites
     for (int l = 0; l < d.n_sites; ++l) {
       const int b = l / d.n_cells;
       const int v = occ[off + l];
       ARG_CHECK(v >= 0 && v < h->n_occ[b], "occupant index out of range");
-      if (h->n_occ[b] >= 2) ch.counts[v] += 1;
+      if (h->n_occ[b] >= 2) cs.counts[v] += 1;
     }
     off += d.n_sites;
   }
   ARG_CHECK(off < (1LL << 31), "total occupation too large");
   h->occ_total = off;
-  HIP_TRY(hipMalloc(&h->d_chains, sizeof(DevChain) * n_chains));
+  HIP_TRY(hipMalloc(&h->d_params, sizeof(ChainParams) * n_chains));
+  HIP_TRY(hipMalloc(&h->d_state, sizeof(ChainState) * n_chains));
   HIP_TRY(hipMalloc(&h->d_occ, (size_t)off));
   HIP_TRY(hipMemcpy(h->d_occ, occ, (size_t)off, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&h->d_binC, sizeof(long long) * (size_t)n_chains * h->n_orbits));
   HIP_TRY(hipMalloc(&h->d_sample_count, sizeof(int) * n_chains));
   HIP_TRY(hipMemset(h->d_sample_count, 0, sizeof(int) * n_chains));
-  HIP_TRY(hipMemcpy(h->d_chains, chains.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_params, params.data(), sizeof(ChainParams) * n_chains, hipMemcpyHostToDevice));
   h->n_chains = n_chains;
   // initial energies (and exact binary counts) from full correlations
-  std::vector<double> corr(h->n_corr);
-  std::vector<long long> C(h->n_orbits);
+  std::vector<double> corr((size_t)n_chains * h->n_corr);
+  st = run_corr(h, 0, n_chains, corr.data());
+  if (st != CLEXMC_OK) return st;
+  std::vector<long long> C((size_t)n_chains * h->n_orbits, 0);
   for (int c = 0; c < n_chains; ++c) {
-    st = run_corr(h, c, corr.data());
-    if (st != CLEXMC_OK) return st;
+    const double *cc = corr.data() + (size_t)c * h->n_corr;
     double E = 0.0;
-    for (int i = 0; i < h->n_corr; ++i) E += h->eci[i] * corr[i];
-    chains[c].E = E;
+    for (int i = 0; i < h->n_corr; ++i) E += h->eci[i] * cc[i];
+    state[c].E = E;
     if (h->binary) {
-      for (int a = 0; a < h->n_orbits; ++a) C[a] = llround(corr[h->orbit_func_begin[a]] * h->orbit_mult[a]);
-      HIP_TRY(hipMemcpy(h->d_binC + (size_t)c * h->n_orbits, C.data(), sizeof(long long) * h->n_orbits,
-                        hipMemcpyHostToDevice));
       double Eb = 0.0;
-      for (int a = 0; a < h->n_orbits; ++a) Eb += h->dsys.bin_w64[a] * (double)C[a];
-      chains[c].E = Eb;
+      for (int a = 0; a < h->n_orbits; ++a) {
+        const long long v = llround(cc[h->orbit_func_begin[a]] * h->orbit_mult[a]);
+        C[(size_t)c * h->n_orbits + a] = v;
+        Eb += h->bin_w[a] * (double)v;
+      }
+      state[c].E = Eb;
     }
   }
-  HIP_TRY(hipMemcpy(h->d_chains, chains.data(), sizeof(DevChain) * n_chains, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_binC, C.data(), sizeof(long long) * C.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_state, state.data(), sizeof(ChainState) * n_chains, hipMemcpyHostToDevice));
   return CLEXMC_OK;
 }
 
-static clexmc_status run_corr(clexmc_handle *h, int chain, double *out_host) {
+static clexmc_status run_corr(clexmc_handle *h, int chain0, int n, double *out_host) {
   double *d_out = nullptr;
-  HIP_TRY(hipMalloc(&d_out, sizeof(double) * h->n_corr));
+  HIP_TRY(hipMalloc(&d_out, sizeof(double) * h->n_corr * (size_t)n));
   CorrArgs a;
   a.sys = h->d_sys;
   a.scs = h->d_scs;
-  a.chains = h->d_chains;
+  a.params = h->d_params;
   a.occ = h->d_occ;
-  a.chain = chain;
+  a.chain0 = chain0;
+  a.n_corr = h->n_corr;
   a.out = d_out;
-  hipError_t e = launch_corr(a, h->n_corr, h->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(out_host, d_out, sizeof(double) * h->n_corr, hipMemcpyDeviceToHost, h->stream);
+  hipError_t e = launch_corr(a, h->n_corr, n, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_host, d_out, sizeof(double) * h->n_corr * (size_t)n, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_out);
   if (e != hipSuccess) {
@@ -520,7 +596,7 @@ clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out)
   if (!h || !corr_out) return CLEXMC_ERR_ARG;
   ARG_CHECK(chain >= 0 && chain < h->n_chains, "chain out of range");
   HIP_TRY(hipSetDevice(h->device));
-  return run_corr(h, chain, corr_out);
+  return run_corr(h, chain, 1, corr_out);
 }
 
 clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
@@ -558,7 +634,7 @@ clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int3
     DeltaArgs a;
     a.sys = h->d_sys;
     a.scs = h->d_scs;
-    a.chains = h->d_chains;
+    a.params = h->d_params;
     a.occ = h->d_occ;
     a.n_prop = n_prop;
     a.sites_per_prop = sites_per_prop;
@@ -605,7 +681,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   }
   int threads = h->block_size;
   if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
-  size_t lds = metropolis_lds_bytes(max_pad, h->n_orbits, h->binary != 0, threads, max_class);
+  const int n_tab = h->binary ? 0 : h->n_flower_tables;
+  size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, threads, max_class);
   int max_lds = 0;
   HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
   if ((long long)lds > (long long)max_lds - 64) {
@@ -639,7 +716,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   RunArgs a;
   a.sys = h->d_sys;
   a.scs = h->d_scs;
-  a.chains = h->d_chains;
+  a.params = h->d_params;
+  a.state = h->d_state;
   a.occ = h->d_occ;
   a.binC = h->d_binC;
   a.sampE = h->d_sampE;
@@ -649,6 +727,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   a.n_passes = n_passes;
   a.sample_period = sample_period;
   a.rounds = std::max(1, canonical_rounds);
+  a.n_orbits = h->n_orbits;
+  a.n_tab = n_tab;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
@@ -700,8 +780,8 @@ clexmc_status clexmc_read_state(clexmc_handle *h, int chain, int8_t *occ, double
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(chain >= 0 && chain < h->n_chains, "chain out of range");
   HIP_TRY(hipSetDevice(h->device));
-  DevChain ch;
-  HIP_TRY(hipMemcpy(&ch, h->d_chains + chain, sizeof(DevChain), hipMemcpyDeviceToHost));
+  ChainState ch;
+  HIP_TRY(hipMemcpy(&ch, h->d_state + chain, sizeof(ChainState), hipMemcpyDeviceToHost));
   const DevSupercell &d = h->scs[h->chain_sc[chain]].dev;
   if (occ) HIP_TRY(hipMemcpy(occ, h->d_occ + h->chain_off[chain], (size_t)d.n_sites, hipMemcpyDeviceToHost));
   if (E_running) *E_running = ch.E;

@@ -3,11 +3,16 @@
 // Layout in device memory (HBM):
 //  * per chain: int8 occupation (N sites, linear index l = b*n_cells + cell);
 //  * per supercell: padded image geometry, independent-set classes as padded
-//    LDS addresses, halo copy lists, per-entry padded neighbour offsets;
+//    LDS addresses, halo copy lists and, per sublattice, the "site program":
+//    the padded LDS offsets of every other site of every cluster that
+//    contains the site, grouped per orbit (binary path) or per cluster with
+//    its table offset (general path);
 //  * system: flower tables (FP32 for the kernels, FP64 for correlations).
 // In LDS (one workgroup per chain): the padded occupation image
 //   [b][k + h][j + h][i + h] (int8), so every neighbour of every site is a
 //   constant offset away; h is the largest cell offset of any cluster.
+// Read-only tables are read through the constant address space so that the
+// uniform per-cluster offsets become scalar (SMEM) loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,15 +22,23 @@ namespace clexmc {
 constexpr int kMaxSublat = 8;
 constexpr int kMaxOcc = 4;
 constexpr int kMaxFunc = 3;
-constexpr int kMaxOrbits = 16;   // binary integer path: orbits held in registers
+constexpr int kMaxOrbits = 64;   // correlations kernel / exact counts storage
+constexpr int kMaxSlots = 8;     // binary path: orbits with >= 1 site (registers)
 constexpr int kMaxSitesPerProp = 4;
+
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> cst(const T *p) {
+  return (cptr<T>)(p);
+}
 
 struct DevSystem {
   int n_sublat, n_occ_max, n_func_max, n_orbits, n_corr, binary;
   int n_occ[kMaxSublat];
   int n_func[kMaxSublat];
   float site_func[kMaxSublat][kMaxFunc][kMaxOcc];
-  // flower (general table path)
+  // flower (general table path, delta batch)
   const int *flower_begin;      // [n_sublat+1]
   const int *entry_k;           // [n_entries]
   const int *entry_site_begin;  // [n_entries+1]
@@ -33,11 +46,10 @@ struct DevSystem {
   const int *entry_mul;         // [n_entry_sites] radix multiplier of each other site
   const int *entry_table;       // [n_entries] value offset into flower_tables
   const float *flower_tables;
-  // binary path: per (sublattice, orbit) entry range
-  const int2 *bin_range;        // [n_sublat][n_orbits] (entry_begin, entry_end)
-  float bin_w[kMaxOrbits];      // J_a / m_a
-  double bin_w64[kMaxOrbits];
-  int orbit_k[kMaxOrbits];
+  // binary path: slots = orbits with >= 1 site
+  int n_slots;
+  int slot_orbit[kMaxSlots];
+  double bin_w64[kMaxOrbits];   // J_a / m_a per orbit (energy from exact counts)
   // correlations
   const int *orbit_k_all;       // [n_orbits]
   const int *orbit_mult;
@@ -51,6 +63,22 @@ struct DevSystem {
   const int *func_orbit;        // [n_corr] orbit of each function
 };
 
+// Binary site program of one sublattice (per supercell).
+struct BinSub {
+  int slot_k[kMaxSlots];      // other sites per cluster (0: point)
+  int slot_n[kMaxSlots];      // clusters of the orbit containing the site
+  int slot_begin[kMaxSlots];  // first offset (index into bin_off)
+  int slot_len[kMaxSlots];    // clusters after padding to a multiple of 4
+  int slot_dummy[kMaxSlots];  // padding clusters (all offsets 0 = the site itself)
+  float slot_w[kMaxSlots];    // J_a / m_a
+};
+
+// General site program of one sublattice: gen_prog[begin .. begin+len) holds,
+// per cluster: k, table_offset, then k pairs (offset, radix multiplier).
+struct GenSub {
+  int begin, n_entries, nf, pad;
+};
+
 struct DevSupercell {
   int L1, L2, L3, h;
   int P1, P2, P3, S;      // padded dims, S = P1*P2*P3 bytes per sublattice
@@ -59,10 +87,13 @@ struct DevSupercell {
   const int *class_begin;  // [n_classes+1]
   const int *class_sublat; // [n_classes]
   const int *class_addr;   // padded LDS address of each class site
-  const int *class_lin;    // linear index of each class site
   const int *halo_begin;   // [n_classes+1]
   const int2 *halo;        // (src, dst) padded addresses
-  const int *entry_off;    // [n_entry_sites] padded relative offset
+  const int *entry_off;    // [n_entry_sites] padded relative offset (general path)
+  const BinSub *bin_sub;   // [n_sublat]
+  const int *bin_off;      // binary site programs
+  const GenSub *gen_sub;   // [n_sublat]
+  const int *gen_prog;     // general site programs
   int n_pair_phases;
   const int *pair_begin;   // [n_pair_phases+1]
   const int2 *pair_addr;   // padded addresses of both sites
@@ -72,7 +103,8 @@ struct DevSupercell {
   int max_class;           // largest class size
 };
 
-struct DevChain {
+// Read-only per-chain parameters (constant during a launch).
+struct ChainParams {
   int sc;
   int occ_offset;          // into the occupation buffer
   float beta;
@@ -80,6 +112,10 @@ struct DevChain {
   double beta64;
   float chem[kMaxSublat * kMaxOcc];
   uint32_t key0, key1;
+};
+
+// Mutable per-chain state.
+struct ChainState {
   unsigned long long draw_base;
   unsigned long long pass_base;
   double E;                // running formation energy per supercell (FP64)

@@ -8,8 +8,10 @@
 //                      Binary (+-1) expansions use integer cluster-product
 //                      counts per orbit (K_a, exact), FP32 dE = dphi*sum w_a K_a
 //                      and exact int64 correlation counts C_a per chain; other
-//                      expansions use FP32 flower tables and an FP64 running
-//                      energy.  Philox4x32-10 supplies the uniforms.
+//                      expansions use FP32 flower tables (staged in LDS) and
+//                      an FP64 running energy.  Philox4x32-10 supplies the
+//                      uniforms; lane accumulators are reduced per wavefront
+//                      with cross-lane shuffles.
 //  delta_batch_kernel  per-proposal dE (BaseMontePotential::occ_delta_per_supercell)
 //  corr_kernel         full per-supercell correlations in FP64
 //  philox_kernel       device Philox for known-answer checks
@@ -37,19 +39,13 @@ struct LaneRng {
       ++draw;
       left = 4;
     }
-    uint32_t x;
-    switch (left) {
-      case 4: x = buf.x; break;
-      case 3: x = buf.y; break;
-      case 2: x = buf.z; break;
-      default: x = buf.w; break;
-    }
+    uint32_t x = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
     --left;
     return u01(x);
   }
 };
 
-// block-uniform random integer in [0, n) for (pass, phase, round)
+// block-uniform random integer in [0, n) for (pass, phase)
 __device__ __forceinline__ uint32_t uniform_int(uint32_t n, unsigned long long pass, uint32_t phase,
                                                 uint32_t chain, uint32_t k0, uint32_t k1) {
   uint4 r = philox4x32_10(make_uint4(0xFFFFFFFFu, (uint32_t)pass, phase, chain ^ 0x9E3779B9u), k0, k1);
@@ -57,43 +53,74 @@ __device__ __forceinline__ uint32_t uniform_int(uint32_t n, unsigned long long p
   return n ? (uint32_t)(x % n) : 0u;
 }
 
-// ---------------------------------------------------------------------------
-// Binary path: K_a(s) = sum over instances of orbit a containing s of
-// prod phi(other sites) = (-1)^(k-1) * (n_a - 2 * sum XOR(other bits)).
-// ---------------------------------------------------------------------------
-template <int NORB>
-__device__ __forceinline__ float binary_field(const int8_t *__restrict__ img, int pa, int b,
-                                              const DevSystem *__restrict__ sys,
-                                              const DevSupercell &sc, int n_orbits, int (&K)[NORB]) {
-  float F = 0.0f;
-  const int2 *rng = sys->bin_range + b * n_orbits;
-  const int *__restrict__ esb = sys->entry_site_begin;
-  const int *__restrict__ off = sc.entry_off;
+__device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
-  for (int a = 0; a < NORB; ++a) {
-    K[a] = 0;
-    if (a < n_orbits) {
-      const int2 r = rng[a];
-      const int n = r.y - r.x;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Binary path: K_a(s) = sum over clusters of orbit a containing s of
+// prod phi(other sites) = (-1)^(k-1) * (n_a - 2 * sum XOR(other bits)).
+// The site program lists the padded offsets of the other sites per orbit,
+// padded to multiples of 4 clusters with offset-0 entries (the site itself):
+// an even number of copies of the site XORs to 0, an odd number contributes
+// the site's own bit, removed below.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float binary_field(const uint8_t *img, int pa, int o, cptr<BinSub> bs,
+                                              cptr<int> off, int n_slots, int (&K)[kMaxSlots]) {
+  float F = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kMaxSlots; ++j) {
+    K[j] = 0;
+    if (j < n_slots) {
+      const int n = bs->slot_n[j];
       if (n > 0) {
-        const int ko = sys->orbit_k[a] - 1;
+        const int ko = bs->slot_k[j];
+        const int q0 = bs->slot_begin[j];
+        const int len = bs->slot_len[j];
         int X = 0;
         if (ko == 1) {
-          const int q0 = esb[r.x];
-          for (int q = 0; q < n; ++q) X += img[pa + off[q0 + q]];
+#pragma unroll 2
+          for (int e = 0; e < len; e += 4) {
+            const int q = q0 + e;
+            const int b0 = img[pa + off[q]];
+            const int b1 = img[pa + off[q + 1]];
+            const int b2 = img[pa + off[q + 2]];
+            const int b3 = img[pa + off[q + 3]];
+            X += (b0 + b1) + (b2 + b3);
+          }
         } else if (ko == 2) {
-          const int q0 = esb[r.x];
-          for (int q = 0; q < n; ++q) X += img[pa + off[q0 + 2 * q]] ^ img[pa + off[q0 + 2 * q + 1]];
-        } else if (ko > 2) {
-          for (int e = r.x; e < r.y; ++e) {
+#pragma unroll 2
+          for (int e = 0; e < len; e += 4) {
+            const int q = q0 + 2 * e;
+            const int b0 = img[pa + off[q]];
+            const int b1 = img[pa + off[q + 1]];
+            const int b2 = img[pa + off[q + 2]];
+            const int b3 = img[pa + off[q + 3]];
+            const int b4 = img[pa + off[q + 4]];
+            const int b5 = img[pa + off[q + 5]];
+            const int b6 = img[pa + off[q + 6]];
+            const int b7 = img[pa + off[q + 7]];
+            X += ((b0 ^ b1) + (b2 ^ b3)) + ((b4 ^ b5) + (b6 ^ b7));
+          }
+        } else if (ko >= 3) {
+          for (int e = 0; e < len; ++e) {
             int x = 0;
-            for (int q = esb[e]; q < esb[e + 1]; ++q) x ^= img[pa + off[q]];
+            for (int q = q0 + e * ko; q < q0 + (e + 1) * ko; ++q) x ^= img[pa + off[q]];
             X += x;
           }
         }
-        const int Ka = (ko & 1) ? -(n - 2 * X) : (n - 2 * X);
-        K[a] = Ka;
-        F += sys->bin_w[a] * (float)Ka;
+        if (ko & 1) X -= bs->slot_dummy[j] * o;
+        const int Kj = (ko & 1) ? (2 * X - n) : (n - 2 * X);
+        K[j] = Kj;
+        F += bs->slot_w[j] * (float)Kj;
       }
     }
   }
@@ -101,27 +128,29 @@ __device__ __forceinline__ float binary_field(const int8_t *__restrict__ img, in
 }
 
 // General path: G_f(s) = sum_e V_e[tau_others][f] from the flower tables.
-__device__ __forceinline__ void general_field(const int8_t *__restrict__ img, int pa, int b,
-                                              const DevSystem *__restrict__ sys,
-                                              const DevSupercell &sc, float (&G)[kMaxFunc]) {
+__device__ __forceinline__ void general_field(const uint8_t *img, int pa, cptr<GenSub> gs, cptr<int> prog,
+                                              const float *tab, float (&G)[kMaxFunc]) {
   G[0] = G[1] = G[2] = 0.0f;
-  const int nf = sys->n_func[b];
-  const int e0 = sys->flower_begin[b], e1 = sys->flower_begin[b + 1];
-  const int *__restrict__ esb = sys->entry_site_begin;
-  const int *__restrict__ off = sc.entry_off;
-  const int *__restrict__ mul = sys->entry_mul;
-  for (int e = e0; e < e1; ++e) {
+  const int nf = gs->nf;
+  const int n = gs->n_entries;
+  int q = gs->begin;
+  for (int e = 0; e < n; ++e) {
+    const int k = prog[q];
+    const int toff = prog[q + 1];
+    q += 2;
     int idx = 0;
-    for (int q = esb[e]; q < esb[e + 1]; ++q) idx += img[pa + off[q]] * mul[q];
-    const float *V = sys->flower_tables + sys->entry_table[e] + idx * nf;
+    for (int i = 0; i < k; ++i) {
+      idx += img[pa + prog[q]] * prog[q + 1];
+      q += 2;
+    }
+    const float *V = tab + toff + idx * nf;
     G[0] += V[0];
     if (nf > 1) G[1] += V[1];
     if (nf > 2) G[2] += V[2];
   }
 }
 
-__device__ __forceinline__ float general_dE(const DevSystem *__restrict__ sys, int b, int o, int nw,
-                                            const float (&G)[kMaxFunc]) {
+__device__ __forceinline__ float general_dE(cptr<DevSystem> sys, int b, int o, int nw, const float (&G)[kMaxFunc]) {
   const int nf = sys->n_func[b];
   float d = (sys->site_func[b][0][nw] - sys->site_func[b][0][o]) * G[0];
   if (nf > 1) d += (sys->site_func[b][1][nw] - sys->site_func[b][1][o]) * G[1];
@@ -142,110 +171,125 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
   const int chain = blockIdx.x;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
-  const DevSystem *__restrict__ sys = args.sys;
-  DevChain *ch = args.chains + chain;
-  const DevSupercell sc = args.scs[ch->sc];
-  const int n_orbits = sys->n_orbits;
-  const int nocc_max = sys->n_occ_max;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int n_waves = nthr >> 6;
+  cptr<DevSystem> sys = cst(args.sys);
+  cptr<ChainParams> cp = cst(args.params + chain);
+  ChainState *cs = args.state + chain;
+  cptr<DevSupercell> sc = cst(args.scs + cp->sc);
 
-  int8_t *img = (int8_t *)smem;
-  const int img_bytes = (sc.pad_bytes + 15) & ~15;
-  // reduction slots: int slots [n_islot][nthr], then double slot [nthr]
-  const int n_islot = (BIN ? n_orbits : 0) + kMaxOcc + 2;
-  int *islot = (int *)(smem + img_bytes);
-  double *dslot = (double *)(islot + n_islot * nthr);
-  int *isum = (int *)(dslot + nthr);             // [n_islot]
-  double *dsum = (double *)(isum + ((n_islot + 1) & ~1));  // [1]
-  unsigned char *scratch = (unsigned char *)(dsum + 2);
-  // canonical class scratch: F/G per class site and K (int8) per orbit
-  float *cF = (float *)scratch;                               // [kMaxFunc][max_class]
-  int8_t *cK = (int8_t *)(cF + kMaxFunc * sc.max_class);      // [NORB][max_class]
+  const int L1 = sc->L1, L2 = sc->L2, L3 = sc->L3, h = sc->h;
+  const int P1 = sc->P1, P12 = sc->P1 * sc->P2, S = sc->S;
+  const int n_cells = sc->n_cells, n_sites = sc->n_sites, pad_bytes = sc->pad_bytes;
+  const int n_classes = sc->n_classes, max_class = sc->max_class;
+  const int n_slots = sys->n_slots;
 
-  // ---- load the padded image ----
-  const int8_t *__restrict__ occG = args.occ + ch->occ_offset;
-  {
-    const int P1 = sc.P1, P12 = sc.P1 * sc.P2, S = sc.S, h = sc.h;
-    for (int p = tid; p < sc.pad_bytes; p += nthr) {
-      int b = p / S;
-      int r = p - b * S;
-      int pk = r / P12;
-      int r2 = r - pk * P12;
-      int pj = r2 / P1;
-      int pi = r2 - pj * P1;
-      int i = ((pi - h) % sc.L1 + sc.L1) % sc.L1;
-      int j = ((pj - h) % sc.L2 + sc.L2) % sc.L2;
-      int k = ((pk - h) % sc.L3 + sc.L3) % sc.L3;
-      img[p] = occG[b * sc.n_cells + i + sc.L1 * (j + sc.L2 * k)];
-    }
+  // ---- LDS layout ----
+  uint8_t *img = (uint8_t *)smem;
+  size_t off_b = (size_t)((pad_bytes + 15) & ~15);
+  float *tabL = (float *)(smem + off_b);
+  const int n_tab = BIN ? 0 : args.n_tab;
+  off_b += (size_t)((n_tab * 4 + 15) & ~15);
+  const int n_islot = (BIN ? kMaxSlots : 0) + kMaxOcc + 2;
+  int *redI = (int *)(smem + off_b);                     // [n_islot][n_waves]
+  off_b += (size_t)n_islot * 16 * sizeof(int);
+  double *redD = (double *)(smem + off_b);               // [n_waves]
+  off_b += 16 * sizeof(double);
+  unsigned long long *maxDraw = (unsigned long long *)(smem + off_b);
+  off_b += 16;
+  float *cF = (float *)(smem + off_b);                   // [kMaxFunc][max_class]
+  int8_t *cK = (int8_t *)(cF + kMaxFunc * max_class);    // [kMaxSlots][max_class]
+
+  // ---- load the padded image (and general tables) ----
+  const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
+  for (int p = tid; p < pad_bytes; p += nthr) {
+    int b = p / S;
+    int r = p - b * S;
+    int pk = r / P12;
+    int r2 = r - pk * P12;
+    int pj = r2 / P1;
+    int pi = r2 - pj * P1;
+    int i = ((pi - h) % L1 + L1) % L1;
+    int j = ((pj - h) % L2 + L2) % L2;
+    int k = ((pk - h) % L3 + L3) % L3;
+    img[p] = (uint8_t)occG[b * n_cells + i + L1 * (j + L2 * k)];
   }
+  if (!BIN) {
+    const float *__restrict__ tg = sys->flower_tables;
+    for (int p = tid; p < n_tab; p += nthr) tabL[p] = tg[p];
+  }
+  if (tid == 0) *maxDraw = 0ull;
   __syncthreads();
 
-  const float beta = ch->beta;
+  const float beta = cp->beta;
   LaneRng rng;
   rng.left = 0;
-  rng.draw = ch->draw_base;
+  rng.draw = cs->draw_base;
   rng.lane = (uint32_t)tid;
   rng.chain = (uint32_t)chain;
-  rng.k0 = ch->key0;
-  rng.k1 = ch->key1;
+  rng.k0 = cp->key0;
+  rng.k1 = cp->key1;
 
   // per-lane accumulators
-  int accC[kMaxOrbits];
+  int accC[kMaxSlots];
 #pragma unroll
-  for (int a = 0; a < kMaxOrbits; ++a) accC[a] = 0;
+  for (int a = 0; a < kMaxSlots; ++a) accC[a] = 0;
   int accN[kMaxOcc] = {0, 0, 0, 0};
   int accAtt = 0, accAcc = 0;
   double accE = 0.0;
 
-  const unsigned long long pass0 = ch->pass_base;
+  const unsigned long long pass0 = cs->pass_base;
   const int rounds = args.rounds < 1 ? 1 : args.rounds;
+  cptr<int> bin_off = cst(sc->bin_off);
+  cptr<int> gen_prog = cst(sc->gen_prog);
 
   for (int pass = 0; pass < args.n_passes; ++pass) {
     const unsigned long long gpass = pass0 + (unsigned long long)pass;
-    for (int c = 0; c < sc.n_classes; ++c) {
-      const int b = sc.class_sublat[c];
-      const int cb = sc.class_begin[c], ce = sc.class_begin[c + 1];
+    for (int c = 0; c < n_classes; ++c) {
+      const int b = sc->class_sublat[c];
+      const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
+      cptr<BinSub> bs = cst(sc->bin_sub + b);
+      cptr<GenSub> gs = cst(sc->gen_sub + b);
+      const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
       if (KIND == 1) {
         // ---- semi-grand canonical: every site of the class proposes a change
         for (int t = cb + tid; t < ce; t += nthr) {
-          const int pa = sc.class_addr[t];
+          const int pa = sc->class_addr[t];
           const int o = img[pa];
-          float u = rng.next();
+          const float u = rng.next();
           if (BIN) {
-            int K[kMaxOrbits];
-            const float F = binary_field<kMaxOrbits>(img, pa, b, sys, sc, n_orbits, K);
-            const int nw = 1 - o;
+            int K[kMaxSlots];
+            const float F = binary_field(img, pa, o, bs, bin_off, n_slots, K);
             const float dphi = o ? -2.0f : 2.0f;
-            const float dE = dphi * F - (ch->chem[b * kMaxOcc + nw] - ch->chem[b * kMaxOcc + o]);
+            const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
             const bool acc = metropolis_accept(dE, beta, u);
             accAtt += 1;
-            if (acc) {
-              img[pa] = (int8_t)nw;
-              accAcc += 1;
-              const int s = o ? -2 : 2;
+            const int sk = acc ? (o ? -2 : 2) : 0;
 #pragma unroll
-              for (int a = 0; a < kMaxOrbits; ++a) accC[a] += s * K[a];
-              accN[nw] += 1;
-              accN[o] -= 1;
-            }
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+            accN[1] += sk / 2;
+            accN[0] -= sk / 2;
+            accAcc += acc ? 1 : 0;
+            if (acc) img[pa] = (uint8_t)(1 - o);
           } else {
             float G[kMaxFunc];
-            general_field(img, pa, b, sys, sc, G);
+            general_field(img, pa, gs, gen_prog, tabL, G);
             const int no = sys->n_occ[b];
-            float u2 = rng.next();
+            const float u2 = rng.next();
             int r = (int)(u2 * (float)(no - 1));
             if (r > no - 2) r = no - 2;
             const int nw = r >= o ? r + 1 : r;
             const float dEf = general_dE(sys, b, o, nw, G);
-            const float dE = dEf - (ch->chem[b * kMaxOcc + nw] - ch->chem[b * kMaxOcc + o]);
+            const float dE = dEf - (cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o]);
             const bool acc = metropolis_accept(dE, beta, u);
             accAtt += 1;
             if (acc) {
-              img[pa] = (int8_t)nw;
+              img[pa] = (uint8_t)nw;
               accAcc += 1;
               accE += (double)dEf;
-              accN[nw] += 1;
-              accN[o] -= 1;
+#pragma unroll
+              for (int q = 0; q < kMaxOcc; ++q) accN[q] += (q == nw) - (q == o);
             }
           }
         }
@@ -254,54 +298,52 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
         // ---- canonical: fields of all class sites, then random matchings
         const int n = ce - cb;
         for (int t = tid; t < n; t += nthr) {
-          const int pa = sc.class_addr[cb + t];
+          const int pa = sc->class_addr[cb + t];
           if (BIN) {
-            int K[kMaxOrbits];
-            cF[t] = binary_field<kMaxOrbits>(img, pa, b, sys, sc, n_orbits, K);
+            int K[kMaxSlots];
+            cF[t] = binary_field(img, pa, img[pa], bs, bin_off, n_slots, K);
 #pragma unroll
-            for (int a = 0; a < kMaxOrbits; ++a)
-              if (a < n_orbits) cK[a * sc.max_class + t] = (int8_t)K[a];
+            for (int a = 0; a < kMaxSlots; ++a) cK[a * max_class + t] = (int8_t)K[a];
           } else {
             float G[kMaxFunc];
-            general_field(img, pa, b, sys, sc, G);
+            general_field(img, pa, gs, gen_prog, tabL, G);
             cF[t] = G[0];
-            cF[sc.max_class + t] = G[1];
-            cF[2 * sc.max_class + t] = G[2];
+            cF[max_class + t] = G[1];
+            cF[2 * max_class + t] = G[2];
           }
         }
         __syncthreads();
         for (int rd = 0; rd < rounds; ++rd) {
           const uint32_t r = uniform_int((uint32_t)n, gpass, (uint32_t)(c * rounds + rd), (uint32_t)chain,
-                                         ch->key0, ch->key1);
+                                         cp->key0, cp->key1);
           for (int p = tid; p < n; p += nthr) {
             int q = (int)r - p;
             if (q < 0) q += n;
             if (p >= q) continue;  // q == p: unpaired; p > q: the partner leads
-            const int pa = sc.class_addr[cb + p], qa = sc.class_addr[cb + q];
+            const int pa = sc->class_addr[cb + p], qa = sc->class_addr[cb + q];
             const int op = img[pa], oq = img[qa];
             if (op == oq) continue;
-            float u = rng.next();
+            const float u = rng.next();
             float dE;
             if (BIN) {
               const float dp = op ? -2.0f : 2.0f;
               dE = dp * cF[p] - dp * cF[q];
             } else {
-              float Gp[kMaxFunc] = {cF[p], cF[sc.max_class + p], cF[2 * sc.max_class + p]};
-              float Gq[kMaxFunc] = {cF[q], cF[sc.max_class + q], cF[2 * sc.max_class + q]};
+              float Gp[kMaxFunc] = {cF[p], cF[max_class + p], cF[2 * max_class + p]};
+              float Gq[kMaxFunc] = {cF[q], cF[max_class + q], cF[2 * max_class + q]};
               dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
             }
             const bool acc = metropolis_accept(dE, beta, u);
             accAtt += 1;
             if (acc) {
-              img[pa] = (int8_t)oq;
-              img[qa] = (int8_t)op;
+              img[pa] = (uint8_t)oq;
+              img[qa] = (uint8_t)op;
               accAcc += 1;
               if (BIN) {
                 const int sp = op ? -2 : 2;
 #pragma unroll
-                for (int a = 0; a < kMaxOrbits; ++a)
-                  if (a < n_orbits)
-                    accC[a] += sp * ((int)cK[a * sc.max_class + p] - (int)cK[a * sc.max_class + q]);
+                for (int a = 0; a < kMaxSlots; ++a)
+                  accC[a] += sp * ((int)cK[a * max_class + p] - (int)cK[a * max_class + q]);
               } else {
                 accE += (double)dE;
               }
@@ -311,55 +353,57 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
         }
       }
       // ---- halo copies of this class
-      for (int t = sc.halo_begin[c] + tid; t < sc.halo_begin[c + 1]; t += nthr) {
-        const int2 hc = sc.halo[t];
+      for (int t = sc->halo_begin[c] + tid; t < sc->halo_begin[c + 1]; t += nthr) {
+        const int2 hc = sc->halo[t];
         img[hc.y] = img[hc.x];
       }
       __syncthreads();
     }
 
     // ---- canonical cross-class mixing: one randomly chosen pair phase
-    if (KIND == 0 && sc.n_pair_phases > 0) {
-      const uint32_t ph = uniform_int((uint32_t)sc.n_pair_phases, gpass, 0xFFFFFFF0u, (uint32_t)chain,
-                                      ch->key0, ch->key1);
-      const int pb = sc.pair_begin[ph], pe = sc.pair_begin[ph + 1];
+    if (KIND == 0 && sc->n_pair_phases > 0) {
+      const uint32_t ph = uniform_int((uint32_t)sc->n_pair_phases, gpass, 0xFFFFFFF0u, (uint32_t)chain,
+                                      cp->key0, cp->key1);
+      const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
-        const int2 pq = sc.pair_addr[t];
-        const int b = sc.pair_sublat[t];
+        const int2 pq = sc->pair_addr[t];
+        const int b = sc->pair_sublat[t];
         const int op = img[pq.x], oq = img[pq.y];
         if (op == oq) continue;
-        float u = rng.next();
+        const float u = rng.next();
         float dE;
-        int Kp[kMaxOrbits], Kq[kMaxOrbits];
+        int Kp[kMaxSlots], Kq[kMaxSlots];
         if (BIN) {
-          const float Fp = binary_field<kMaxOrbits>(img, pq.x, b, sys, sc, n_orbits, Kp);
-          const float Fq = binary_field<kMaxOrbits>(img, pq.y, b, sys, sc, n_orbits, Kq);
+          cptr<BinSub> bs = cst(sc->bin_sub + b);
+          const float Fp = binary_field(img, pq.x, op, bs, bin_off, n_slots, Kp);
+          const float Fq = binary_field(img, pq.y, oq, bs, bin_off, n_slots, Kq);
           const float dp = op ? -2.0f : 2.0f;
           dE = dp * Fp - dp * Fq;
         } else {
+          cptr<GenSub> gs = cst(sc->gen_sub + b);
           float Gp[kMaxFunc], Gq[kMaxFunc];
-          general_field(img, pq.x, b, sys, sc, Gp);
-          general_field(img, pq.y, b, sys, sc, Gq);
+          general_field(img, pq.x, gs, gen_prog, tabL, Gp);
+          general_field(img, pq.y, gs, gen_prog, tabL, Gq);
           dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
         }
         const bool acc = metropolis_accept(dE, beta, u);
         accAtt += 1;
         if (acc) {
-          img[pq.x] = (int8_t)oq;
-          img[pq.y] = (int8_t)op;
+          img[pq.x] = (uint8_t)oq;
+          img[pq.y] = (uint8_t)op;
           accAcc += 1;
           if (BIN) {
             const int sp = op ? -2 : 2;
 #pragma unroll
-            for (int a = 0; a < kMaxOrbits; ++a) accC[a] += sp * (Kp[a] - Kq[a]);
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sp * (Kp[a] - Kq[a]);
           } else {
             accE += (double)dE;
           }
         }
       }
       __syncthreads();
-      for (int t = sc.pair_halo_begin[ph] + tid; t < sc.pair_halo_begin[ph + 1]; t += nthr) {
-        const int2 hc = sc.pair_halo[t];
+      for (int t = sc->pair_halo_begin[ph] + tid; t < sc->pair_halo_begin[ph + 1]; t += nthr) {
+        const int2 hc = sc->pair_halo[t];
         img[hc.y] = img[hc.x];
       }
       __syncthreads();
@@ -372,95 +416,98 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
       int s = 0;
       if (BIN) {
 #pragma unroll
-        for (int a = 0; a < kMaxOrbits; ++a)
-          if (a < n_orbits) islot[(s + a) * nthr + tid] = accC[a];
-        s += n_orbits;
+        for (int a = 0; a < kMaxSlots; ++a) {
+          const int v = wave_sum(accC[a]);
+          if (lane == 0) redI[(s + a) * n_waves + wave] = v;
+          accC[a] = 0;
+        }
+        s += kMaxSlots;
       }
 #pragma unroll
-      for (int o = 0; o < kMaxOcc; ++o) islot[(s + o) * nthr + tid] = accN[o];
+      for (int o = 0; o < kMaxOcc; ++o) {
+        const int v = wave_sum(accN[o]);
+        if (lane == 0) redI[(s + o) * n_waves + wave] = v;
+        accN[o] = 0;
+      }
       s += kMaxOcc;
-      islot[s * nthr + tid] = accAtt;
-      islot[(s + 1) * nthr + tid] = accAcc;
-      dslot[tid] = accE;
-#pragma unroll
-      for (int a = 0; a < kMaxOrbits; ++a) accC[a] = 0;
-#pragma unroll
-      for (int o = 0; o < kMaxOcc; ++o) accN[o] = 0;
-      accAtt = accAcc = 0;
-      accE = 0.0;
-      __syncthreads();
-      if (tid < n_islot) {
-        long long acc = 0;
-        for (int l = 0; l < nthr; ++l) acc += islot[tid * nthr + l];
-        isum[tid] = (int)acc;
-      } else if (tid == n_islot) {
-        double acc = 0.0;
-        for (int l = 0; l < nthr; ++l) acc += dslot[l];
-        dsum[0] = acc;
+      {
+        const int v0 = wave_sum(accAtt);
+        const int v1 = wave_sum(accAcc);
+        if (lane == 0) {
+          redI[s * n_waves + wave] = v0;
+          redI[(s + 1) * n_waves + wave] = v1;
+        }
+        accAtt = accAcc = 0;
+      }
+      if (!BIN) {
+        const double v = wave_sum(accE);
+        if (lane == 0) redD[wave] = v;
+        accE = 0.0;
       }
       __syncthreads();
       if (tid == 0) {
+        int sumI[kMaxSlots + kMaxOcc + 2];
+        for (int q = 0; q < n_islot; ++q) {
+          int acc = 0;
+          for (int w = 0; w < n_waves; ++w) acc += redI[q * n_waves + w];
+          sumI[q] = acc;
+        }
         int s2 = 0;
         double E;
         if (BIN) {
-          long long *C = args.binC + (long long)chain * n_orbits;
+          long long *C = args.binC + (long long)chain * args.n_orbits;
+          for (int j = 0; j < n_slots; ++j) C[sys->slot_orbit[j]] += sumI[j];
           E = 0.0;
-          for (int a = 0; a < n_orbits; ++a) {
-            C[a] += isum[a];
-            E += sys->bin_w64[a] * (double)C[a];
-          }
-          s2 = n_orbits;
+          for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)C[a];
+          s2 = kMaxSlots;
         } else {
-          E = ch->E + dsum[0];
+          double dsum = 0.0;
+          for (int w = 0; w < n_waves; ++w) dsum += redD[w];
+          E = cs->E + dsum;
         }
-        ch->E = E;
-        for (int o = 0; o < kMaxOcc; ++o) ch->counts[o] += isum[s2 + o];
-        ch->n_attempt += isum[s2 + kMaxOcc];
-        ch->n_accept += isum[s2 + kMaxOcc + 1];
+        cs->E = E;
+        for (int o = 0; o < kMaxOcc; ++o) cs->counts[o] += sumI[s2 + o];
+        cs->n_attempt += sumI[s2 + kMaxOcc];
+        cs->n_accept += sumI[s2 + kMaxOcc + 1];
         if (sample_due) {
           const int idx = args.sample_count[chain];
           if (idx < args.sample_cap) {
             const long long o = (long long)chain * args.sample_cap + idx;
             args.sampE[o] = E;
-            for (int q = 0; q < kMaxOcc; ++q) args.sampN[o * kMaxOcc + q] = (int)ch->counts[q];
+            for (int q = 0; q < kMaxOcc; ++q) args.sampN[o * kMaxOcc + q] = (int)cs->counts[q];
             args.sample_count[chain] = idx + 1;
           }
-          const double n1 = (double)ch->counts[1];
-          ch->sumE += E;
-          ch->sumE2 += E * E;
-          ch->sumN1 += n1;
-          ch->sumN12 += n1 * n1;
-          ch->sumEN1 += E * n1;
-          ch->n_samples_acc += 1;
+          const double n1 = (double)cs->counts[1];
+          cs->sumE += E;
+          cs->sumE2 += E * E;
+          cs->sumN1 += n1;
+          cs->sumN12 += n1 * n1;
+          cs->sumEN1 += E * n1;
+          cs->n_samples_acc += 1;
         }
       }
       __syncthreads();
     }
-    (void)nocc_max;
   }
 
   // ---- write back occupation and RNG state
   {
-    int8_t *occW = args.occ + ch->occ_offset;
-    const int P1 = sc.P1, P12 = sc.P1 * sc.P2, h = sc.h;
-    for (int l = tid; l < sc.n_sites; l += nthr) {
-      int b = l / sc.n_cells;
-      int cc = l - b * sc.n_cells;
-      int i = cc % sc.L1;
-      int j = (cc / sc.L1) % sc.L2;
-      int k = cc / (sc.L1 * sc.L2);
-      occW[l] = img[b * sc.S + (k + h) * P12 + (j + h) * P1 + (i + h)];
+    int8_t *occW = args.occ + cp->occ_offset;
+    for (int l = tid; l < n_sites; l += nthr) {
+      int b = l / n_cells;
+      int cc = l - b * n_cells;
+      int i = cc % L1;
+      int j = (cc / L1) % L2;
+      int k = cc / (L1 * L2);
+      occW[l] = (int8_t)img[b * S + (k + h) * P12 + (j + h) * P1 + (i + h)];
     }
   }
   // all lanes advance by the same bound so the next launch starts fresh
-  __shared__ unsigned long long max_draw;
-  if (tid == 0) max_draw = 0ull;
-  __syncthreads();
-  atomicMax(&max_draw, rng.draw);
+  atomicMax(maxDraw, rng.draw);
   __syncthreads();
   if (tid == 0) {
-    ch->draw_base = max_draw + 1ull;
-    ch->pass_base = pass0 + (unsigned long long)args.n_passes;
+    cs->draw_base = *maxDraw + 1ull;
+    cs->pass_base = pass0 + (unsigned long long)args.n_passes;
   }
 }
 
@@ -472,9 +519,9 @@ __global__ void delta_batch_kernel(DeltaArgs a) {
   if (p >= a.n_prop) return;
   const DevSystem *__restrict__ sys = a.sys;
   const int chain = a.chain[p];
-  const DevChain &ch = a.chains[chain];
-  const DevSupercell &sc = a.scs[ch.sc];
-  const int8_t *__restrict__ occ = a.occ + ch.occ_offset;
+  const ChainParams &cp = a.params[chain];
+  const DevSupercell &sc = a.scs[cp.sc];
+  const int8_t *__restrict__ occ = a.occ + cp.occ_offset;
   const int m = a.sites_per_prop;
   const int *sites = a.sites + p * m;
   const int *nocc = a.new_occ + p * m;
@@ -511,22 +558,24 @@ __global__ void delta_batch_kernel(DeltaArgs a) {
     }
     float d = 0.0f;
     for (int f = 0; f < nf; ++f) d += (sys->site_func[b][f][nw] - sys->site_func[b][f][o]) * G[f];
-    if (a.include_chem) d -= ch.chem[b * kMaxOcc + nw] - ch.chem[b * kMaxOcc + o];
+    if (a.include_chem) d -= cp.chem[b * kMaxOcc + nw] - cp.chem[b * kMaxOcc + o];
     dE += d;
   }
   a.dE[p] = dE;
 }
 
 // ---------------------------------------------------------------------------
-// Full correlations: one block per (function), FP64 deterministic reduction
+// Full correlations: one block per (function, chain), FP64 deterministic
+// reduction
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a) {
   __shared__ double red[256];
   const int f = blockIdx.x;
+  const int chain = a.chain0 + blockIdx.y;
   const DevSystem *__restrict__ sys = a.sys;
-  const DevChain &ch = a.chains[a.chain];
-  const DevSupercell &sc = a.scs[ch.sc];
-  const int8_t *__restrict__ occ = a.occ + ch.occ_offset;
+  const ChainParams &cp = a.params[chain];
+  const DevSupercell &sc = a.scs[cp.sc];
+  const int8_t *__restrict__ occ = a.occ + cp.occ_offset;
   const int orb = sys->func_orbit[f];
   const int k = sys->orbit_k_all[orb];
   const int nf = sys->orbit_nfunc[orb];
@@ -558,7 +607,7 @@ __global__ __launch_bounds__(256) void corr_kernel(CorrArgs a) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.out[f] = red[0] / (double)sys->orbit_mult[orb];
+  if (threadIdx.x == 0) a.out[(long long)blockIdx.y * a.n_corr + f] = red[0] / (double)sys->orbit_mult[orb];
 }
 
 __global__ void philox_kernel(int n, const uint4 *ctr, const uint2 *key, uint4 *out) {
@@ -595,8 +644,8 @@ hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_corr(const CorrArgs &a, int n_corr, hipStream_t st) {
-  hipLaunchKernelGGL(corr_kernel, dim3(n_corr), dim3(256), 0, st, a);
+hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st) {
+  hipLaunchKernelGGL(corr_kernel, dim3(n_corr, n_chains), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -605,13 +654,14 @@ hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, 
   return hipGetLastError();
 }
 
-size_t metropolis_lds_bytes(int pad_bytes, int n_orbits, bool binary, int threads, int max_class) {
+size_t metropolis_lds_bytes(int pad_bytes, int n_tab, bool binary, int threads, int max_class) {
+  (void)threads;
   size_t img = (size_t)((pad_bytes + 15) & ~15);
-  int n_islot = (binary ? n_orbits : 0) + kMaxOcc + 2;
-  size_t slots = (size_t)n_islot * threads * sizeof(int) + (size_t)threads * sizeof(double);
-  size_t sums = (size_t)((n_islot + 1) & ~1) * sizeof(int) + 2 * sizeof(double);
-  size_t scratch = (size_t)kMaxFunc * max_class * sizeof(float) + (size_t)kMaxOrbits * max_class;
-  return img + slots + sums + scratch + 64;
+  size_t tab = binary ? 0 : (size_t)((n_tab * 4 + 15) & ~15);
+  const int n_islot = (binary ? kMaxSlots : 0) + kMaxOcc + 2;
+  size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16;
+  size_t scratch = (size_t)kMaxFunc * max_class * sizeof(float) + (size_t)kMaxSlots * max_class;
+  return img + tab + red + scratch + 64;
 }
 
 }  // namespace clexmc
