@@ -1,0 +1,155 @@
+"""Statistical parity of the GPU Metropolis path (3 sigma).
+
+* exact enumeration (16-site FCC supercell, binary CE with pairs to the 2nd
+  shell and nearest-neighbour triplets): Boltzmann averages of E, E^2, N_B,
+  N_B^2 in the semi-grand canonical and canonical ensembles;
+* the CPU oracle's restatement of the reference loop (random-site
+  proposals, mt19937_64) at the 12x12x12 canonical workload: mean formation
+  energy per unit cell.
+
+Errors are standard errors over independent chains (GPU) or autocorrelation-
+corrected (oracle); agreement is required within 3 sigma of the combined
+error.  Runs use fixed seeds, so outcomes are reproducible.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from casmcode_clexmonte_amd.analysis import mean_and_error
+from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.system import BasisSet, ClusterExpansion, FlatCE, Supercell, fcc_prim
+from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
+
+pytestmark = pytest.mark.gpu
+
+NSIG = 3.0
+
+
+def small_flat(seed=0):
+    prim = fcc_prim()
+    bs = BasisSet(prim, {2: 1.05, 3: 0.75}, "chebychev")
+    eci = np.random.default_rng(seed).normal(0, 0.05, bs.n_functions)
+    return FlatCE(ClusterExpansion(bs, eci))
+
+
+def _chain_means(E, N1, burn):
+    e = E[:, burn:]
+    n = N1[:, burn:].astype(float)
+    return {"E": e.mean(1), "E2": (e * e).mean(1), "n1": n.mean(1), "n12": (n * n).mean(1)}
+
+
+def _check(per_chain, exact, keys):
+    for k in keys:
+        v = per_chain[k]
+        m = v.mean()
+        se = v.std(ddof=1) / np.sqrt(len(v))
+        assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
+
+
+def test_sgc_exact_enumeration():
+    print("[case] sgc exact enumeration 2x2x4", flush=True)
+    flat = small_flat()
+    sc = Supercell(flat.ce.prim, (2, 2, 4))
+    T, mu = 1200.0, 0.03
+    chem = [0.0, mu]
+    exact = O.OracleSystem(flat, sc.dims).enumerate("sgc", T, chem=chem)
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    occs = [random_occupation(sc.n_sites, 0.5, 100 + c, exact=False) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 1000 + np.arange(nch), chem=[chem] * nch)
+    eng.run_passes("semigrand_canonical", 4000, 1)
+    E, cnt = eng.read_samples()
+    _check(_chain_means(E, cnt[:, :, 1], 200), exact, ["E", "E2", "n1", "n12"])
+    eng.close()
+
+
+def test_canonical_exact_enumeration():
+    print("[case] canonical exact enumeration 2x2x4", flush=True)
+    flat = small_flat()
+    sc = Supercell(flat.ce.prim, (2, 2, 4))
+    T = 1200.0
+    nb = 8
+    exact = O.OracleSystem(flat, sc.dims).enumerate("canonical", T, fixed_counts=[sc.n_sites - nb, nb])
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=64, seed=5)
+    occs = [random_occupation(sc.n_sites, nb / sc.n_sites, 300 + c) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 2000 + np.arange(nch))
+    eng.run_passes("canonical", 6000, 1, canonical_rounds=2)
+    E, cnt = eng.read_samples()
+    assert np.all(cnt[:, :, 1] == nb)
+    _check(_chain_means(E, cnt[:, :, 1], 300), exact, ["E", "E2"])
+    eng.close()
+
+
+def test_canonical_workload_vs_oracle():
+    """12x12x12 FCC, x_B = 0.5, T = 600 K (the reference's CPU-runnable
+    canonical case): GPU chains versus the oracle's restatement of the
+    reference loop."""
+    print("[case] canonical 12^3 vs oracle", flush=True)
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    T = 600.0
+    nc = sc.n_cells
+    # oracle: 2 chains, 300 equilibration + 1200 sampled passes
+    osys = O.OracleSystem(flat, sc.dims)
+    o_means, o_vars = [], []
+    for c in range(2):
+        r = osys.run(random_occupation(sc.n_sites, 0.5, 40 + c), "canonical", T, 300, 1200, 77 + c,
+                     full_recompute_samplers=False)
+        m, e = mean_and_error(r["E"] / nc)
+        o_means.append(m)
+        o_vars.append(e * e)
+    o_mean = float(np.mean(o_means))
+    o_err = float(np.sqrt(np.sum(o_vars)) / len(o_means))
+    # GPU: 32 chains, 300 + 1200 passes (canonical matching rounds: 2)
+    nch = 32
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=16, seed=3)
+    occs = [random_occupation(sc.n_sites, 0.5, 500 + c) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 3000 + np.arange(nch))
+    eng.run_passes("canonical", 1500, 1, canonical_rounds=2)
+    E, cnt = eng.read_samples()
+    per_chain = E[:, 300:].mean(1) / nc
+    g_mean = float(per_chain.mean())
+    g_err = float(per_chain.std(ddof=1) / np.sqrt(nch))
+    tol = NSIG * np.sqrt(g_err ** 2 + o_err ** 2)
+    print(f"oracle {o_mean:.6f} +- {o_err:.6f}  gpu {g_mean:.6f} +- {g_err:.6f}")
+    assert abs(g_mean - o_mean) <= tol, (g_mean, o_mean, tol)
+    eng.close()
+
+
+def test_sgc_workload_point_vs_oracle():
+    """Semi-grand canonical flips at one (T, mu) grid point on a 12^3 cell."""
+    print("[case] sgc 12^3 vs oracle", flush=True)
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    T, mu = 900.0, -0.05
+    chem = [0.0, mu]
+    nc = sc.n_cells
+    osys = O.OracleSystem(flat, sc.dims)
+    oE, oX = [], []
+    for c in range(2):
+        r = osys.run(random_occupation(sc.n_sites, 0.5, 60 + c, exact=False), "sgc", T, 200, 1000, 91 + c,
+                     chem=chem, full_recompute_samplers=False)
+        oE.append(mean_and_error(r["E"] / nc))
+        oX.append(mean_and_error(r["counts"][:, 1] / nc))
+    nch = 32
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    occs = [random_occupation(sc.n_sites, 0.5, 700 + c, exact=False) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 4000 + np.arange(nch), chem=[chem] * nch)
+    eng.run_passes("semigrand_canonical", 1200, 1)
+    E, cnt = eng.read_samples()
+    for name, gpu_series, orc in (("E", E[:, 200:] / nc, oE), ("x", cnt[:, 200:, 1] / nc, oX)):
+        per_chain = gpu_series.mean(1)
+        g_mean, g_err = per_chain.mean(), per_chain.std(ddof=1) / np.sqrt(nch)
+        o_mean = np.mean([m for m, _ in orc])
+        o_err = np.sqrt(np.sum([e * e for _, e in orc])) / len(orc)
+        tol = NSIG * np.sqrt(g_err ** 2 + o_err ** 2)
+        print(f"{name}: oracle {o_mean:.6f} +- {o_err:.6f}  gpu {g_mean:.6f} +- {g_err:.6f}")
+        assert abs(g_mean - o_mean) <= tol, (name, g_mean, o_mean, tol)
+    eng.close()
