@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <numeric>
+#include <set>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -233,6 +236,172 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   return CLEXMC_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Independent-set partition computed by the library (clexmc_add_supercell with
+// n_classes == 0): periodic colouring by residues of the cell coordinates when
+// one exists (equal class sizes), otherwise a greedy colouring; plus
+// canonical cross-class pair phases (pairs (s, s+v) for random translations
+// v, greedily kept so that no cluster contains sites of two pairs and the two
+// sites of a pair share no cluster).  Selection never looks at occupations.
+// ---------------------------------------------------------------------------
+static uint64_t splitmix64(uint64_t &x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair_req, uint64_t seed,
+                                    std::vector<int> &cb, std::vector<int> &cs, std::vector<int> &pb,
+                                    std::vector<int> &ps) {
+  const int nb = h->n_sublat;
+  const long nc = (long)L[0] * L[1] * L[2];
+  const long N = nc * nb;
+  auto lin = [&](int b, int i, int j, int k) {
+    i = ((i % L[0]) + L[0]) % L[0];
+    j = ((j % L[1]) + L[1]) % L[1];
+    k = ((k % L[2]) + L[2]) % L[2];
+    return (long)b * nc + i + (long)L[0] * (j + (long)L[1] * k);
+  };
+  // interaction offsets per sublattice
+  std::vector<std::vector<std::array<int, 4>>> offs(nb);
+  for (int b = 0; b < nb; ++b) {
+    std::set<std::array<int, 4>> u;
+    for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e)
+      for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q)
+        u.insert({h->entry_sites[4 * q], h->entry_sites[4 * q + 1], h->entry_sites[4 * q + 2], h->entry_sites[4 * q + 3]});
+    offs[b].assign(u.begin(), u.end());
+  }
+  // clusters must not wrap onto themselves
+  for (int b = 0; b < nb; ++b)
+    for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+      std::vector<long> sites = {lin(b, 0, 0, 0)};
+      for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q)
+        sites.push_back(lin(h->entry_sites[4 * q], h->entry_sites[4 * q + 1], h->entry_sites[4 * q + 2],
+                            h->entry_sites[4 * q + 3]));
+      std::sort(sites.begin(), sites.end());
+      ARG_CHECK(std::adjacent_find(sites.begin(), sites.end()) == sites.end(),
+                "supercell too small for the cluster expansion: a cluster wraps onto itself");
+    }
+  // periodic colouring
+  long best_n = -1;
+  int bm[3] = {0, 0, 0};
+  for (int m1 = 1; m1 <= L[0]; ++m1) {
+    if (L[0] % m1) continue;
+    for (int m2 = 1; m2 <= L[1]; ++m2) {
+      if (L[1] % m2) continue;
+      for (int m3 = 1; m3 <= L[2]; ++m3) {
+        if (L[2] % m3) continue;
+        bool ok = true;
+        for (int b = 0; b < nb && ok; ++b)
+          for (auto const &o : offs[b])
+            if (o[0] == b && o[1] % m1 == 0 && o[2] % m2 == 0 && o[3] % m3 == 0) {
+              ok = false;
+              break;
+            }
+        long n = (long)m1 * m2 * m3;
+        if (ok && (best_n < 0 || n < best_n)) {
+          best_n = n;
+          bm[0] = m1;
+          bm[1] = m2;
+          bm[2] = m3;
+        }
+      }
+    }
+  }
+  cb.assign(1, 0);
+  cs.clear();
+  if (best_n > 0) {
+    for (int b = 0; b < nb; ++b) {
+      if (h->n_occ[b] < 2) continue;
+      for (int r3 = 0; r3 < bm[2]; ++r3)
+        for (int r2 = 0; r2 < bm[1]; ++r2)
+          for (int r1 = 0; r1 < bm[0]; ++r1) {
+            for (int k = r3; k < L[2]; k += bm[2])
+              for (int j = r2; j < L[1]; j += bm[1])
+                for (int i = r1; i < L[0]; i += bm[0]) cs.push_back((int)lin(b, i, j, k));
+            cb.push_back((int)cs.size());
+          }
+    }
+  } else {
+    std::vector<std::set<long>> nbr(N);
+    for (long l = 0; l < N; ++l) {
+      int b = (int)(l / nc);
+      if (h->n_occ[b] < 2) continue;
+      long c = l % nc;
+      int i = (int)(c % L[0]), j = (int)((c / L[0]) % L[1]), k = (int)(c / ((long)L[0] * L[1]));
+      for (auto const &o : offs[b]) {
+        long t = lin(o[0], i + o[1], j + o[2], k + o[3]);
+        if (t != l) {
+          nbr[l].insert(t);
+          nbr[t].insert(l);
+        }
+      }
+    }
+    std::vector<int> color(N, -1);
+    int maxc = -1;
+    for (long l = 0; l < N; ++l) {
+      if (h->n_occ[l / nc] < 2) continue;
+      std::set<int> used;
+      for (long t : nbr[l]) used.insert(color[t]);
+      int c = 0;
+      while (used.count(c)) ++c;
+      color[l] = c;
+      maxc = std::max(maxc, c);
+    }
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c <= maxc; ++c) {
+        size_t before = cs.size();
+        for (long l = b * nc; l < (b + 1) * nc; ++l)
+          if (color[l] == c) cs.push_back((int)l);
+        if (cs.size() > before) cb.push_back((int)cs.size());
+      }
+  }
+  // pair phases
+  pb.assign(1, 0);
+  ps.clear();
+  std::vector<int> var;
+  for (int b = 0; b < nb; ++b)
+    if (h->n_occ[b] >= 2) var.push_back(b);
+  uint64_t st = seed ^ 0xC0FFEEull;
+  int tries = 0;
+  std::vector<char> blocked(N);
+  std::vector<long> order(nc);
+  while ((int)pb.size() - 1 < n_pair_req && tries < 64 * std::max(1, n_pair_req)) {
+    ++tries;
+    int b = var[splitmix64(st) % var.size()];
+    int v[3] = {(int)(splitmix64(st) % L[0]), (int)(splitmix64(st) % L[1]), (int)(splitmix64(st) % L[2])};
+    if (v[0] == 0 && v[1] == 0 && v[2] == 0) continue;
+    bool bad = false;
+    for (auto const &o : offs[b])
+      if (o[0] == b && ((o[1] - v[0]) % L[0]) == 0 && ((o[2] - v[1]) % L[1]) == 0 && ((o[3] - v[2]) % L[2]) == 0) {
+        bad = true;
+        break;
+      }
+    if (bad) continue;
+    std::fill(blocked.begin(), blocked.end(), 0);
+    std::iota(order.begin(), order.end(), 0);
+    for (long i = nc - 1; i > 0; --i) std::swap(order[i], order[splitmix64(st) % (uint64_t)(i + 1)]);
+    size_t before = ps.size();
+    for (long c : order) {
+      int i = (int)(c % L[0]), j = (int)((c / L[0]) % L[1]), k = (int)(c / ((long)L[0] * L[1]));
+      long s = lin(b, i, j, k), t = lin(b, i + v[0], j + v[1], k + v[2]);
+      if (blocked[s] || blocked[t]) continue;
+      ps.push_back((int)s);
+      ps.push_back((int)t);
+      blocked[s] = blocked[t] = 1;
+      for (long x : {s, t}) {
+        long cx = x % nc;
+        int ii = (int)(cx % L[0]), jj = (int)((cx / L[0]) % L[1]), kk = (int)(cx / ((long)L[0] * L[1]));
+        for (auto const &o : offs[b]) blocked[lin(o[0], ii + o[1], jj + o[2], kk + o[3])] = 1;
+      }
+    }
+    if (ps.size() > before) pb.push_back((int)(ps.size() / 2));
+  }
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc *sc, int *id_out) {
   if (!h || !sc || !id_out) return CLEXMC_ERR_ARG;
   HIP_TRY(hipSetDevice(h->device));
@@ -292,7 +461,22 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
           if (a != main) out.push_back(a);
         }
   };
-  // classes
+  // classes (given, or computed here when n_classes == 0)
+  clexmc_supercell_desc auto_desc;
+  std::vector<int> a_cb, a_cs, a_pb, a_ps;
+  if (sc->n_classes == 0) {
+    const int L[3] = {L1, L2, L3};
+    clexmc_status pst = auto_partition(h, L, sc->n_pair_phases, 0x5eedull + h->scs.size(), a_cb, a_cs, a_pb, a_ps);
+    if (pst != CLEXMC_OK) return pst;
+    auto_desc = *sc;
+    auto_desc.n_classes = (int)a_cb.size() - 1;
+    auto_desc.class_begin = a_cb.data();
+    auto_desc.class_sites = a_cs.data();
+    auto_desc.n_pair_phases = (int)a_pb.size() - 1;
+    auto_desc.pair_phase_begin = a_pb.data();
+    auto_desc.pair_sites = a_ps.data();
+    sc = &auto_desc;
+  }
   ARG_CHECK(sc->n_classes >= 1, "no independent-set classes");
   std::vector<int> cb(sc->class_begin, sc->class_begin + sc->n_classes + 1);
   ARG_CHECK(cb[0] == 0, "class_begin[0] must be 0");

@@ -929,3 +929,107 @@ def parse_casm_function_latex(s: str) -> List[Tuple[float, Dict[int, int]]]:
             d[int(fm.group(3))] = int(fm.group(2))
         terms.append((coeff, d))
     return terms
+
+
+# ---------------------------------------------------------------------------
+# Composition axes and the flat file read by the C++ calculators
+# ---------------------------------------------------------------------------
+class CompositionAxes:
+    """Standard parametric composition axes: origin = every site holding its
+    first occupant, end member i = every variable site holding component
+    i+1 (where allowed).  x = R^+ (n - n0) with n the number of each
+    component per unit cell (the role of the reference's
+    CompositionConverter, call sites SemiGrandCanonicalCalculator.cc:144,
+    172-178)."""
+
+    def __init__(self, prim: Prim):
+        self.components = prim.components()
+        nc = len(self.components)
+        self.n_sublat = prim.n_sublat
+        self.occ_component = []
+        for b in range(prim.n_sublat):
+            self.occ_component.append([self.components.index(s) for s in prim.occupants[b]])
+        origin = np.zeros(nc)
+        for b in range(prim.n_sublat):
+            origin[self.occ_component[b][0]] += 1
+        ends = []
+        for c in range(1, nc):
+            e = np.zeros(nc)
+            for b in range(prim.n_sublat):
+                if c in self.occ_component[b]:
+                    e[c] += 1
+                else:
+                    e[self.occ_component[b][0]] += 1
+            if not np.allclose(e, origin):
+                ends.append(e)
+        self.origin = origin
+        self.R = np.array(ends).T if ends else np.zeros((nc, 0))  # columns: end - origin
+        self.R = self.R - origin[:, None] if ends else self.R
+        self.Rpinv = np.linalg.pinv(self.R) if ends else np.zeros((0, nc))
+
+    @property
+    def n_independent(self) -> int:
+        return self.R.shape[1]
+
+    def mol_composition(self, occ: np.ndarray, n_cells: int) -> np.ndarray:
+        n = np.zeros(len(self.components))
+        occ = np.asarray(occ)
+        for b in range(self.n_sublat):
+            sub = occ[b * n_cells:(b + 1) * n_cells]
+            cnt = np.bincount(sub, minlength=len(self.occ_component[b]))
+            for o, c in enumerate(self.occ_component[b]):
+                n[c] += cnt[o]
+        return n / n_cells
+
+    def param_composition(self, mol: np.ndarray) -> np.ndarray:
+        return self.Rpinv @ (np.asarray(mol) - self.origin)
+
+    def chem_weights(self, param_chem_pot) -> np.ndarray:
+        """Per (sublattice, occupant) weights w such that the semi-grand
+        potential change of an event is dE - sum (w[new] - w[old]):
+        w_s = (mu^T R^+)_component(s)."""
+        mu = np.asarray(param_chem_pot, dtype=float)
+        wc = mu @ self.Rpinv if self.n_independent else np.zeros(len(self.components))
+        nom = max(len(o) for o in self.occ_component)
+        w = np.zeros((self.n_sublat, nom))
+        for b in range(self.n_sublat):
+            for o, c in enumerate(self.occ_component[b]):
+                w[b, o] = wc[c]
+        return w
+
+
+def save_flat(flat: FlatCE, path: str) -> None:
+    """Write the flattened cluster expansion and composition axes as text
+    lines ``name count v0 v1 ...`` (read by the C++ System)."""
+    prim = flat.ce.prim
+    axes = CompositionAxes(prim)
+    nom = flat.n_occ_max
+    occ_comp = np.full((prim.n_sublat, nom), -1, dtype=int)
+    for b in range(prim.n_sublat):
+        occ_comp[b, :len(axes.occ_component[b])] = axes.occ_component[b]
+    items = [
+        ("n_sublat", [flat.n_sublat]), ("n_occ", flat.n_occ), ("n_occ_max", [flat.n_occ_max]),
+        ("n_func_max", [flat.n_func_max]), ("site_func", flat.site_func.ravel()),
+        ("n_orbits", [flat.n_orbits]), ("orbit_k", flat.orbit_k), ("orbit_mult", flat.orbit_mult),
+        ("orbit_nfunc", flat.orbit_nfunc), ("orbit_func_begin", flat.orbit_func_begin),
+        ("corr_orbit_begin", flat.corr_orbit_begin), ("corr_elem_begin", flat.corr_elem_begin),
+        ("corr_elem_sites", flat.corr_elem_sites.ravel()), ("orbit_table_offset", flat.orbit_table_offset),
+        ("orbit_tables", flat.orbit_tables), ("n_corr", [flat.n_corr]), ("eci", flat.eci),
+        ("flower_begin", flat.flower_begin), ("entry_orbit", flat.entry_orbit), ("entry_k", flat.entry_k),
+        ("entry_site_begin", flat.entry_site_begin), ("entry_sites", flat.entry_sites.ravel()),
+        ("entry_table", flat.entry_table), ("flower_tables", flat.flower_tables),
+        ("binary", [1 if flat.binary else 0]),
+        ("n_components", [len(axes.components)]), ("occ_component", occ_comp.ravel()),
+        ("comp_origin", axes.origin), ("comp_R", axes.R.T.ravel()), ("n_independent", [axes.n_independent]),
+    ]
+    if flat.binary:
+        items += [("bin_seg_begin", flat.bin_seg_begin), ("bin_segs", flat.bin_segs.ravel()),
+                  ("bin_orbit_weight", flat.bin_orbit_weight)]
+    with open(path, "w") as f:
+        for name, vals in items:
+            vals = np.asarray(vals).ravel()
+            if vals.dtype.kind == "f":
+                body = " ".join(repr(float(v)) for v in vals)
+            else:
+                body = " ".join(str(int(v)) for v in vals)
+            f.write(f"{name} {vals.size} {body}\n")

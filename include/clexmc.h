@@ -97,7 +97,10 @@ typedef struct {
 } clexmc_system_desc;
 
 /* Supercell (diagonal, L1 x L2 x L3 primitive cells) with its independent-set
- * partition: sites of one class share no cluster. */
+ * partition: sites of one class share no cluster.  With n_classes == 0 the
+ * library computes the partition itself (periodic colouring by residues of
+ * the cell coordinates, or a greedy colouring) and n_pair_phases canonical
+ * mixing phases; class/pair arrays are then ignored. */
 typedef struct {
   int L[3];
   int n_classes;

@@ -1,0 +1,100 @@
+"""The Python calculator face (MonteCalculator / MontePotential /
+run_fixture) on the GPU: potential values against the oracle and the
+summary layout the reference's python tests check
+(python/tests/conftest.py:140-259, validate_summary_file)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from casmcode_clexmonte_amd.calculator import (MonteCalculator, MonteCarloState, MontePotential, System,
+                                               merge_summaries)
+from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
+
+
+def validate_summary_data(subdata, expected_keys, expected_size):
+    # restates the reference's conftest.py validate_summary_data (:140-154)
+    for x in expected_keys:
+        assert x in subdata
+        if "component_names" in subdata[x]:
+            for y in subdata[x]["component_names"]:
+                assert len(subdata[x][y]) == expected_size
+        elif "value" in subdata[x]:
+            assert subdata[x]["shape"] == []
+            assert len(subdata[x]["value"]) == expected_size
+        else:
+            assert len(subdata[x]) == expected_size
+
+
+def validate_statistics_data(subdata, expected_keys, expected_size):
+    for x in expected_keys:
+        assert x in subdata
+        if "component_names" in subdata[x]:
+            for y in subdata[x]["component_names"]:
+                for z in ["mean", "calculated_precision"]:
+                    assert len(subdata[x][y][z]) == expected_size
+        else:
+            assert subdata[x]["shape"] == []
+            for z in ["mean", "calculated_precision"]:
+                assert len(subdata[x]["value"][z]) == expected_size
+
+
+def validate_summary(data, expected_size, is_canonical):
+    if is_canonical:
+        cond_keys, an_keys = ["temperature", "param_composition", "mol_composition"], ["heat_capacity"]
+    else:
+        cond_keys = ["temperature", "param_chem_pot"]
+        an_keys = ["heat_capacity", "mol_susc", "param_susc", "mol_thermochem_susc", "param_thermochem_susc"]
+    validate_summary_data(data["analysis"], an_keys, expected_size)
+    validate_summary_data(data["completion_check_results"],
+                          ["N_samples", "N_samples_for_statistics", "acceptance_rate", "count", "elapsed_clocktime",
+                           "N_samples_for_all_to_equilibrate", "all_converged", "all_equilibrated"], expected_size)
+    validate_summary_data(data["conditions"], cond_keys, expected_size)
+    validate_statistics_data(data["statistics"], ["clex.formation_energy", "mol_composition", "param_composition",
+                                                  "potential_energy"], expected_size)
+
+
+@pytest.fixture(scope="module")
+def system():
+    return System(fcc_binary_ce())
+
+
+@pytest.mark.gpu
+def test_python_potential_and_canonical_series(system):
+    print("[case] python calculator canonical", flush=True)
+    calc = MonteCalculator("canonical", system)
+    dims = (6, 6, 6)
+    occ = random_occupation(216, 0.5, 2)
+    state = MonteCarloState(dims, occ.copy(), {"temperature": 900.0})
+    pot = MontePotential(calc, state)
+    osys = O.OracleSystem(system.flat, dims)
+    assert abs(pot.per_supercell() - osys.energy(occ)) < 1e-9
+    d = pot.occ_delta_per_supercell([3, 17], [int(occ[17]), int(occ[3])])
+    ref = osys.delta(occ, [3, 17], [int(occ[17]), int(occ[3])])
+    assert abs(d - ref) <= 1e-5 * max(abs(ref), 0.01) + 1e-6
+    summaries = []
+    for T in (1200.0, 900.0, 600.0):
+        state.conditions["temperature"] = T
+        p = calc.make_default_sampling_fixture_params(n_equilibration_passes=20, n_passes=100)
+        summaries.append(calc.run_fixture(state, p))
+        assert int(state.occupation.sum()) == int(occ.sum())
+    validate_summary(merge_summaries(summaries), 3, True)
+
+
+@pytest.mark.gpu
+def test_python_sgc_multistate(system):
+    print("[case] python calculator sgc multistate", flush=True)
+    calc = MonteCalculator("semigrand_canonical", system)
+    states = [MonteCarloState((8, 8, 8), random_occupation(512, 0.5, 10 + i, exact=False),
+                              {"temperature": 1000.0, "param_chem_pot": [mu]})
+              for i, mu in enumerate(np.linspace(-0.1, 0.1, 8))]
+    p = calc.make_default_sampling_fixture_params(n_equilibration_passes=50, n_passes=200)
+    out = calc.run_multistate(states, p)
+    validate_summary(merge_summaries(out), 8, False)
+    x = [o["statistics"]["param_composition"]["a"]["mean"][0] for o in out]
+    assert np.all(np.diff(x) > 0)  # composition increases with the B chemical potential
+
+
+def test_calculator_rejects_bad_conditions(system):
+    """Condition validation happens before any device call."""
+    with pytest.raises(Exception):
+        MonteCalculator("grand_canonical_typo", system)
