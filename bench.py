@@ -66,7 +66,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-passes", type=int, default=40)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="copy N MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this run
+        # calibrate the counters read on the Metropolis kernel
+        import torch
+        n = args.calibrate_copy << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": n}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
         sys.exit(2)
