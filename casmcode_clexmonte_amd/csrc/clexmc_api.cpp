@@ -65,6 +65,16 @@ struct clexmc_handle {
   int block_size = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  // learned proposal
+  int n_env = 0;
+  std::vector<int4> env_sites;      // [n_sublat][kEnvMax]
+  int4 *d_env_sites = nullptr;
+  MlpDev *d_mlp = nullptr;
+  float *d_mlp_params = nullptr, *d_adam_m = nullptr, *d_adam_v = nullptr;
+  long long adam_t = 0;
+  float mlp_eps = 0.05f;
+  int proposal = 0;                 // 0 plain, 1 learned (MLP), 2 grouped uniform
+  int rounds_per_group = 64;
 };
 
 #define HIP_TRY(expr)                                                              \
@@ -107,6 +117,21 @@ static void free_all(std::vector<void *> &v) {
   for (void *p : v) hipFree(p);
   v.clear();
 }
+
+struct DevBufs {
+  std::vector<void *> p;
+  ~DevBufs() {
+    for (void *x : p) hipFree(x);
+  }
+  template <typename T>
+  hipError_t alloc(T **out, size_t n) {
+    void *x = nullptr;
+    hipError_t e = hipMalloc(&x, sizeof(T) * std::max<size_t>(n, 1));
+    if (e == hipSuccess) p.push_back(x);
+    *out = (T *)x;
+    return e;
+  }
+};
 
 extern "C" {
 
@@ -217,6 +242,39 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
       h->bin_w[a] = s->bin_orbit_weight[a];
       if (s->orbit_k[a] >= 1) d.slot_orbit[d.n_slots++] = a;
     }
+  }
+  // environment of the learned proposal: sites of the three shortest pair orbits
+  {
+    std::vector<int> pair_orbits;
+    for (int a = 0; a < s->n_orbits && pair_orbits.size() < 3; ++a)
+      if (s->orbit_k[a] == 2) pair_orbits.push_back(a);
+    h->env_sites.assign((size_t)s->n_sublat * kEnvMax, make_int4(0, 0, 0, 0));
+    int n_env = -1;
+    for (int b = 0; b < s->n_sublat; ++b) {
+      if (s->n_occ[b] < 2) continue;
+      int cnt = 0;
+      for (int a : pair_orbits)
+        for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+          if (h->entry_orbit[e] != a) continue;
+          const int q = h->entry_site_begin[e];
+          if (cnt < kEnvMax)
+            h->env_sites[(size_t)b * kEnvMax + cnt] = make_int4(h->entry_sites[4 * q], h->entry_sites[4 * q + 1],
+                                                                h->entry_sites[4 * q + 2], h->entry_sites[4 * q + 3]);
+          cnt++;
+        }
+      cnt = std::min(cnt, kEnvMax);
+      n_env = n_env < 0 ? cnt : std::min(n_env, cnt);
+    }
+    h->n_env = std::max(0, n_env);
+    const int np = kHidden * kEnvMax + 2 * kHidden + 1;
+    HIP_TRY(upload(h, A, h->env_sites.data(), h->env_sites.size(), &h->d_env_sites));
+    HIP_TRY(upload(h, A, (const MlpDev *)nullptr, 1, &h->d_mlp));
+    HIP_TRY(upload(h, A, (const float *)nullptr, (size_t)np, &h->d_mlp_params));
+    HIP_TRY(upload(h, A, (const float *)nullptr, (size_t)np, &h->d_adam_m));
+    HIP_TRY(upload(h, A, (const float *)nullptr, (size_t)np, &h->d_adam_v));
+    // zero network (w = eps + 1/2 everywhere) until parameters are set or trained
+    HIP_TRY(launch_mlp_publish(h->d_mlp_params, h->d_mlp, h->n_env, h->mlp_eps, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
   }
   // correlations
   std::vector<int> func_orbit(s->n_corr, 0);
@@ -571,6 +629,14 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       }
     }
   }
+  // environment offsets of the learned proposal
+  std::vector<int> env_off((size_t)h->n_sublat * kEnvMax, 0);
+  for (int b = 0; b < h->n_sublat; ++b)
+    for (int k = 0; k < h->n_env; ++k) {
+      const int4 e = h->env_sites[(size_t)b * kEnvMax + k];
+      env_off[(size_t)b * kEnvMax + k] =
+          (int)((long long)(e.x - b) * S + (long long)e.w * d.P1 * d.P2 + (long long)e.z * d.P1 + e.y);
+    }
   // general site programs: per cluster k, table offset, then (offset, radix)
   std::vector<GenSub> gen_sub(h->n_sublat);
   std::vector<int> gen_prog;
@@ -624,6 +690,7 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, bin_off.data(), bin_off.size(), (int **)&d.bin_off));
   HIP_TRY(upload(h, A, gen_sub.data(), gen_sub.size(), (GenSub **)&d.gen_sub));
   HIP_TRY(upload(h, A, gen_prog.data(), gen_prog.size(), (int **)&d.gen_prog));
+  HIP_TRY(upload(h, A, env_off.data(), env_off.size(), (int **)&d.env_off));
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
   HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
@@ -913,8 +980,25 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   a.rounds = std::max(1, canonical_rounds);
   a.n_orbits = h->n_orbits;
   a.n_tab = n_tab;
-  HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
+  if (h->proposal != 0) {
+    if (!h->binary) {
+      h->err = "the learned proposal is implemented for binary (+-1) cluster expansions";
+      return CLEXMC_ERR_UNSUPPORTED;
+    }
+    lds = learned_lds_bytes(max_pad);
+    if ((long long)lds > (long long)max_lds - 64) {
+      h->err = "chain image does not fit in LDS";
+      return CLEXMC_ERR_UNSUPPORTED;
+    }
+    LearnArgs la;
+    la.mlp = h->d_mlp;
+    la.rounds_per_group = h->rounds_per_group;
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    HIP_TRY(launch_learned(a, la, kind, h->proposal == 1, h->n_chains, threads, lds, h->stream));
+  } else {
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
+  }
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -1006,6 +1090,214 @@ clexmc_status clexmc_philox_device(clexmc_handle *h, int n, const uint32_t *ctr,
     h->err = std::string("philox: ") + hipGetErrorString(e);
     return CLEXMC_ERR_HIP;
   }
+  return CLEXMC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned proposal
+// ---------------------------------------------------------------------------
+static const int kNParams = kHidden * kEnvMax + 2 * kHidden + 1;
+
+clexmc_status clexmc_set_proposal(clexmc_handle *h, int proposal, int rounds_per_group) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(proposal >= 0 && proposal <= 2, "proposal must be 0 (plain), 1 (learned) or 2 (grouped uniform)");
+  ARG_CHECK(rounds_per_group >= 1 && rounds_per_group <= 4096, "rounds_per_group out of range");
+  h->proposal = proposal;
+  h->rounds_per_group = rounds_per_group;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_info(clexmc_handle *h, int *n_env, int *n_params, int *env_sites) {
+  if (!h) return CLEXMC_ERR_ARG;
+  if (n_env) *n_env = h->n_env;
+  if (n_params) *n_params = kNParams;
+  if (env_sites)
+    for (size_t i = 0; i < h->env_sites.size(); ++i) {
+      env_sites[4 * i] = h->env_sites[i].x;
+      env_sites[4 * i + 1] = h->env_sites[i].y;
+      env_sites[4 * i + 2] = h->env_sites[i].z;
+      env_sites[4 * i + 3] = h->env_sites[i].w;
+    }
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_set_params(clexmc_handle *h, const float *params, float eps) {
+  if (!h || !params) return CLEXMC_ERR_ARG;
+  ARG_CHECK(eps > 0.0f && eps < 1e3f, "eps must be positive");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpy(h->d_mlp_params, params, sizeof(float) * kNParams, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(h->d_adam_m, 0, sizeof(float) * kNParams));
+  HIP_TRY(hipMemset(h->d_adam_v, 0, sizeof(float) * kNParams));
+  h->adam_t = 0;
+  h->mlp_eps = eps;
+  HIP_TRY(launch_mlp_publish(h->d_mlp_params, h->d_mlp, h->n_env, eps, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_get_params(clexmc_handle *h, float *params) {
+  if (!h || !params) return CLEXMC_ERR_ARG;
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpy(params, h->d_mlp_params, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_forward(clexmc_handle *h, int n, const int8_t *X, float *out) {
+  if (!h || !X || !out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n >= 1, "n >= 1 required");
+  HIP_TRY(hipSetDevice(h->device));
+  int8_t *dX = nullptr;
+  float *dO = nullptr;
+  HIP_TRY(hipMalloc(&dX, (size_t)n * kEnvMax));
+  HIP_TRY(hipMalloc(&dO, sizeof(float) * 2 * (size_t)n));
+  hipError_t e = hipMemcpy(dX, X, (size_t)n * kEnvMax, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_mlp_infer(h->d_mlp, n, dX, dO, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dO, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost);
+  hipFree(dX);
+  hipFree(dO);
+  if (e != hipSuccess) {
+    h->err = std::string("mlp_forward: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  return CLEXMC_OK;
+}
+
+static TrainArgs train_args(clexmc_handle *h, int n, const int8_t *dX, const float *dy, float *dH, float *ddl,
+                            float *dloss, float *dgrad) {
+  TrainArgs t;
+  t.n = n;
+  t.n_env = h->n_env;
+  t.X = dX;
+  t.y = dy;
+  t.W1 = h->d_mlp_params;
+  t.b1 = h->d_mlp_params + kHidden * kEnvMax;
+  t.W2 = h->d_mlp_params + kHidden * kEnvMax + kHidden;
+  t.b2 = h->d_mlp_params + kHidden * kEnvMax + 2 * kHidden;
+  t.H = dH;
+  t.dl = ddl;
+  t.loss = dloss;
+  t.grad = dgrad;
+  return t;
+}
+
+
+
+clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, float *grad_out,
+                                   float *loss_out) {
+  if (!h || !X || !y || !grad_out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n >= 1, "n >= 1 required");
+  HIP_TRY(hipSetDevice(h->device));
+  DevBufs B;
+  int8_t *dX;
+  float *dy, *dH, *ddl, *dloss, *dg;
+  HIP_TRY(B.alloc(&dX, (size_t)n * kEnvMax));
+  HIP_TRY(B.alloc(&dy, n));
+  HIP_TRY(B.alloc(&dH, (size_t)n * kHidden));
+  HIP_TRY(B.alloc(&ddl, n));
+  HIP_TRY(B.alloc(&dloss, n));
+  HIP_TRY(B.alloc(&dg, kNParams));
+  HIP_TRY(hipMemcpy(dX, X, (size_t)n * kEnvMax, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dy, y, sizeof(float) * n, hipMemcpyHostToDevice));
+  HIP_TRY(launch_mlp_train_step(train_args(h, n, dX, dy, dH, ddl, dloss, dg), h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(grad_out, dg, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
+  if (loss_out) {
+    std::vector<float> l(n);
+    HIP_TRY(hipMemcpy(l.data(), dloss, sizeof(float) * n, hipMemcpyDeviceToHost));
+    double acc = 0.0;
+    for (float v : l) acc += v;
+    *loss_out = (float)(acc / n);
+  }
+  return CLEXMC_OK;
+}
+
+static clexmc_status collect_device(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *dX,
+                                    float *dy) {
+  CollectArgs c;
+  c.sys = h->d_sys;
+  c.scs = h->d_scs;
+  c.params = h->d_params;
+  c.occ = h->d_occ;
+  c.env_sites = h->d_env_sites;
+  c.n_env = h->n_env;
+  c.n_per_chain = n_per_chain;
+  c.seed = seed;
+  c.include_chem = include_chem;
+  c.X = dX;
+  c.y = dy;
+  HIP_TRY(launch_mlp_collect(c, h->n_chains, h->stream));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_collect(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *X_out,
+                                 float *y_out) {
+  if (!h || !X_out || !y_out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  ARG_CHECK(h->binary, "training data collection is implemented for binary cluster expansions");
+  ARG_CHECK(n_per_chain >= 1, "n_per_chain >= 1 required");
+  HIP_TRY(hipSetDevice(h->device));
+  const size_t n = (size_t)n_per_chain * h->n_chains;
+  DevBufs B;
+  int8_t *dX;
+  float *dy;
+  HIP_TRY(B.alloc(&dX, n * kEnvMax));
+  HIP_TRY(B.alloc(&dy, n));
+  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy);
+  if (st != CLEXMC_OK) return st;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(X_out, dX, n * kEnvMax, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(y_out, dy, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_mlp_train(clexmc_handle *h, int n_per_chain, int n_steps, float lr, uint32_t seed,
+                               int include_chem, float *loss_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  ARG_CHECK(h->binary, "training is implemented for binary cluster expansions");
+  ARG_CHECK(n_per_chain >= 1 && n_steps >= 1 && lr > 0.0f, "bad training arguments");
+  HIP_TRY(hipSetDevice(h->device));
+  const size_t n = (size_t)n_per_chain * h->n_chains;
+  ARG_CHECK(n < (1u << 24), "training batch too large");
+  DevBufs B;
+  int8_t *dX;
+  float *dy, *dH, *ddl, *dloss, *dg;
+  HIP_TRY(B.alloc(&dX, n * kEnvMax));
+  HIP_TRY(B.alloc(&dy, n));
+  HIP_TRY(B.alloc(&dH, n * kHidden));
+  HIP_TRY(B.alloc(&ddl, n));
+  HIP_TRY(B.alloc(&dloss, n));
+  HIP_TRY(B.alloc(&dg, kNParams));
+  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy);
+  if (st != CLEXMC_OK) return st;
+  std::vector<float> l(n);
+  for (int it = 0; it < n_steps; ++it) {
+    HIP_TRY(launch_mlp_train_step(train_args(h, (int)n, dX, dy, dH, ddl, dloss, dg), h->stream));
+    h->adam_t += 1;
+    AdamArgs ad;
+    ad.n_params = kNParams;
+    ad.p = h->d_mlp_params;
+    ad.m = h->d_adam_m;
+    ad.v = h->d_adam_v;
+    ad.grad = dg;
+    ad.lr = lr;
+    ad.b1m = 0.9f;
+    ad.b2m = 0.999f;
+    ad.b1t = (float)std::pow(0.9, (double)h->adam_t);
+    ad.b2t = (float)std::pow(0.999, (double)h->adam_t);
+    HIP_TRY(launch_mlp_adam(ad, h->stream));
+    if (loss_out) {
+      HIP_TRY(hipMemcpyAsync(l.data(), dloss, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      double acc = 0.0;
+      for (float v : l) acc += v;
+      loss_out[it] = (float)(acc / n);
+    }
+  }
+  HIP_TRY(launch_mlp_publish(h->d_mlp_params, h->d_mlp, h->n_env, h->mlp_eps, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return CLEXMC_OK;
 }
 

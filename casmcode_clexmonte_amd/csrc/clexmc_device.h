@@ -25,6 +25,9 @@ constexpr int kMaxFunc = 3;
 constexpr int kMaxOrbits = 64;   // correlations kernel / exact counts storage
 constexpr int kMaxSlots = 8;     // binary path: orbits with >= 1 site (registers)
 constexpr int kMaxSitesPerProp = 4;
+// learned proposal: local-environment MLP (n_env <= kEnvMax inputs, kHidden units)
+constexpr int kEnvMax = 48;
+constexpr int kHidden = 64;
 
 template <typename T>
 using cptr = const __attribute__((address_space(4))) T *;
@@ -101,6 +104,19 @@ struct DevSupercell {
   const int *pair_halo_begin; // [n_pair_phases+1]
   const int2 *pair_halo;
   int max_class;           // largest class size
+  const int *env_off;      // [n_sublat][kEnvMax] padded offsets of the environment sites
+};
+
+// Learned-proposal network (device copy).  FP32 master weights are kept by
+// the host-side trainer; inference uses the BF16 copy of W1.
+struct MlpDev {
+  int n_env;                       // inputs used (<= kEnvMax), rest zero
+  float eps;                       // weight floor: w = eps + sigmoid(logit)
+  float b2;
+  int pad;
+  float b1[kHidden];
+  float W2[kHidden];
+  uint16_t W1b[kHidden * kEnvMax]; // bf16 bits, [hidden][input]
 };
 
 // Read-only per-chain parameters (constant during a launch).

@@ -49,6 +49,53 @@ struct CorrArgs {
   double *out;           // [n_chains][n_corr]
 };
 
+struct LearnArgs {
+  const MlpDev *mlp;
+  int rounds_per_group;
+};
+
+struct CollectArgs {
+  const DevSystem *sys;
+  const DevSupercell *scs;
+  const ChainParams *params;
+  const int8_t *occ;
+  const int4 *env_sites;   // [n_sublat][kEnvMax] (b, di, dj, dk)
+  int n_env;
+  int n_per_chain;
+  uint32_t seed;
+  int include_chem;
+  int8_t *X;               // [n_chains * n_per_chain][kEnvMax]
+  float *y;                // [n_chains * n_per_chain]
+};
+
+struct TrainArgs {
+  int n;
+  int n_env;
+  const int8_t *X;         // [n][kEnvMax]
+  const float *y;          // [n]
+  const float *W1, *b1, *W2, *b2;   // views into the FP32 parameter vector
+  float *H;                // [kHidden][n] pre-activations
+  float *dl;               // [n]
+  float *loss;             // [n]
+  float *grad;             // [n_params]
+};
+
+struct AdamArgs {
+  int n_params;
+  float *p, *m, *v;
+  const float *grad;
+  float lr, b1m, b2m, b1t, b2t;  // b1t = b1m^t, b2t = b2m^t
+};
+
+hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, bool use_mlp, int n_chains,
+                          int threads, size_t lds, hipStream_t st);
+size_t learned_lds_bytes(int pad_bytes);
+hipError_t launch_mlp_infer(const MlpDev *mlp, int n, const int8_t *X, float *out, hipStream_t st);
+hipError_t launch_mlp_collect(const CollectArgs &a, int n_chains, hipStream_t st);
+hipError_t launch_mlp_train_step(const TrainArgs &a, hipStream_t st);
+hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);
+hipError_t launch_mlp_publish(const float *p, MlpDev *m, int n_env, float eps, hipStream_t st);
+
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, int n_chains, int threads,
                              size_t lds, hipStream_t st);
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);

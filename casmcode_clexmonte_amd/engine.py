@@ -68,7 +68,9 @@ def lib():
         for name in ["clexmc_add_supercell", "clexmc_load_chains", "clexmc_delta_energy_batch",
                      "clexmc_correlations", "clexmc_run_passes", "clexmc_read_samples",
                      "clexmc_clear_samples", "clexmc_read_state", "clexmc_read_binary_counts",
-                     "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device"]:
+                     "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device",
+                     "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
+                     "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -82,7 +84,9 @@ def exported_symbols() -> List[str]:
             "clexmc_delta_energy_batch", "clexmc_correlations", "clexmc_run_passes",
             "clexmc_read_samples", "clexmc_clear_samples", "clexmc_read_state",
             "clexmc_read_binary_counts", "clexmc_last_run_ms", "clexmc_set_block_size",
-            "clexmc_philox_device", "clexmc_last_error", "clexmc_destroy", "clexmc_version"]
+            "clexmc_philox_device", "clexmc_last_error", "clexmc_destroy", "clexmc_version",
+            "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
+            "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train"]
 
 
 def _p(a, t):
@@ -352,3 +356,61 @@ class Engine:
         self._check(lib().clexmc_philox_device(self.h, C.c_int(ctr.shape[0]), _p(ctr, u32p), _p(key, u32p),
                                                _p(out, u32p)), "clexmc_philox_device")
         return out
+
+    # -- learned proposal ------------------------------------------------------
+    def set_proposal(self, proposal: str, rounds_per_group: int = 64):
+        code = {"plain": 0, "learned": 1, "grouped_uniform": 2}[proposal]
+        self._check(lib().clexmc_set_proposal(self.h, C.c_int(code), C.c_int(rounds_per_group)),
+                    "clexmc_set_proposal")
+
+    def mlp_info(self):
+        n_env = C.c_int()
+        n_params = C.c_int()
+        env = np.zeros((self.flat.n_sublat, 48, 4), dtype=np.int32)
+        self._check(lib().clexmc_mlp_info(self.h, C.byref(n_env), C.byref(n_params), _p(env, i32p)),
+                    "clexmc_mlp_info")
+        return n_env.value, n_params.value, env
+
+    def mlp_set_params(self, params: np.ndarray, eps: float = 0.05):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(lib().clexmc_mlp_set_params(self.h, _p(p, f32p), C.c_float(eps)), "clexmc_mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        _, n_params, _ = self.mlp_info()
+        p = np.zeros(n_params, dtype=np.float32)
+        self._check(lib().clexmc_mlp_get_params(self.h, _p(p, f32p)), "clexmc_mlp_get_params")
+        return p
+
+    def mlp_forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.int8).reshape(-1, 48)
+        out = np.zeros((X.shape[0], 2), dtype=np.float32)
+        self._check(lib().clexmc_mlp_forward(self.h, C.c_int(X.shape[0]), _p(X, i8p), _p(out, f32p)),
+                    "clexmc_mlp_forward")
+        return out
+
+    def mlp_gradients(self, X: np.ndarray, y: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.int8).reshape(-1, 48)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        _, n_params, _ = self.mlp_info()
+        g = np.zeros(n_params, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().clexmc_mlp_gradients(self.h, C.c_int(X.shape[0]), _p(X, i8p), _p(y, f32p), _p(g, f32p),
+                                               C.byref(loss)), "clexmc_mlp_gradients")
+        return g, loss.value
+
+    def mlp_collect(self, n_per_chain: int, seed: int = 0, include_chem: bool = False):
+        n = n_per_chain * self.n_chains
+        X = np.zeros((n, 48), dtype=np.int8)
+        y = np.zeros(n, dtype=np.float32)
+        self._check(lib().clexmc_mlp_collect(self.h, C.c_int(n_per_chain), C.c_uint32(seed),
+                                             C.c_int(1 if include_chem else 0), _p(X, i8p), _p(y, f32p)),
+                    "clexmc_mlp_collect")
+        return X, y
+
+    def mlp_train(self, n_per_chain: int, n_steps: int, lr: float = 1e-3, seed: int = 0,
+                  include_chem: bool = False) -> np.ndarray:
+        loss = np.zeros(n_steps, dtype=np.float32)
+        self._check(lib().clexmc_mlp_train(self.h, C.c_int(n_per_chain), C.c_int(n_steps), C.c_float(lr),
+                                           C.c_uint32(seed), C.c_int(1 if include_chem else 0), _p(loss, f32p)),
+                    "clexmc_mlp_train")
+        return loss

@@ -84,3 +84,55 @@ def fcc_sgc_grid(sc: Supercell, n_T: int = 16, n_mu: int = 16, T_range=(200.0, 1
             c += 1
     return ChainBatch(np.array(temps), np.array(chem), occs, np.array(seeds, dtype=np.uint64),
                       np.array(mu_list))
+
+
+# ---------------------------------------------------------------------------
+# Learned-proposal workload: ordering-favouring FCC binary CE
+# ---------------------------------------------------------------------------
+ORDERING_ECI_SEED = 20240612
+
+
+def fcc_ordering_ce(seed: int = ORDERING_ECI_SEED, nn_pair: float = 0.04, sigma: float = 0.005) -> ClusterExpansion:
+    """Binary FCC (same clusters as fcc_binary_ce) with a positive nearest-
+    neighbour pair ECI (+0.04 eV, favouring unlike neighbours: L1_2 / L1_0
+    ordering) and small seeded ECIs on the other clusters."""
+    bs = fcc_binary_basis()
+    rng = np.random.default_rng(seed)
+    eci = np.zeros(bs.n_functions)
+    nn_orbit = next(o for o in bs.orbits if o.size == 2)
+    for orb in bs.orbits:
+        for fi in orb.functions:
+            if orb is nn_orbit:
+                eci[fi] = nn_pair
+            elif orb.size >= 2:
+                eci[fi] = rng.normal(0.0, sigma)
+    return ClusterExpansion(bs, eci)
+
+
+def random_mlp_params(seed: int, n_env: int, scale: float = 0.3) -> np.ndarray:
+    """Random network parameters [W1 (64x48) | b1 | W2 | b2] (FP32)."""
+    rng = np.random.default_rng(seed)
+    W1 = np.zeros((64, 48))
+    W1[:, :n_env] = rng.normal(0.0, scale / np.sqrt(max(n_env, 1)), (64, n_env))
+    b1 = rng.normal(0.0, 0.1, 64)
+    W2 = rng.normal(0.0, scale, 64)
+    b2 = np.array([0.0])
+    return np.concatenate([W1.ravel(), b1, W2, b2]).astype(np.float32)
+
+
+def l12_occupation(sc: Supercell) -> np.ndarray:
+    """Perfect L1_2 (A3B) arrangement on an FCC supercell: B (occupant 1) on
+    the sites whose Cartesian coordinates are integer multiples of the
+    conventional lattice parameter, A on the face centres.  Needs even
+    supercell dimensions."""
+    prim = sc.prim
+    a = 2.0 * abs(prim.lattice[0][1])
+    occ = np.zeros(sc.n_sites, dtype=np.int8)
+    L1, L2, _ = sc.dims
+    for l in range(sc.n_sites):
+        c = l % sc.n_cells
+        i, j, k = c % L1, (c // L1) % L2, c // (L1 * L2)
+        r = prim.cart(0, (i, j, k)) / a
+        if np.allclose(r, np.round(r), atol=1e-9):
+            occ[l] = 1
+    return occ

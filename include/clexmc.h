@@ -179,6 +179,43 @@ clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
 /* Tuning knobs (block size of the Metropolis kernel; 0 = automatic). */
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
 
+/* ---- learned proposal (no counterpart in the reference: it plugs a
+ * proposal into the Metropolis loop occupation_metropolis_v2,
+ * occupation_metropolis.hh:97-105, keeping Metropolis-Hastings acceptance
+ * with the exact proposal ratio) ---- */
+
+/* Proposal used by clexmc_run_passes: 0 plain (independent-set sweeps and
+ * random matchings), 1 learned (MLP-weighted sequential proposals within
+ * groups of 64 independent sites), 2 the same grouped proposals with uniform
+ * weights.  rounds_per_group: proposals per group and class phase. */
+clexmc_status clexmc_set_proposal(clexmc_handle *h, int proposal, int rounds_per_group);
+
+/* Network size: n_env inputs (sites of the three shortest pair orbits),
+ * n_params = 64*48 + 64 + 64 + 1 FP32 parameters [W1 (64x48 row-major) | b1 |
+ * W2 | b2]; env_sites [n_sublat][48][4] (b, di, dj, dk) if non-NULL. */
+clexmc_status clexmc_mlp_info(clexmc_handle *h, int *n_env, int *n_params, int *env_sites);
+clexmc_status clexmc_mlp_set_params(clexmc_handle *h, const float *params, float eps);
+clexmc_status clexmc_mlp_get_params(clexmc_handle *h, float *params);
+
+/* MFMA (BF16) inference on feature rows X [n][48] (+1/-1/0): out [n][2] =
+ * logits for the current and the flipped occupation. */
+clexmc_status clexmc_mlp_forward(clexmc_handle *h, int n, const int8_t *X, float *out);
+
+/* FP32 training forward/backward on (X, y): mean binary cross-entropy and its
+ * gradient with respect to the n_params parameters. */
+clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, float *grad_out,
+                                   float *loss_out);
+
+/* Training samples from the loaded chains' current configurations:
+ * X [n_chains*n_per_chain][48], y = sigmoid(-beta * flip energy). */
+clexmc_status clexmc_mlp_collect(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *X_out,
+                                 float *y_out);
+
+/* Collects n_per_chain samples per chain and runs n_steps Adam steps on the
+ * device (hand-written forward/backward kernels); loss_out [n_steps] or NULL. */
+clexmc_status clexmc_mlp_train(clexmc_handle *h, int n_per_chain, int n_steps, float lr, uint32_t seed,
+                               int include_chem, float *loss_out);
+
 /* Philox4x32-10 evaluated on the device (known-answer checks). */
 clexmc_status clexmc_philox_device(clexmc_handle *h, int n, const uint32_t *ctr /*[n][4]*/,
                                    const uint32_t *key /*[n][2]*/, uint32_t *out /*[n][4]*/);

@@ -1,0 +1,311 @@
+"""Learned proposal: network kernels versus the FP64 CPU oracle, detailed
+balance of the grouped weighted proposal, and the gain of the trained
+network over the same proposal with uniform weights.
+
+* MFMA inference (BF16 W1, FP32 accumulate) versus the oracle evaluated with
+  the same BF16-rounded W1: |logit error| <= 1e-4 (absolute, logits O(1)).
+* Training gradients (FP32 forward, FP64-accumulated backward) versus the
+  oracle: relative L2 error <= 1e-4.
+* Exact enumeration (16-site FCC supercell): Boltzmann averages within 3
+  sigma with frozen random weights (learned) and with uniform weights.
+* Exact int64 correlation counts after learned passes equal the oracle's
+  full recompute (the proposal never touches the energy bookkeeping).
+* 12x12x12 ordering CE, canonical, learned (random weights) versus the
+  oracle's restatement of the reference loop: <E> within 3 sigma.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import mlp_oracle as M
+from casmcode_clexmonte_amd.analysis import mean_and_error, tau_int
+from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.system import BasisSet, ClusterExpansion, FlatCE, Supercell, fcc_prim
+from casmcode_clexmonte_amd.workloads import (fcc_binary_ce, fcc_ordering_ce, random_mlp_params,
+                                              random_occupation)
+
+pytestmark = pytest.mark.gpu
+
+NSIG = 3.0
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def small_flat(seed=0):
+    prim = fcc_prim()
+    bs = BasisSet(prim, {2: 1.05, 3: 0.75}, "chebychev")
+    eci = np.random.default_rng(seed).normal(0, 0.05, bs.n_functions)
+    return FlatCE(ClusterExpansion(bs, eci))
+
+
+def _random_features(n, n_env, seed):
+    rng = np.random.default_rng(seed)
+    X = np.where(rng.random((n, 48)) < 0.5, -1, 1).astype(np.int8)
+    X[:, n_env:] = 0
+    return X
+
+
+def test_env_sites_are_first_three_shells():
+    _print_case("environment sites")
+    eng = Engine(FlatCE(fcc_binary_ce()))
+    n_env, n_params, env = eng.mlp_info()
+    assert n_env == 42 and n_params == M.N_PARAMS
+    # 12 + 6 + 24 distinct neighbours at increasing distance (FCC a = 1)
+    prim = fcc_prim()
+    L = np.array(prim.lattice, dtype=float)
+    d = [np.linalg.norm(np.array(e[1:4], dtype=float) @ L) for e in env[0, :n_env]]
+    assert len({tuple(e) for e in env[0, :n_env]}) == 42
+    assert np.allclose(sorted(d)[:12], 1 / np.sqrt(2)) and np.allclose(sorted(d)[12:18], 1.0)
+    assert np.allclose(sorted(d)[18:42], np.sqrt(1.5))
+    eng.close()
+
+
+def test_mlp_inference_mfma_vs_oracle():
+    _print_case("mlp inference (MFMA) vs oracle")
+    eng = Engine(FlatCE(fcc_binary_ce()))
+    n_env = eng.mlp_info()[0]
+    for seed in (1, 2):
+        p = random_mlp_params(seed, n_env, scale=0.8)
+        p[-1] = 0.3
+        eng.mlp_set_params(p)
+        X = _random_features(200, n_env, seed + 10)   # 200: ragged last wavefront
+        got = eng.mlp_forward(X)
+        ref = M.logits(p, X, n_env, bf16_w1=True)
+        err = np.abs(got - ref).max()
+        print(f"seed {seed}: max |dlogit| = {err:.3e}, max |logit| = {np.abs(ref).max():.3f}")
+        assert err <= 1e-4, err
+        # the flipped-occupation column equals the forward pass of -X
+        ref_neg = M.logits(p, -X.astype(np.int64), n_env, bf16_w1=True)
+        assert np.abs(got[:, 1] - ref_neg[:, 0]).max() <= 1e-4
+    eng.close()
+
+
+def test_mlp_gradients_vs_oracle():
+    _print_case("mlp gradients vs oracle")
+    eng = Engine(FlatCE(fcc_binary_ce()))
+    n_env = eng.mlp_info()[0]
+    rng = np.random.default_rng(5)
+    for n in (1, 77, 4096):
+        p = random_mlp_params(int(rng.integers(1 << 30)), n_env, scale=0.8)
+        p[-1] = float(rng.normal())
+        eng.mlp_set_params(p)
+        X = _random_features(n, n_env, n)
+        y = rng.random(n).astype(np.float32)
+        g, loss = eng.mlp_gradients(X, y)
+        lref, gref = M.loss_and_grad(p, X, y, n_env)
+        rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+        print(f"n {n}: loss {loss:.6f} vs {lref:.6f}; relative gradient error {rel:.2e}")
+        assert rel <= 1e-4, rel
+        assert abs(loss - lref) <= 1e-5 * max(1.0, abs(lref))
+        # padding inputs get no gradient
+        assert np.all(g[:64 * 48].reshape(64, 48)[:, n_env:] == 0)
+    eng.close()
+
+
+def test_mlp_collect_labels_are_heat_bath_probabilities():
+    """Collected features and labels versus the oracle's flip energies."""
+    _print_case("mlp collect vs oracle")
+    flat = FlatCE(fcc_binary_ce())
+    sc = Supercell(flat.ce.prim, (8, 8, 8))
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    occ = random_occupation(sc.n_sites, 0.4, 3)
+    T = 700.0
+    eng.load_chains([sid], [occ], [T], [5], chem=[[0.0, 0.02]])
+    X, y = eng.mlp_collect(256, seed=9, include_chem=True)
+    n_env, _, env = eng.mlp_info()
+    osys = O.OracleSystem(flat, sc.dims)
+    beta = 1.0 / (8.6173303e-05 * T)
+    # recover each sample's site by matching its features (the kernel's site
+    # choice is internal); check the label of every site whose features match
+    L1, L2, L3 = sc.dims
+    feats = np.zeros((sc.n_sites, 48), dtype=np.int8)
+    for l in range(sc.n_sites):
+        i, j, k = l % L1, (l // L1) % L2, l // (L1 * L2)
+        for q in range(n_env):
+            b2, di, dj, dk = env[0, q]
+            m = sc.linear_index(b2, i + di, j + dj, k + dk)
+            feats[l, q] = 1 if occ[m] == occ[l] else -1
+    checked = 0
+    for s in range(len(y)):
+        cand = np.where((feats == X[s]).all(axis=1))[0]
+        assert len(cand) >= 1
+        ok = False
+        for l in cand:
+            dE = osys.delta(occ, [int(l)], [1 - int(occ[l])]) - (0.02 if occ[l] == 0 else -0.02)
+            if abs(y[s] - 1.0 / (1.0 + np.exp(beta * dE))) <= 2e-5:
+                ok = True
+                break
+        assert ok, s
+        checked += 1
+    assert checked == len(y)
+    eng.close()
+
+
+@pytest.mark.parametrize("proposal", ["learned", "grouped_uniform"])
+def test_learned_sgc_exact_enumeration(proposal):
+    _print_case(f"{proposal} sgc exact enumeration 2x2x4")
+    flat = small_flat()
+    sc = Supercell(flat.ce.prim, (2, 2, 4))
+    T, mu = 1200.0, 0.03
+    chem = [0.0, mu]
+    exact = O.OracleSystem(flat, sc.dims).enumerate("sgc", T, chem=chem)
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    n_env = eng.mlp_info()[0]
+    eng.mlp_set_params(random_mlp_params(17, n_env, scale=2.0), eps=0.05)
+    eng.set_proposal(proposal, rounds_per_group=4)
+    occs = [random_occupation(sc.n_sites, 0.5, 100 + c, exact=False) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 5000 + np.arange(nch), chem=[chem] * nch)
+    eng.run_passes("semigrand_canonical", 4000, 1)
+    E, cnt = eng.read_samples()
+    burn = 200
+    e = E[:, burn:]
+    n = cnt[:, burn:, 1].astype(float)
+    per = {"E": e.mean(1), "E2": (e * e).mean(1), "n1": n.mean(1), "n12": (n * n).mean(1)}
+    for k in ("E", "E2", "n1", "n12"):
+        m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
+        print(f"{k}: gpu {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
+        assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
+    eng.close()
+
+
+@pytest.mark.parametrize("proposal", ["learned", "grouped_uniform"])
+def test_learned_canonical_exact_enumeration(proposal):
+    _print_case(f"{proposal} canonical exact enumeration 2x2x4")
+    flat = small_flat()
+    sc = Supercell(flat.ce.prim, (2, 2, 4))
+    T = 1200.0
+    nb = 8
+    exact = O.OracleSystem(flat, sc.dims).enumerate("canonical", T, fixed_counts=[sc.n_sites - nb, nb])
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=64, seed=5)
+    n_env = eng.mlp_info()[0]
+    eng.mlp_set_params(random_mlp_params(23, n_env, scale=2.0), eps=0.05)
+    eng.set_proposal(proposal, rounds_per_group=4)
+    occs = [random_occupation(sc.n_sites, nb / sc.n_sites, 300 + c) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 7000 + np.arange(nch))
+    eng.run_passes("canonical", 6000, 1)
+    E, cnt = eng.read_samples()
+    assert np.all(cnt[:, :, 1] == nb)
+    e = E[:, 300:]
+    per = {"E": e.mean(1), "E2": (e * e).mean(1)}
+    for k in ("E", "E2"):
+        m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
+        print(f"{k}: gpu {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
+        assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
+    eng.close()
+
+
+@pytest.mark.parametrize("kind", ["semigrand_canonical", "canonical"])
+def test_learned_run_exact_counts(kind):
+    _print_case(f"learned run exact counts {kind}")
+    flat = FlatCE(fcc_ordering_ce())
+    sc = Supercell(flat.ce.prim, (12, 12, 12))
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=8, seed=1)
+    eng.mlp_set_params(random_mlp_params(3, eng.mlp_info()[0], scale=1.0))
+    eng.set_proposal("learned", rounds_per_group=16)
+    occs = [random_occupation(sc.n_sites, 0.25, s) for s in range(4)]
+    chem = [[0.0, mu] for mu in (-0.1, 0.0, 0.05, 0.1)]
+    eng.load_chains([sid] * 4, occs, [400.0, 800.0, 1200.0, 2000.0], [11, 12, 13, 14], chem=chem)
+    eng.run_passes(kind, 30, 1)
+    osys = O.OracleSystem(flat, sc.dims)
+    E, cnt = eng.read_samples()
+    for c in range(4):
+        st = eng.read_state(c)
+        C = eng.read_binary_counts(c)
+        corr = osys.corr(st["occ"])
+        assert np.array_equal(C, np.rint(corr * flat.orbit_mult).astype(np.int64))
+        Eref = osys.energy(st["occ"])
+        assert abs(st["E"] - Eref) <= 1e-9 * max(1.0, abs(Eref))
+        nb = int(st["occ"].sum())
+        assert cnt[c, -1, 1] == nb
+        if kind == "canonical":
+            assert nb == int(occs[c].sum())
+        assert st["n_accept"] > 0
+    eng.close()
+
+
+def test_learned_canonical_vs_oracle_ordering_ce():
+    _print_case("learned canonical 12^3 ordering CE vs oracle")
+    ce = fcc_ordering_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    T = 1000.0
+    nc = sc.n_cells
+    osys = O.OracleSystem(flat, sc.dims)
+    o_means, o_vars = [], []
+    for c in range(2):
+        r = osys.run(random_occupation(sc.n_sites, 0.25, 40 + c), "canonical", T, 300, 1200, 77 + c,
+                     full_recompute_samplers=False)
+        m, e = mean_and_error(r["E"] / nc)
+        o_means.append(m)
+        o_vars.append(e * e)
+    o_mean = float(np.mean(o_means))
+    o_err = float(np.sqrt(np.sum(o_vars)) / len(o_means))
+    nch = 32
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=16, seed=3)
+    eng.mlp_set_params(random_mlp_params(8, eng.mlp_info()[0], scale=1.5))
+    eng.set_proposal("learned", rounds_per_group=32)
+    occs = [random_occupation(sc.n_sites, 0.25, 500 + c) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 3000 + np.arange(nch))
+    eng.run_passes("canonical", 1500, 1)
+    E, _ = eng.read_samples()
+    per_chain = E[:, 300:].mean(1) / nc
+    g_mean = float(per_chain.mean())
+    g_err = float(per_chain.std(ddof=1) / np.sqrt(nch))
+    tol = NSIG * np.sqrt(g_err ** 2 + o_err ** 2)
+    print(f"oracle {o_mean:.6f} +- {o_err:.6f}  gpu learned {g_mean:.6f} +- {g_err:.6f}")
+    assert abs(g_mean - o_mean) <= tol, (g_mean, o_mean, tol)
+    eng.close()
+
+
+def test_trained_network_gain_over_uniform_weights():
+    """What training buys: on the ordering CE at x_B = 0.25 below the
+    ordering temperature, the grouped proposal with the trained network
+    accepts several times more of its proposals than the same proposal with
+    uniform weights."""
+    _print_case("trained network gain")
+    ce = fcc_ordering_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (16, 16, 16))
+    nc = sc.n_cells
+    T = 500.0
+    nch = 32
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=16, seed=4)
+    occs = [random_occupation(sc.n_sites, 0.25, 900 + c) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 9000 + np.arange(nch))
+    eng.run_passes("canonical", 400, 400)          # plain sampler: equilibrate, generate training data
+    loss = eng.mlp_train(n_per_chain=1024, n_steps=300, lr=3e-3, seed=1)
+    print(f"training loss {loss[0]:.4f} -> {loss[-1]:.4f}")
+    assert loss[-1] < 0.8 * loss[0]
+
+    def measure(proposal):
+        eng.set_proposal(proposal, rounds_per_group=32)
+        s0 = [eng.read_state(c) for c in range(nch)]
+        eng.clear_samples()
+        eng.run_passes("canonical", 300, 1)
+        s1 = [eng.read_state(c) for c in range(nch)]
+        att = sum(b["n_attempt"] - a["n_attempt"] for a, b in zip(s0, s1))
+        acc = sum(b["n_accept"] - a["n_accept"] for a, b in zip(s0, s1))
+        E, _ = eng.read_samples()
+        return acc / att, E / nc
+
+    r_uni, E_uni = measure("grouped_uniform")
+    r_lrn, E_lrn = measure("learned")
+    tau_u = np.median([tau_int(e) for e in E_uni[:, 50:]])
+    tau_l = np.median([tau_int(e) for e in E_lrn[:, 50:]])
+    print(f"acceptance: uniform weights {r_uni:.4f}, trained network {r_lrn:.4f} ({r_lrn / r_uni:.1f}x); "
+          f"tau_int(E) passes: uniform {tau_u:.1f}, trained {tau_l:.1f}")
+    assert r_lrn >= 3.0 * r_uni
+    mu_, su = E_uni[:, 50:].mean(1).mean(), E_uni[:, 50:].mean(1).std(ddof=1) / np.sqrt(nch)
+    ml_, sl = E_lrn[:, 50:].mean(1).mean(), E_lrn[:, 50:].mean(1).std(ddof=1) / np.sqrt(nch)
+    print(f"<E>/cell: uniform {mu_:.6f} +- {su:.6f}, trained {ml_:.6f} +- {sl:.6f}")
+    eng.close()
