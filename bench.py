@@ -10,6 +10,18 @@ passes run untimed, then K passes run in one persistent launch bracketed by
 device synchronisation.  Inputs (occupations, tables) are resident in HBM
 before the timed region.
 
+Second workload (effective samples per second, learned versus plain
+proposal): binary FCC ordering CE (nearest-neighbour pair +0.04 eV), x_B =
+0.25, 32x32x32 supercell (32,768 sites), 128 canonical chains at temperatures
+bracketing the L1_2 ordering temperature T_c (0.7..1.3 T_c; T_c from the CPU
+oracle's heat-capacity scan in tests/golden/fcc_ordering_tc_scan.json).  The
+plain sampler equilibrates the chains and generates the training data; the
+network is trained on the device; then each proposal runs the same number of
+sampled passes and ESS/s = sum over chains of n_samples / (2 tau_int(E)) per
+second of sampling.  Every proposal starts from the same equilibrated
+configurations; the <E> of every chain is compared between proposals (3
+sigma, tau-corrected errors).
+
 Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
 build measures one GPU.
 """
@@ -57,6 +69,81 @@ def cpu_baseline(flat, sc, batch, n_chains: int, n_passes: int, threads: int):
     return r["n_attempt"] / r["elapsed_s"], r
 
 
+def learned_comparison(args):
+    """ESS/s of the plain, learned and informed proposals on the ordering
+    workload (see the module docstring)."""
+    from casmcode_clexmonte_amd.engine import Engine
+    from casmcode_clexmonte_amd.system import FlatCE, Supercell
+    from casmcode_clexmonte_amd.workloads import fcc_ordering_ce, l12_occupation, random_occupation
+
+    with open(os.path.join(ROOT, "tests", "golden", "fcc_ordering_tc_scan.json")) as f:
+        Tc = float(json.load(f)["Tc"])
+    ce = fcc_ordering_ce()
+    flat = FlatCE(ce)
+    L = args.c4_size
+    sc = Supercell(ce.prim, (L, L, L))
+    nch = args.c4_chains
+    temps = Tc * np.linspace(0.7, 1.3, nch)
+    occ_ord = l12_occupation(sc)
+    occs = [occ_ord if T < Tc else random_occupation(sc.n_sites, 0.25, 4000 + c) for c, T in enumerate(temps)]
+    eng = Engine(flat, device=0)
+    sid = eng.add_supercell(sc, n_pair_phases=32, seed=11)
+    eng.load_chains([sid] * nch, occs, temps, 77000 + np.arange(nch))
+    eng.run_passes("canonical", args.c4_equil, args.c4_equil, canonical_rounds=2)
+    t0 = time.perf_counter()
+    loss = eng.mlp_train(n_per_chain=args.c4_train_samples, n_steps=args.c4_train_steps, lr=3e-3, seed=5)
+    train_s = time.perf_counter() - t0
+    # every proposal starts from the same equilibrated configurations
+    snap = [eng.read_state(c)["occ"].copy() for c in range(nch)]
+    res = {}
+    means = {}
+    for i, prop in enumerate(("plain", "learned", "informed")):
+        eng.load_chains([sid] * nch, snap, temps, 91000 + 1000 * i + np.arange(nch))
+        eng.set_proposal(prop, rounds_per_group=args.c4_rounds)
+        eng.clear_samples()
+        a0 = [eng.read_state(c) for c in range(nch)]
+        t0 = time.perf_counter()
+        eng.run_passes("canonical", args.c4_passes, 1, canonical_rounds=2)
+        wall = time.perf_counter() - t0
+        a1 = [eng.read_state(c) for c in range(nch)]
+        E, _ = eng.read_samples()
+        E = E / sc.n_cells
+        burn = args.c4_passes // 10
+        taus = np.array([tau_int(E[c, burn:]) for c in range(nch)])
+        n = E.shape[1] - burn
+        ess = float(np.sum(n / (2.0 * taus)))
+        att = sum(b["n_attempt"] - a["n_attempt"] for a, b in zip(a0, a1))
+        acc = sum(b["n_accept"] - a["n_accept"] for a, b in zip(a0, a1))
+        m = E[:, burn:].mean(1)
+        se = E[:, burn:].std(1) * np.sqrt(2.0 * taus / n)
+        means[prop] = (m, se)
+        # ESS over the post-burn-in samples per second of the passes that made them
+        res[prop] = {"ess_per_s": ess / (wall * n / E.shape[1]),
+                     "passes_per_s": args.c4_passes / wall, "attempts_per_s": att / wall,
+                     "acceptance": acc / max(1, att), "tau_int_E_median_passes": float(np.median(taus)),
+                     "tau_int_E_max_passes": float(np.max(taus))}
+        # ESS/s by temperature band (fractions of T_c)
+        bands = {}
+        for lo, hi in ((0.0, 0.9), (0.9, 1.0), (1.0, 1.1), (1.1, 9.9)):
+            sel = (temps >= lo * Tc) & (temps < hi * Tc)
+            if sel.any():
+                bands[f"{lo:.1f}-{hi:.1f}Tc"] = float(np.sum(n / (2.0 * taus[sel])) / (wall * n / E.shape[1]))
+        res[prop]["ess_per_s_by_T_band"] = bands
+    for prop in ("learned", "informed"):
+        (m0, s0), (m1, s1) = means["plain"], means[prop]
+        z = np.abs(m1 - m0) / np.sqrt(s0 ** 2 + s1 ** 2 + 1e-30)
+        res[prop]["E_agree_3sigma_frac"] = float(np.mean(z <= 3.0))
+        res[prop]["E_max_z"] = float(z.max())
+        res[prop]["ess_per_s_vs_plain"] = res[prop]["ess_per_s"] / res["plain"]["ess_per_s"]
+    eng.close()
+    return {"workload": f"fcc_ordering_{L}x{L}x{L}_canonical_xB0.25_{nch}chains_T{temps[0]:.0f}-{temps[-1]:.0f}K",
+            "Tc_K": Tc, "sampled_passes": args.c4_passes, "equilibration_passes": args.c4_equil,
+            "rounds_per_group": args.c4_rounds,
+            "training": {"samples": args.c4_train_samples * nch, "adam_steps": args.c4_train_steps,
+                         "seconds": train_s, "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
+            **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +153,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-passes", type=int, default=40)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal ESS comparison")
+    ap.add_argument("--c4-size", type=int, default=32)
+    ap.add_argument("--c4-chains", type=int, default=128)
+    ap.add_argument("--c4-equil", type=int, default=1000)
+    ap.add_argument("--c4-passes", type=int, default=2000)
+    ap.add_argument("--c4-rounds", type=int, default=32)
+    ap.add_argument("--c4-train-samples", type=int, default=512)
+    ap.add_argument("--c4-train-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="copy N MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -143,6 +238,9 @@ def main():
         "tau_int_E_median_passes": float(np.median(taus)),
         "acceptance": float(sum(eng.read_state(c)["n_accept"] for c in range(n_chains)) / max(1, att1)),
     }
+    eng.close()
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_comparison(args)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_rate, r = cpu_baseline(flat, sc, batch, args.cpu_chains, args.cpu_passes, threads)
@@ -153,7 +251,6 @@ def main():
                                          f"{r['elapsed_s']:.1f} s wall"}
         out["speedup_vs_cpu"] = value / cpu_rate
     print(json.dumps(out))
-    eng.close()
 
 
 if __name__ == "__main__":

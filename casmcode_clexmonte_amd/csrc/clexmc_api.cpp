@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <numeric>
+#include <random>
 #include <set>
 #include <cmath>
 #include <cstdio>
@@ -72,8 +73,9 @@ struct clexmc_handle {
   MlpDev *d_mlp = nullptr;
   float *d_mlp_params = nullptr, *d_adam_m = nullptr, *d_adam_v = nullptr;
   long long adam_t = 0;
+  bool mlp_initialized = false;     // parameters set by the caller or a training run
   float mlp_eps = 0.05f;
-  int proposal = 0;                 // 0 plain, 1 learned (MLP), 2 grouped uniform
+  int proposal = 0;                 // 0 plain, 1 learned (MLP), 2 grouped uniform, 3 informed
   int rounds_per_group = 64;
 };
 
@@ -994,7 +996,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
     la.mlp = h->d_mlp;
     la.rounds_per_group = h->rounds_per_group;
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    HIP_TRY(launch_learned(a, la, kind, h->proposal == 1, h->n_chains, threads, lds, h->stream));
+    const int weights = h->proposal == 1 ? 1 : (h->proposal == 3 ? 2 : 0);
+    HIP_TRY(launch_learned(a, la, kind, weights, h->n_chains, threads, lds, h->stream));
   } else {
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
@@ -1101,7 +1104,8 @@ static const int kNParams = kHidden * kEnvMax + 2 * kHidden + 1;
 
 clexmc_status clexmc_set_proposal(clexmc_handle *h, int proposal, int rounds_per_group) {
   if (!h) return CLEXMC_ERR_ARG;
-  ARG_CHECK(proposal >= 0 && proposal <= 2, "proposal must be 0 (plain), 1 (learned) or 2 (grouped uniform)");
+  ARG_CHECK(proposal >= 0 && proposal <= 3,
+            "proposal must be 0 (plain), 1 (learned), 2 (grouped uniform) or 3 (grouped informed)");
   ARG_CHECK(rounds_per_group >= 1 && rounds_per_group <= 4096, "rounds_per_group out of range");
   h->proposal = proposal;
   h->rounds_per_group = rounds_per_group;
@@ -1130,6 +1134,7 @@ clexmc_status clexmc_mlp_set_params(clexmc_handle *h, const float *params, float
   HIP_TRY(hipMemset(h->d_adam_m, 0, sizeof(float) * kNParams));
   HIP_TRY(hipMemset(h->d_adam_v, 0, sizeof(float) * kNParams));
   h->adam_t = 0;
+  h->mlp_initialized = true;
   h->mlp_eps = eps;
   HIP_TRY(launch_mlp_publish(h->d_mlp_params, h->d_mlp, h->n_env, eps, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1164,13 +1169,35 @@ clexmc_status clexmc_mlp_forward(clexmc_handle *h, int n, const int8_t *X, float
   return CLEXMC_OK;
 }
 
-static TrainArgs train_args(clexmc_handle *h, int n, const int8_t *dX, const float *dy, float *dH, float *ddl,
-                            float *dloss, float *dgrad) {
+clexmc_status clexmc_wave_pick_device(clexmc_handle *h, int n_rows, const float *v, const float *u, float *scan_out,
+                                      int *pick_out) {
+  if (!h || !v || !u || !scan_out || !pick_out) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_rows >= 1 && n_rows <= 65535, "n_rows out of range");
+  HIP_TRY(hipSetDevice(h->device));
+  DevBufs B;
+  float *dv, *du, *ds;
+  int *dp;
+  HIP_TRY(B.alloc(&dv, (size_t)n_rows * 64));
+  HIP_TRY(B.alloc(&du, (size_t)n_rows));
+  HIP_TRY(B.alloc(&ds, (size_t)n_rows * 64));
+  HIP_TRY(B.alloc(&dp, (size_t)n_rows));
+  HIP_TRY(hipMemcpy(dv, v, sizeof(float) * 64 * (size_t)n_rows, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(du, u, sizeof(float) * (size_t)n_rows, hipMemcpyHostToDevice));
+  HIP_TRY(launch_wave_pick(n_rows, dv, du, ds, dp, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(scan_out, ds, sizeof(float) * 64 * (size_t)n_rows, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pick_out, dp, sizeof(int) * (size_t)n_rows, hipMemcpyDeviceToHost));
+  return CLEXMC_OK;
+}
+
+static TrainArgs train_args(clexmc_handle *h, int n, const int8_t *dX, const float *dy, const float *dt, float *dH,
+                            float *ddl, float *dloss, float *dgrad) {
   TrainArgs t;
   t.n = n;
   t.n_env = h->n_env;
   t.X = dX;
   t.y = dy;
+  t.t = dt;
   t.W1 = h->d_mlp_params;
   t.b1 = h->d_mlp_params + kHidden * kEnvMax;
   t.W2 = h->d_mlp_params + kHidden * kEnvMax + kHidden;
@@ -1184,23 +1211,27 @@ static TrainArgs train_args(clexmc_handle *h, int n, const int8_t *dX, const flo
 
 
 
-clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, float *grad_out,
-                                   float *loss_out) {
+clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, const float *t,
+                                   float *grad_out, float *loss_out) {
   if (!h || !X || !y || !grad_out) return CLEXMC_ERR_ARG;
   ARG_CHECK(n >= 1, "n >= 1 required");
   HIP_TRY(hipSetDevice(h->device));
   DevBufs B;
   int8_t *dX;
-  float *dy, *dH, *ddl, *dloss, *dg;
+  float *dy, *dt = nullptr, *dH, *ddl, *dloss, *dg;
   HIP_TRY(B.alloc(&dX, (size_t)n * kEnvMax));
   HIP_TRY(B.alloc(&dy, n));
+  if (t) {
+    HIP_TRY(B.alloc(&dt, n));
+    HIP_TRY(hipMemcpy(dt, t, sizeof(float) * n, hipMemcpyHostToDevice));
+  }
   HIP_TRY(B.alloc(&dH, (size_t)n * kHidden));
   HIP_TRY(B.alloc(&ddl, n));
   HIP_TRY(B.alloc(&dloss, n));
   HIP_TRY(B.alloc(&dg, kNParams));
   HIP_TRY(hipMemcpy(dX, X, (size_t)n * kEnvMax, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dy, y, sizeof(float) * n, hipMemcpyHostToDevice));
-  HIP_TRY(launch_mlp_train_step(train_args(h, n, dX, dy, dH, ddl, dloss, dg), h->stream));
+  HIP_TRY(launch_mlp_train_step(train_args(h, n, dX, dy, dt, dH, ddl, dloss, dg), h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   HIP_TRY(hipMemcpy(grad_out, dg, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
   if (loss_out) {
@@ -1214,7 +1245,7 @@ clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, con
 }
 
 static clexmc_status collect_device(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *dX,
-                                    float *dy) {
+                                    float *dy, float *dt) {
   CollectArgs c;
   c.sys = h->d_sys;
   c.scs = h->d_scs;
@@ -1227,12 +1258,13 @@ static clexmc_status collect_device(clexmc_handle *h, int n_per_chain, uint32_t 
   c.include_chem = include_chem;
   c.X = dX;
   c.y = dy;
+  c.t = dt;
   HIP_TRY(launch_mlp_collect(c, h->n_chains, h->stream));
   return CLEXMC_OK;
 }
 
 clexmc_status clexmc_mlp_collect(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *X_out,
-                                 float *y_out) {
+                                 float *y_out, float *t_out) {
   if (!h || !X_out || !y_out) return CLEXMC_ERR_ARG;
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
   ARG_CHECK(h->binary, "training data collection is implemented for binary cluster expansions");
@@ -1241,14 +1273,16 @@ clexmc_status clexmc_mlp_collect(clexmc_handle *h, int n_per_chain, uint32_t see
   const size_t n = (size_t)n_per_chain * h->n_chains;
   DevBufs B;
   int8_t *dX;
-  float *dy;
+  float *dy, *dt;
   HIP_TRY(B.alloc(&dX, n * kEnvMax));
   HIP_TRY(B.alloc(&dy, n));
-  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy);
+  HIP_TRY(B.alloc(&dt, n));
+  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy, dt);
   if (st != CLEXMC_OK) return st;
   HIP_TRY(hipStreamSynchronize(h->stream));
   HIP_TRY(hipMemcpy(X_out, dX, n * kEnvMax, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(y_out, dy, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (t_out) HIP_TRY(hipMemcpy(t_out, dt, sizeof(float) * n, hipMemcpyDeviceToHost));
   return CLEXMC_OK;
 }
 
@@ -1263,18 +1297,36 @@ clexmc_status clexmc_mlp_train(clexmc_handle *h, int n_per_chain, int n_steps, f
   ARG_CHECK(n < (1u << 24), "training batch too large");
   DevBufs B;
   int8_t *dX;
-  float *dy, *dH, *ddl, *dloss, *dg;
+  float *dy, *dt, *dH, *ddl, *dloss, *dg;
   HIP_TRY(B.alloc(&dX, n * kEnvMax));
   HIP_TRY(B.alloc(&dy, n));
+  HIP_TRY(B.alloc(&dt, n));
   HIP_TRY(B.alloc(&dH, n * kHidden));
   HIP_TRY(B.alloc(&ddl, n));
   HIP_TRY(B.alloc(&dloss, n));
   HIP_TRY(B.alloc(&dg, kNParams));
-  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy);
+  if (!h->mlp_initialized) {
+    // He initialisation of a fresh network (a zero network has dead ReLUs):
+    // W1 ~ N(0, 2/n_env) on the used inputs, b1 = 0.1, W2 ~ N(0, 1/64), b2 = 0
+    std::mt19937_64 rng(0x5EEDull ^ ((uint64_t)seed << 20));
+    std::normal_distribution<float> nw1(0.0f, std::sqrt(2.0f / (float)std::max(1, h->n_env)));
+    std::normal_distribution<float> nw2(0.0f, 0.125f);
+    std::vector<float> p0(kNParams, 0.0f);
+    for (int j = 0; j < kHidden; ++j)
+      for (int k = 0; k < h->n_env; ++k) p0[(size_t)j * kEnvMax + k] = nw1(rng);
+    for (int j = 0; j < kHidden; ++j) p0[kHidden * kEnvMax + j] = 0.1f;
+    for (int j = 0; j < kHidden; ++j) p0[kHidden * kEnvMax + kHidden + j] = nw2(rng);
+    HIP_TRY(hipMemcpy(h->d_mlp_params, p0.data(), sizeof(float) * kNParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(h->d_adam_m, 0, sizeof(float) * kNParams));
+    HIP_TRY(hipMemset(h->d_adam_v, 0, sizeof(float) * kNParams));
+    h->adam_t = 0;
+    h->mlp_initialized = true;
+  }
+  clexmc_status st = collect_device(h, n_per_chain, seed, include_chem, dX, dy, dt);
   if (st != CLEXMC_OK) return st;
   std::vector<float> l(n);
   for (int it = 0; it < n_steps; ++it) {
-    HIP_TRY(launch_mlp_train_step(train_args(h, (int)n, dX, dy, dH, ddl, dloss, dg), h->stream));
+    HIP_TRY(launch_mlp_train_step(train_args(h, (int)n, dX, dy, dt, dH, ddl, dloss, dg), h->stream));
     h->adam_t += 1;
     AdamArgs ad;
     ad.n_params = kNParams;

@@ -28,6 +28,10 @@ constexpr int kMaxSitesPerProp = 4;
 // learned proposal: local-environment MLP (n_env <= kEnvMax inputs, kHidden units)
 constexpr int kEnvMax = 48;
 constexpr int kHidden = 64;
+// The network output z is a flip-energy score in units of kMlpEnergyUnit
+// (eV); a chain at inverse temperature beta uses the logit beta*unit*z, so
+// one network serves every temperature of a batch.
+constexpr float kMlpEnergyUnit = 0.1f;
 
 template <typename T>
 using cptr = const __attribute__((address_space(4))) T *;

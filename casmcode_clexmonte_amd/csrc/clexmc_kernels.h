@@ -66,6 +66,7 @@ struct CollectArgs {
   int include_chem;
   int8_t *X;               // [n_chains * n_per_chain][kEnvMax]
   float *y;                // [n_chains * n_per_chain]
+  float *t;                // [n_chains * n_per_chain] logit scale beta * kMlpEnergyUnit
 };
 
 struct TrainArgs {
@@ -73,6 +74,7 @@ struct TrainArgs {
   int n_env;
   const int8_t *X;         // [n][kEnvMax]
   const float *y;          // [n]
+  const float *t;          // [n] logit scale of each sample (nullptr: 1)
   const float *W1, *b1, *W2, *b2;   // views into the FP32 parameter vector
   float *H;                // [kHidden][n] pre-activations
   float *dl;               // [n]
@@ -87,10 +89,12 @@ struct AdamArgs {
   float lr, b1m, b2m, b1t, b2t;  // b1t = b1m^t, b2t = b2m^t
 };
 
-hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, bool use_mlp, int n_chains,
+hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
                           int threads, size_t lds, hipStream_t st);
 size_t learned_lds_bytes(int pad_bytes);
 hipError_t launch_mlp_infer(const MlpDev *mlp, int n, const int8_t *X, float *out, hipStream_t st);
+hipError_t launch_wave_pick(int n_rows, const float *v, const float *u, float *scan_out, int *pick_out,
+                            hipStream_t st);
 hipError_t launch_mlp_collect(const CollectArgs &a, int n_chains, hipStream_t st);
 hipError_t launch_mlp_train_step(const TrainArgs &a, hipStream_t st);
 hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);

@@ -10,15 +10,26 @@
 // accumulation.  Training runs FP32 forward/backward kernels on samples drawn
 // from the chains' current (plain-sampler) configurations.
 //
+// The network output z is a score of the site's flip in energy units; a
+// chain at inverse temperature beta uses the logit t*z with
+// t = beta * kMlpEnergyUnit, so the trained weights are shared by all chain
+// temperatures.  Training labels are the heat-bath (Barker) flip
+// probabilities sigmoid(-beta e_s) of sites of plain-sampler configurations.
+//
 // Proposal (exact Metropolis-Hastings): a class of an independent-set
 // partition is split into groups of 64 sites (one per lane; the grouping is a
 // random rotation chosen independently of the occupation).  Within a group a
 // wavefront makes sequential proposals: canonical, an A site a and a B site b
 // with probabilities w_a / Z_A and w_b / Z_B; semi-grand canonical, one site
-// s with probability w_s / Z.  w = eps + sigmoid(logit).  Sites of one class
-// share no cluster, so dE = e_a + e_b is exact and, after an accepted move,
-// each moved site just swaps its (w, w_flipped) and negates its flip energy;
-// Z_A, Z_B are updated in O(1) and the reverse-proposal probability is exact.
+// s with probability w_s / Z.  Weights (WEIGHTS template argument):
+//   0 uniform  w = 1;
+//   1 learned  w = eps + sigmoid(t z(env));
+//   2 informed w = eps + sigmoid(-beta e_s), the exact Barker weight of the
+//     site's flip energy (no network: the ablation of the learned weights).
+// Sites of one class share no cluster, so dE = e_a + e_b is exact and, after
+// an accepted move, each moved site just swaps its (w, w_flipped) and negates
+// its flip energy; Z_A, Z_B are updated in O(1) and the reverse-proposal
+// probability is exact.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -66,21 +77,29 @@ __device__ __forceinline__ void load_frags(cptr<MlpDev> m, int lane, MlpFrags &F
   F.n_env = m->n_env;
 }
 
-// Logits of the 64 sites of a wavefront for their current occupation (outC)
-// and the flipped one (outN); feat(s, k) returns the +1/-1/0 input k of the
-// group's site s.  Results are written to LDS scratch by lanes with l%16 == 0.
-template <typename Feat>
-__device__ __forceinline__ void mlp_wave(const MlpFrags &F, Feat feat, int lane, float *outC, float *outN) {
+// Network outputs of the 64 sites of a wavefront for their current
+// occupation (outC) and the flipped one (outN).  fm_lo/fm_hi hold each lane's
+// own site's environment as bits (bit k set: environment site k holds the
+// same occupant); inputs k >= n_env are zero.  Results are written to LDS
+// scratch by lanes with l%16 == 0.
+__device__ __forceinline__ void mlp_wave(const MlpFrags &F, uint32_t fm_lo, uint32_t fm_hi, int lane, float *outC,
+                                         float *outN) {
   const int g = lane >> 4, c = lane & 15;
 #pragma unroll
   for (int tm = 0; tm < 4; ++tm) {
-    short4_t A[3];
     const int s = 16 * tm + c;
+    const uint32_t lo = (uint32_t)__shfl((int)fm_lo, s, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)fm_hi, s, 64);
+    short4_t A[3];
 #pragma unroll
     for (int tk = 0; tk < 3; ++tk) {
       short4_t v;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = bf16_of_sign(feat(s, 16 * tk + 4 * g + q));
+      for (int q = 0; q < 4; ++q) {
+        const int k = 16 * tk + 4 * g + q;
+        const uint32_t bit = k < 32 ? (lo >> k) & 1u : (hi >> (k - 32)) & 1u;
+        v[q] = k < F.n_env ? (bit ? (short)0x3F80 : (short)0xBF80) : (short)0;
+      }
       A[tk] = v;
     }
     float pc[4] = {0.f, 0.f, 0.f, 0.f}, pn[4] = {0.f, 0.f, 0.f, 0.f};
@@ -135,15 +154,29 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
-// lane chosen with probability v_lane / sum(v) given t = u * sum(v)
-__device__ __forceinline__ int wave_select(float v, float t, int lane) {
-  float x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const float y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  unsigned long long m = __ballot(x > t && v > 0.f);
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP: row shifts
+// by 1, 2, 4, 8 (rows of 16 lanes; lanes without a source add 0), then the
+// row-15 and row-31 broadcasts that carry row totals into the next rows.
+__device__ __forceinline__ float wave_scan_incl(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xA, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xC, 0xF, false));
+  return x;
+}
+
+__device__ __forceinline__ float lane_f(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Weighted choice of a lane: v >= 0 per lane, scan = wave_scan_incl(v),
+// t = u * total.  Returns the first lane whose inclusive sum exceeds t among
+// lanes with v > 0 (the last such lane if rounding leaves none), -1 if all v
+// are 0.  The result is wavefront-uniform.
+__device__ __forceinline__ int wave_pick(float v, float scan, float t) {
+  unsigned long long m = __ballot(scan > t && v > 0.f);
   if (m == 0ull) {
     m = __ballot(v > 0.f);
     if (m == 0ull) return -1;
@@ -195,10 +228,10 @@ __device__ __forceinline__ float binary_field_l(const uint8_t *img, int pa, int 
 }
 
 // ---------------------------------------------------------------------------
-// Learned-proposal Metropolis kernel (binary path).  KIND 0 canonical, 1 SGC.
-// USE_MLP false gives the same group structure with uniform weights (w = 1).
+// Grouped-proposal Metropolis kernel (binary path).  KIND 0 canonical, 1 SGC;
+// WEIGHTS 0 uniform, 1 learned network, 2 informed (exact Barker weights).
 // ---------------------------------------------------------------------------
-template <int KIND, bool USE_MLP>
+template <int KIND, int WEIGHTS>
 __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int chain = blockIdx.x;
@@ -244,23 +277,30 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
   if (tid == 0) *maxDraw = 0ull;
   __syncthreads();
 
+  constexpr bool USE_MLP = WEIGHTS == 1;
   MlpFrags F;
   if (USE_MLP) load_frags(mlp, lane, F);
   const float beta = cp->beta;
+  const float tsc = beta * kMlpEnergyUnit;
+  const float eps_w = mlp->eps;
   // per-lane and per-wave random streams
   unsigned long long draw = cs->draw_base;
   uint32_t k0 = cp->key0, k1 = cp->key1;
+  draw = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(draw >> 32)) << 32) |
+         (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)draw);
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
 
   int accC[kMaxSlots];
 #pragma unroll
   for (int a = 0; a < kMaxSlots; ++a) accC[a] = 0;
-  int accN1 = 0, accAtt = 0, accAcc = 0;
+  int accN1 = 0, accAtt = 0, accAcc = 0;  // per lane (pair phases, composition)
+  int rAtt = 0, rAcc = 0;                  // per wavefront (grouped proposals; wave-uniform)
 
   const unsigned long long pass0 = cs->pass_base;
   cptr<int> bin_off = cst(sc->bin_off);
   const int R = la.rounds_per_group < 1 ? 1 : la.rounds_per_group;
-  float *myC = logitC + wave * 64;
-  float *myN = logitN + wave * 64;
+  float *myC = logitC + wave_u * 64;
+  float *myN = logitN + wave_u * 64;
 
   for (int pass = 0; pass < args.n_passes; ++pass) {
     const unsigned long long gpass = pass0 + (unsigned long long)pass;
@@ -275,100 +315,110 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
       uint4 rr = philox4x32_10(make_uint4(0xFFFFFFFDu, (uint32_t)gpass, (uint32_t)c, (uint32_t)chain), k0, k1);
       const int rot = n ? (int)(rr.x % (uint32_t)n) : 0;
       const int n_groups = (n + 63) / 64;
-      for (int grp = wave; grp < n_groups; grp += n_waves) {
+      for (int grp = wave_u; grp < n_groups; grp += n_waves) {
         const int p = grp * 64 + lane;
         const bool valid = p < n;
         int q = p + rot;
         if (q >= n) q -= n;
         const int pa = valid ? sc->class_addr[cb + q] : sc->class_addr[cb];
-        int o = valid ? (int)img[pa] : 0;
+        const int o0 = valid ? (int)img[pa] : 0;
+        int o = o0;
         int K[kMaxSlots];
         const float Fl = binary_field_l(img, pa, o, bs, bin_off, n_slots, K);
         float e = (o ? -2.0f : 2.0f) * Fl;  // flip energy of the site
         if (KIND == 1) e -= o ? (chem0 - chem1) : (chem1 - chem0);
         float w = 1.0f, wn = 1.0f;
         if (USE_MLP) {
-          const int n_env = F.n_env;
-          auto feat = [&](int s, int kk) -> int {
-            const int pas = __shfl(pa, s, 64);
-            const int os = __shfl(o, s, 64);
-            if (kk >= n_env) return 0;
-            return (int)img[pas + env[kk]] == os ? 1 : -1;
-          };
-          mlp_wave(F, feat, lane, myC, myN);
+          uint32_t fm_lo = 0u, fm_hi = 0u;
+          for (int kk = 0; kk < F.n_env; ++kk) {
+            const uint32_t same = (int)img[pa + env[kk]] == o ? 1u : 0u;
+            if (kk < 32) fm_lo |= same << kk;
+            else fm_hi |= same << (kk - 32);
+          }
+          mlp_wave(F, fm_lo, fm_hi, lane, myC, myN);
           __builtin_amdgcn_s_waitcnt(0);
           __builtin_amdgcn_wave_barrier();
-          w = F.eps + sigmoidf_(myC[lane]);
-          wn = F.eps + sigmoidf_(myN[lane]);
+          w = F.eps + sigmoidf_(tsc * myC[lane]);
+          wn = F.eps + sigmoidf_(tsc * myN[lane]);
+        } else if (WEIGHTS == 2) {
+          w = eps_w + sigmoidf_(-beta * e);
+          wn = eps_w + sigmoidf_(beta * e);
         }
         if (!valid) {
           w = 0.0f;
           wn = 0.0f;
         }
+        uint4 ul = make_uint4(0u, 0u, 0u, 0u);
         for (int rd = 0; rd < R; ++rd) {
-          // three wave-uniform uniforms
-          const uint4 ur = philox4x32_10(make_uint4(0xFFFF0000u | (uint32_t)wave, (uint32_t)draw, (uint32_t)(draw >> 32),
-                                                   (uint32_t)chain), k0, k1);
-          ++draw;
-          const float u1 = u01(ur.x), u2 = u01(ur.y), u3 = u01(ur.z);
+          // uniforms of 64 rounds are drawn lane-parallel (lane l: round l's
+          // Philox block) and read out per round
+          const int slot = rd & 63;
+          if (slot == 0) {
+            const unsigned long long dl = draw + (unsigned long long)lane;
+            ul = philox4x32_10(make_uint4(0xFFFF0000u | (uint32_t)wave_u, (uint32_t)dl, (uint32_t)(dl >> 32),
+                                          (uint32_t)chain), k0, k1);
+            draw += 64ull;
+          }
+          const float u1 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.x, slot));
+          const float u2 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.y, slot));
+          const float u3 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.z, slot));
           if (KIND == 0) {
-            const float vA = (valid && o == 0) ? w : 0.0f;
-            const float vB = (valid && o == 1) ? w : 0.0f;
-            const float ZA = wave_sum_f(vA), ZB = wave_sum_f(vB);
+            const float vA = (o == 0) ? w : 0.0f;
+            const float vB = (o == 1) ? w : 0.0f;
+            const float sA = wave_scan_incl(vA), sB = wave_scan_incl(vB);
+            const float ZA = lane_f(sA, 63), ZB = lane_f(sB, 63);
             if (!(ZA > 0.0f) || !(ZB > 0.0f)) break;
-            const int la_ = wave_select(vA, u1 * ZA, lane);
-            const int lb_ = wave_select(vB, u2 * ZB, lane);
+            const int la_ = wave_pick(vA, sA, u1 * ZA);
+            const int lb_ = wave_pick(vB, sB, u2 * ZB);
             if (la_ < 0 || lb_ < 0) break;
-            const float wa = __shfl(w, la_, 64), wna = __shfl(wn, la_, 64), ea = __shfl(e, la_, 64);
-            const float wb = __shfl(w, lb_, 64), wnb = __shfl(wn, lb_, 64), eb = __shfl(e, lb_, 64);
+            const float wa = lane_f(w, la_), wna = lane_f(wn, la_), ea = lane_f(e, la_);
+            const float wb = lane_f(w, lb_), wnb = lane_f(wn, lb_), eb = lane_f(e, lb_);
             const float dE = ea + eb;
             const float ZA2 = ZA - wa + wnb, ZB2 = ZB - wb + wna;
-            const float ratio = (wnb * wna * ZA * ZB) / (wa * wb * ZA2 * ZB2);
-            const float pacc = __expf(-beta * dE) * ratio;
-            const bool acc = u3 < pacc;
-            if (lane == 0) {
-              accAtt += 1;
-              accAcc += acc ? 1 : 0;
-            }
-            if (acc && (lane == la_ || lane == lb_)) {
-              const int sk = o ? -2 : 2;
-#pragma unroll
-              for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
-              o = 1 - o;
-              img[pa] = (uint8_t)o;
-              e = -e;
-              const float t = w;
-              w = wn;
-              wn = t;
+            const float ratio = __fdividef(wnb * wna * ZA * ZB, wa * wb * ZA2 * ZB2);
+            const bool acc = u3 < __expf(-beta * dE) * ratio;
+            rAtt += 1;
+            if (acc) {
+              rAcc += 1;
+              if (lane == la_ || lane == lb_) {
+                o ^= 1;
+                e = -e;
+                const float t = w;
+                w = wn;
+                wn = t;
+              }
             }
           } else {
-            const float v = valid ? w : 0.0f;
-            const float Z = wave_sum_f(v);
+            const float sv = wave_scan_incl(w);
+            const float Z = lane_f(sv, 63);
             if (!(Z > 0.0f)) break;
-            const int ls = wave_select(v, u1 * Z, lane);
+            const int ls = wave_pick(w, sv, u1 * Z);
             if (ls < 0) break;
-            const float ws = __shfl(w, ls, 64), wns = __shfl(wn, ls, 64), es = __shfl(e, ls, 64);
+            const float ws = lane_f(w, ls), wns = lane_f(wn, ls), es = lane_f(e, ls);
             const float Z2 = Z - ws + wns;
-            const float ratio = (wns * Z) / (ws * Z2);
+            const float ratio = __fdividef(wns * Z, ws * Z2);
             const bool acc = u2 < __expf(-beta * es) * ratio;
-            if (lane == 0) {
-              accAtt += 1;
-              accAcc += acc ? 1 : 0;
-            }
-            if (acc && lane == ls) {
-              const int sk = o ? -2 : 2;
-#pragma unroll
-              for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
-              accN1 += o ? -1 : 1;
-              o = 1 - o;
-              img[pa] = (uint8_t)o;
-              e = -e;
-              const float t = w;
-              w = wn;
-              wn = t;
+            rAtt += 1;
+            if (acc) {
+              rAcc += 1;
+              if (lane == ls) {
+                o ^= 1;
+                e = -e;
+                const float t = w;
+                w = wn;
+                wn = t;
+              }
             }
             (void)u3;
           }
+        }
+        // sites of one class share no cluster: apply the group's net changes
+        if (valid && o != o0) {
+          img[pa] = (uint8_t)o;
+          const int sk = o0 ? -2 : 2;
+#pragma unroll
+          for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+          if (KIND == 1) accN1 += o0 ? -1 : 1;
         }
       }
       __syncthreads();
@@ -429,13 +479,14 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
         accC[a] = 0;
       }
       {
-        const int v0 = wave_sum_i(accN1), v1 = wave_sum_i(accAtt), v2 = wave_sum_i(accAcc);
+        const int v0 = wave_sum_i(accN1), v1 = wave_sum_i(accAtt) + rAtt, v2 = wave_sum_i(accAcc) + rAcc;
         if (lane == 0) {
           redI[(kMaxSlots + 0) * 16 + wave] = v0;
           redI[(kMaxSlots + 1) * 16 + wave] = v1;
           redI[(kMaxSlots + 2) * 16 + wave] = v2;
         }
         accN1 = accAtt = accAcc = 0;
+        rAtt = rAcc = 0;
       }
       __syncthreads();
       if (tid == 0) {
@@ -502,12 +553,16 @@ __global__ __launch_bounds__(64) void mlp_infer_kernel(const MlpDev *mlp, int n,
   const int base = blockIdx.x * 64;
   MlpFrags F;
   load_frags(cst(mlp), lane, F);
-  auto feat = [&](int s, int k) -> int {
-    const int row = base + s;
-    if (row >= n || k >= F.n_env) return 0;
-    return (int)X[(long long)row * kEnvMax + k];
-  };
-  mlp_wave(F, feat, lane, sC, sN);
+  // row of this lane as environment bits (input k > 0 <=> bit k set)
+  uint32_t lo = 0u, hi = 0u;
+  const int row = base + lane;
+  if (row < n)
+    for (int k = 0; k < F.n_env; ++k) {
+      const uint32_t bit = X[(long long)row * kEnvMax + k] > 0 ? 1u : 0u;
+      if (k < 32) lo |= bit << k;
+      else hi |= bit << (k - 32);
+    }
+  mlp_wave(F, lo, hi, lane, sC, sN);
   __syncthreads();
   if (base + lane < n) {
     out[2 * (base + lane)] = sC[lane];
@@ -516,40 +571,43 @@ __global__ __launch_bounds__(64) void mlp_infer_kernel(const MlpDev *mlp, int n,
 }
 
 // ---------------------------------------------------------------------------
-// Training data from the chains' current configurations: features of random
-// variable sites and the heat-bath flip probability sigmoid(-beta e_s) as a
-// soft label (e_s: flip energy incl. the chemical term when include_chem).
+// Diagnostic: the wavefront scan and weighted lane choice of the grouped
+// proposals on given weights (known-answer checks against the CPU)
 // ---------------------------------------------------------------------------
-__global__ void mlp_collect_kernel(CollectArgs a) {
-  const int chain = blockIdx.y;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n_per_chain) return;
+__global__ __launch_bounds__(64) void wave_pick_kernel(int n_rows, const float *v, const float *u, float *scan_out,
+                                                       int *pick_out) {
+  const int row = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (row >= n_rows) return;
+  const float x = v[(long long)row * 64 + lane];
+  const float sc = wave_scan_incl(x);
+  const float total = lane_f(sc, 63);
+  const int pk = wave_pick(x, sc, u[row] * total);
+  scan_out[(long long)row * 64 + lane] = sc;
+  if (lane == 0) pick_out[row] = pk;
+}
+
+hipError_t launch_wave_pick(int n_rows, const float *v, const float *u, float *scan_out, int *pick_out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(wave_pick_kernel, dim3(n_rows), dim3(64), 0, st, n_rows, v, u, scan_out, pick_out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Training data from the chains' current configurations: features of random
+// variable sites, the heat-bath flip probability sigmoid(-beta e_s) as a
+// soft label (e_s: flip energy incl. the chemical term when include_chem)
+// and the chain's logit scale t = beta * kMlpEnergyUnit.
+// ---------------------------------------------------------------------------
+// Flip energy of variable site l (binary: to the other occupant) from the
+// flower tables, including the chemical term when include_chem.
+__device__ float collect_flip_energy(const CollectArgs &a, const ChainParams &cp, const DevSupercell &sc,
+                                     const int8_t *__restrict__ occ, int l) {
   const DevSystem *__restrict__ sys = a.sys;
-  const ChainParams &cp = a.params[chain];
-  const DevSupercell &sc = a.scs[cp.sc];
-  const int8_t *__restrict__ occ = a.occ + cp.occ_offset;
-  const uint4 r = philox4x32_10(make_uint4((uint32_t)t, a.seed, 0xC011EC7u, (uint32_t)chain), cp.key0, cp.key1);
-  // a variable site
-  int l = (int)(r.x % (uint32_t)sc.n_sites);
-  for (int tries = 0; tries < sc.n_sites && sys->n_occ[l / sc.n_cells] < 2; ++tries) l = (l + 1) % sc.n_sites;
   const int b = l / sc.n_cells;
   const int c = l - b * sc.n_cells;
   const int i0 = c % sc.L1, j0 = (c / sc.L1) % sc.L2, k0 = c / (sc.L1 * sc.L2);
   const int o = occ[l];
-  const long long out = (long long)chain * a.n_per_chain + t;
-  for (int k = 0; k < kEnvMax; ++k) {
-    int8_t f = 0;
-    if (k < a.n_env) {
-      const int4 d = a.env_sites[b * kEnvMax + k];
-      int ii = (i0 + d.y) % sc.L1, jj = (j0 + d.z) % sc.L2, kk = (k0 + d.w) % sc.L3;
-      if (ii < 0) ii += sc.L1;
-      if (jj < 0) jj += sc.L2;
-      if (kk < 0) kk += sc.L3;
-      f = occ[d.x * sc.n_cells + ii + sc.L1 * (jj + sc.L2 * kk)] == o ? 1 : -1;
-    }
-    a.X[out * kEnvMax + k] = f;
-  }
-  // flip energy (binary: to the other occupant) from the flower tables
   const int nw = 1 - o;
   float G[kMaxFunc] = {0.f, 0.f, 0.f};
   const int nf = sys->n_func[b];
@@ -569,27 +627,78 @@ __global__ void mlp_collect_kernel(CollectArgs a) {
   float dE = 0.0f;
   for (int f = 0; f < nf; ++f) dE += (sys->site_func[b][f][nw] - sys->site_func[b][f][o]) * G[f];
   if (a.include_chem) dE -= cp.chem[b * kMaxOcc + nw] - cp.chem[b * kMaxOcc + o];
-  a.y[out] = 1.0f / (1.0f + __expf(cp.beta * dE));
+  return dE;
+}
+
+// Sample t of a chain: 8 random variable sites are drawn; even t keep the
+// first, odd t the one with the largest flip probability.  Rare mobile
+// environments (the sites a weighted proposal should find) thus make up
+// about half of the training set even in well-ordered configurations.
+__global__ void mlp_collect_kernel(CollectArgs a) {
+  const int chain = blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_per_chain) return;
+  const DevSystem *__restrict__ sys = a.sys;
+  const ChainParams &cp = a.params[chain];
+  const DevSupercell &sc = a.scs[cp.sc];
+  const int8_t *__restrict__ occ = a.occ + cp.occ_offset;
+  const uint4 r0 = philox4x32_10(make_uint4((uint32_t)t, a.seed, 0xC011EC7u, (uint32_t)chain), cp.key0, cp.key1);
+  const uint4 r1 = philox4x32_10(make_uint4((uint32_t)t, a.seed, 0xC011EC8u, (uint32_t)chain), cp.key0, cp.key1);
+  const uint32_t rr[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+  const int n_cand = (t & 1) ? 8 : 1;
+  int l = -1;
+  float best_y = -1.0f;
+  for (int m = 0; m < n_cand; ++m) {
+    int lc = (int)(rr[m] % (uint32_t)sc.n_sites);
+    for (int tries = 0; tries < sc.n_sites && sys->n_occ[lc / sc.n_cells] < 2; ++tries) lc = (lc + 1) % sc.n_sites;
+    const float dE = collect_flip_energy(a, cp, sc, occ, lc);
+    const float y = 1.0f / (1.0f + __expf(cp.beta * dE));
+    if (y > best_y) {
+      best_y = y;
+      l = lc;
+    }
+  }
+  const int b = l / sc.n_cells;
+  const int c = l - b * sc.n_cells;
+  const int i0 = c % sc.L1, j0 = (c / sc.L1) % sc.L2, k0 = c / (sc.L1 * sc.L2);
+  const int o = occ[l];
+  const long long out = (long long)chain * a.n_per_chain + t;
+  for (int k = 0; k < kEnvMax; ++k) {
+    int8_t f = 0;
+    if (k < a.n_env) {
+      const int4 d = a.env_sites[b * kEnvMax + k];
+      int ii = (i0 + d.y) % sc.L1, jj = (j0 + d.z) % sc.L2, kk = (k0 + d.w) % sc.L3;
+      if (ii < 0) ii += sc.L1;
+      if (jj < 0) jj += sc.L2;
+      if (kk < 0) kk += sc.L3;
+      f = occ[d.x * sc.n_cells + ii + sc.L1 * (jj + sc.L2 * kk)] == o ? 1 : -1;
+    }
+    a.X[out * kEnvMax + k] = f;
+  }
+  a.y[out] = best_y;
+  a.t[out] = cp.beta * kMlpEnergyUnit;
 }
 
 // ---------------------------------------------------------------------------
 // Training: FP32 forward (per sample), gradients (per parameter), Adam.
-// Loss: binary cross-entropy of sigmoid(logit) against the soft label y,
-// averaged over the batch.
+// Loss: binary cross-entropy of sigmoid(t z) against the soft label y,
+// averaged over the batch (z: network output, t: the sample's logit scale).
 // ---------------------------------------------------------------------------
 __global__ void mlp_forward_train_kernel(TrainArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const float *W1 = a.W1;  // [64][48]
-  float logit = a.b2[0];
+  float z = a.b2[0];
   for (int j = 0; j < kHidden; ++j) {
     float hj = a.b1[j];
     for (int k = 0; k < a.n_env; ++k) hj += W1[j * kEnvMax + k] * (float)a.X[(long long)s * kEnvMax + k];
     a.H[(long long)j * a.n + s] = hj;
-    logit += a.W2[j] * fmaxf(hj, 0.f);
+    z += a.W2[j] * fmaxf(hj, 0.f);
   }
+  const float ts = a.t ? a.t[s] : 1.0f;
+  const float logit = ts * z;
   const float p = 1.0f / (1.0f + __expf(-logit));
-  a.dl[s] = p - a.y[s];
+  a.dl[s] = ts * (p - a.y[s]);  // d loss / d z
   const float y = a.y[s];
   const float lp = logit >= 0 ? -log1pf(__expf(-logit)) : logit - log1pf(__expf(logit));      // log p
   const float lq = logit >= 0 ? -logit - log1pf(__expf(-logit)) : -log1pf(__expf(logit));     // log(1-p)
@@ -662,22 +771,26 @@ __global__ void mlp_publish_kernel(const float *p, MlpDev *m, int n_env, float e
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-template <int KIND, bool USE_MLP>
+template <int KIND, int WEIGHTS>
 static hipError_t launch_l(const RunArgs &args, const LearnArgs &la, int n_chains, int threads, size_t lds,
                            hipStream_t st) {
-  auto kern = learned_kernel<KIND, USE_MLP>;
+  auto kern = learned_kernel<KIND, WEIGHTS>;
   hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args, la);
   return hipGetLastError();
 }
 
-hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, bool use_mlp, int n_chains,
+hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
                           int threads, size_t lds, hipStream_t st) {
-  if (kind == 0) return use_mlp ? launch_l<0, true>(args, la, n_chains, threads, lds, st)
-                                : launch_l<0, false>(args, la, n_chains, threads, lds, st);
-  return use_mlp ? launch_l<1, true>(args, la, n_chains, threads, lds, st)
-                 : launch_l<1, false>(args, la, n_chains, threads, lds, st);
+  if (kind == 0) {
+    if (weights == 1) return launch_l<0, 1>(args, la, n_chains, threads, lds, st);
+    if (weights == 2) return launch_l<0, 2>(args, la, n_chains, threads, lds, st);
+    return launch_l<0, 0>(args, la, n_chains, threads, lds, st);
+  }
+  if (weights == 1) return launch_l<1, 1>(args, la, n_chains, threads, lds, st);
+  if (weights == 2) return launch_l<1, 2>(args, la, n_chains, threads, lds, st);
+  return launch_l<1, 0>(args, la, n_chains, threads, lds, st);
 }
 
 size_t learned_lds_bytes(int pad_bytes) {

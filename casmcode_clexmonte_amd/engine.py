@@ -70,7 +70,8 @@ def lib():
                      "clexmc_clear_samples", "clexmc_read_state", "clexmc_read_binary_counts",
                      "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device",
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
-                     "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train"]:
+                     "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
+                     "clexmc_wave_pick_device"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -86,7 +87,8 @@ def exported_symbols() -> List[str]:
             "clexmc_read_binary_counts", "clexmc_last_run_ms", "clexmc_set_block_size",
             "clexmc_philox_device", "clexmc_last_error", "clexmc_destroy", "clexmc_version",
             "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
-            "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train"]
+            "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
+            "clexmc_wave_pick_device"]
 
 
 def _p(a, t):
@@ -359,7 +361,7 @@ class Engine:
 
     # -- learned proposal ------------------------------------------------------
     def set_proposal(self, proposal: str, rounds_per_group: int = 64):
-        code = {"plain": 0, "learned": 1, "grouped_uniform": 2}[proposal]
+        code = {"plain": 0, "learned": 1, "grouped_uniform": 2, "informed": 3}[proposal]
         self._check(lib().clexmc_set_proposal(self.h, C.c_int(code), C.c_int(rounds_per_group)),
                     "clexmc_set_proposal")
 
@@ -388,24 +390,29 @@ class Engine:
                     "clexmc_mlp_forward")
         return out
 
-    def mlp_gradients(self, X: np.ndarray, y: np.ndarray):
+    def mlp_gradients(self, X: np.ndarray, y: np.ndarray, t: Optional[np.ndarray] = None):
         X = np.ascontiguousarray(X, dtype=np.int8).reshape(-1, 48)
         y = np.ascontiguousarray(y, dtype=np.float32)
+        tp = f32p()
+        if t is not None:
+            t = np.ascontiguousarray(t, dtype=np.float32)
+            tp = _p(t, f32p)
         _, n_params, _ = self.mlp_info()
         g = np.zeros(n_params, dtype=np.float32)
         loss = C.c_float()
-        self._check(lib().clexmc_mlp_gradients(self.h, C.c_int(X.shape[0]), _p(X, i8p), _p(y, f32p), _p(g, f32p),
-                                               C.byref(loss)), "clexmc_mlp_gradients")
+        self._check(lib().clexmc_mlp_gradients(self.h, C.c_int(X.shape[0]), _p(X, i8p), _p(y, f32p), tp,
+                                               _p(g, f32p), C.byref(loss)), "clexmc_mlp_gradients")
         return g, loss.value
 
     def mlp_collect(self, n_per_chain: int, seed: int = 0, include_chem: bool = False):
         n = n_per_chain * self.n_chains
         X = np.zeros((n, 48), dtype=np.int8)
         y = np.zeros(n, dtype=np.float32)
+        t = np.zeros(n, dtype=np.float32)
         self._check(lib().clexmc_mlp_collect(self.h, C.c_int(n_per_chain), C.c_uint32(seed),
-                                             C.c_int(1 if include_chem else 0), _p(X, i8p), _p(y, f32p)),
-                    "clexmc_mlp_collect")
-        return X, y
+                                             C.c_int(1 if include_chem else 0), _p(X, i8p), _p(y, f32p),
+                                             _p(t, f32p)), "clexmc_mlp_collect")
+        return X, y, t
 
     def mlp_train(self, n_per_chain: int, n_steps: int, lr: float = 1e-3, seed: int = 0,
                   include_chem: bool = False) -> np.ndarray:
@@ -414,3 +421,12 @@ class Engine:
                                            C.c_uint32(seed), C.c_int(1 if include_chem else 0), _p(loss, f32p)),
                     "clexmc_mlp_train")
         return loss
+
+    def wave_pick(self, v: np.ndarray, u: np.ndarray):
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 64)
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        scan = np.zeros_like(v)
+        pick = np.zeros(v.shape[0], dtype=np.int32)
+        self._check(lib().clexmc_wave_pick_device(self.h, C.c_int(v.shape[0]), _p(v, f32p), _p(u, f32p),
+                                                  _p(scan, f32p), _p(pick, i32p)), "clexmc_wave_pick_device")
+        return scan, pick

@@ -92,18 +92,24 @@ def fcc_sgc_grid(sc: Supercell, n_T: int = 16, n_mu: int = 16, T_range=(200.0, 1
 ORDERING_ECI_SEED = 20240612
 
 
-def fcc_ordering_ce(seed: int = ORDERING_ECI_SEED, nn_pair: float = 0.04, sigma: float = 0.005) -> ClusterExpansion:
-    """Binary FCC (same clusters as fcc_binary_ce) with a positive nearest-
-    neighbour pair ECI (+0.04 eV, favouring unlike neighbours: L1_2 / L1_0
-    ordering) and small seeded ECIs on the other clusters."""
+def fcc_ordering_ce(seed: int = ORDERING_ECI_SEED, nn_pair: float = 0.04, nnn_pair: float = -0.004,
+                    sigma: float = 0.001) -> ClusterExpansion:
+    """Binary FCC (same clusters as fcc_binary_ce) with ordering ECIs: a
+    positive nearest-neighbour pair ECI (+0.04 eV, favouring unlike
+    neighbours), a weak negative second-neighbour pair ECI (-0.004 eV; it
+    lifts the degeneracy of the nearest-neighbour model at x_B = 1/4 in
+    favour of L1_2, whose second-neighbour pairs are all like pairs) and
+    small seeded ECIs N(0, 0.001 eV) on the other pairs and triplets."""
     bs = fcc_binary_basis()
     rng = np.random.default_rng(seed)
     eci = np.zeros(bs.n_functions)
-    nn_orbit = next(o for o in bs.orbits if o.size == 2)
+    pairs = sorted((o for o in bs.orbits if o.size == 2), key=lambda o: o.max_length)
     for orb in bs.orbits:
         for fi in orb.functions:
-            if orb is nn_orbit:
+            if orb is pairs[0]:
                 eci[fi] = nn_pair
+            elif orb is pairs[1]:
+                eci[fi] = nnn_pair
             elif orb.size >= 2:
                 eci[fi] = rng.normal(0.0, sigma)
     return ClusterExpansion(bs, eci)

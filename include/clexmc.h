@@ -187,7 +187,9 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
 /* Proposal used by clexmc_run_passes: 0 plain (independent-set sweeps and
  * random matchings), 1 learned (MLP-weighted sequential proposals within
  * groups of 64 independent sites), 2 the same grouped proposals with uniform
- * weights.  rounds_per_group: proposals per group and class phase. */
+ * weights, 3 the same with informed weights eps + sigmoid(-beta e_s) of the
+ * exact flip energy (no network).  rounds_per_group: proposals per group and
+ * class phase.  Binary cluster expansions only. */
 clexmc_status clexmc_set_proposal(clexmc_handle *h, int proposal, int rounds_per_group);
 
 /* Network size: n_env inputs (sites of the three shortest pair orbits),
@@ -197,24 +199,38 @@ clexmc_status clexmc_mlp_info(clexmc_handle *h, int *n_env, int *n_params, int *
 clexmc_status clexmc_mlp_set_params(clexmc_handle *h, const float *params, float eps);
 clexmc_status clexmc_mlp_get_params(clexmc_handle *h, float *params);
 
-/* MFMA (BF16) inference on feature rows X [n][48] (+1/-1/0): out [n][2] =
- * logits for the current and the flipped occupation. */
+/* MFMA (BF16) inference on feature rows X [n][48] (+1/-1 on the first n_env
+ * inputs, the rest ignored): out [n][2] =
+ * network outputs z for the current and the flipped occupation (a chain at
+ * inverse temperature beta uses the logit beta * 0.1 eV * z). */
 clexmc_status clexmc_mlp_forward(clexmc_handle *h, int n, const int8_t *X, float *out);
 
-/* FP32 training forward/backward on (X, y): mean binary cross-entropy and its
- * gradient with respect to the n_params parameters. */
-clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, float *grad_out,
-                                   float *loss_out);
+/* FP32 training forward/backward on (X, y, t): mean binary cross-entropy of
+ * sigmoid(t * z) against y and its gradient with respect to the n_params
+ * parameters (t NULL: t = 1). */
+clexmc_status clexmc_mlp_gradients(clexmc_handle *h, int n, const int8_t *X, const float *y, const float *t,
+                                   float *grad_out, float *loss_out);
 
 /* Training samples from the loaded chains' current configurations:
- * X [n_chains*n_per_chain][48], y = sigmoid(-beta * flip energy). */
+ * X [n_chains*n_per_chain][48], y = sigmoid(-beta * flip energy),
+ * t = beta * 0.1 eV (t_out may be NULL). */
 clexmc_status clexmc_mlp_collect(clexmc_handle *h, int n_per_chain, uint32_t seed, int include_chem, int8_t *X_out,
-                                 float *y_out);
+                                 float *y_out, float *t_out);
 
 /* Collects n_per_chain samples per chain and runs n_steps Adam steps on the
- * device (hand-written forward/backward kernels); loss_out [n_steps] or NULL. */
+ * device (hand-written forward/backward kernels); loss_out [n_steps] or NULL.
+ * A network never set by clexmc_mlp_set_params nor trained before starts
+ * from a seeded He initialisation. */
 clexmc_status clexmc_mlp_train(clexmc_handle *h, int n_per_chain, int n_steps, float lr, uint32_t seed,
                                int include_chem, float *loss_out);
+
+/* The grouped proposals' wavefront primitives on given data (known-answer
+ * checks): per row of 64 weights v >= 0, the inclusive prefix sums and the
+ * chosen lane for the uniform u[row] (first lane with prefix > u*total and
+ * v > 0; -1 if all weights are 0). */
+clexmc_status clexmc_wave_pick_device(clexmc_handle *h, int n_rows, const float *v /*[n_rows][64]*/,
+                                      const float *u /*[n_rows]*/, float *scan_out /*[n_rows][64]*/,
+                                      int *pick_out /*[n_rows]*/);
 
 /* Philox4x32-10 evaluated on the device (known-answer checks). */
 clexmc_status clexmc_philox_device(clexmc_handle *h, int n, const uint32_t *ctr /*[n][4]*/,

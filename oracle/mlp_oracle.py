@@ -3,8 +3,9 @@ ONLY: used by tests/ as the checker of the GPU inference and training
 kernels).
 
 Network: inputs f (n_env of +-1, zero padded to 48), h = W1 f + b1,
-logit = W2 . relu(h) + b2; the flipped-occupation logit uses -f.
-Loss: mean binary cross-entropy of sigmoid(logit) against soft labels y.
+z = W2 . relu(h) + b2; the flipped-occupation output uses -f.  A sample with
+logit scale t (beta * 0.1 eV) has logit t * z.
+Loss: mean binary cross-entropy of sigmoid(t z) against soft labels y.
 Parameter vector: [W1 (64 x 48 row-major) | b1 (64) | W2 (64) | b2].
 """
 import numpy as np
@@ -44,19 +45,21 @@ def logits(p, X, n_env, bf16_w1=False):
     return np.stack([lc, ln], axis=1)
 
 
-def loss_and_grad(p, X, y, n_env):
+def loss_and_grad(p, X, y, n_env, t=None):
     W1, b1, W2, b2 = unpack(p)
     W1 = W1.copy()
     W1[:, n_env:] = 0.0
     X = np.asarray(X, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
+    t = np.ones(len(y)) if t is None else np.asarray(t, dtype=np.float64)
     h = X @ W1.T + b1
     r = np.maximum(h, 0)
-    z = r @ W2 + b2
+    z = (r @ W2 + b2) * t
     pz = 1.0 / (1.0 + np.exp(-z))
-    loss = np.mean(-(y * np.log(pz) + (1 - y) * np.log(1 - pz)))
+    # log sigmoid(z) = -log(1 + e^-z), log(1 - sigmoid(z)) = -log(1 + e^z)
+    loss = np.mean(y * np.logaddexp(0.0, -z) + (1 - y) * np.logaddexp(0.0, z))
     n = len(y)
-    dz = (pz - y) / n
+    dz = t * (pz - y) / n
     gW2 = r.T @ dz
     gb2 = dz.sum()
     dh = np.outer(dz, W2) * (h > 0)
@@ -67,12 +70,12 @@ def loss_and_grad(p, X, y, n_env):
     return loss, g
 
 
-def finite_difference(p, X, y, n_env, idx, eps=1e-6):
+def finite_difference(p, X, y, n_env, idx, eps=1e-6, t=None):
     g = []
     for i in idx:
         pp = np.array(p, dtype=np.float64)
         pm = pp.copy()
         pp[i] += eps
         pm[i] -= eps
-        g.append((loss_and_grad(pp, X, y, n_env)[0] - loss_and_grad(pm, X, y, n_env)[0]) / (2 * eps))
+        g.append((loss_and_grad(pp, X, y, n_env, t)[0] - loss_and_grad(pm, X, y, n_env, t)[0]) / (2 * eps))
     return np.array(g)

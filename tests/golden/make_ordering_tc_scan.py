@@ -1,11 +1,11 @@
 """Heat-capacity scan of the ordering FCC CE (nearest-neighbour pair
-+0.04 eV, x_B = 0.25) with the CPU oracle, to place the learned-proposal
++0.04 eV, second-neighbour pair -0.004 eV, x_B = 0.25) with the CPU oracle, to place the learned-proposal
 workload's temperatures around the L1_2 order-disorder transition.
 
 Canonical Metropolis (the oracle's restatement of the reference loop) on a
 16x16x16 supercell, started from the perfect L1_2 arrangement (B on the
 simple-cubic corner sublattice of the conventional cell), per temperature
-2,000 equilibration + 6,000 sampled passes.  C/k_B per cell =
+1,500 equilibration + 4,000 sampled passes.  C/k_B per cell =
 var(E) / (k_B T)^2 / n_cells.  Writes fcc_ordering_tc_scan.json beside this
 script; T_c is the temperature of the largest C.
 
@@ -35,8 +35,8 @@ def main():
     sc = Supercell(ce.prim, (L, L, L))
     osys = O.OracleSystem(flat, sc.dims)
     occ0 = l12_occupation(sc)
-    temps = np.arange(40.0, 401.0, 10.0)
-    n_equil, n_pass = 2000, 6000
+    temps = np.arange(60.0, 301.0, 10.0)
+    n_equil, n_pass = 1500, 4000
 
     def one(it):
         i, T = it

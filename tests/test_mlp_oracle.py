@@ -58,10 +58,11 @@ def test_gradient_finite_differences():
     X = np.where(rng.random((50, 48)) < 0.5, -1.0, 1.0)
     X[:, 42:] = 0
     y = rng.random(50)
-    _, g = M.loss_and_grad(p, X, y, 42)
     idx = [0, 5, 41, 100, 47 * 64 + 3, 3072, 3100, 3136, 3150, 3200]
-    fd = M.finite_difference(p, X, y, 42, idx)
-    assert np.abs(fd - g[idx]).max() <= 1e-8
+    for t in (None, rng.uniform(0.5, 5.0, 50)):
+        _, g = M.loss_and_grad(p, X, y, 42, t)
+        fd = M.finite_difference(p, X, y, 42, idx, t=t)
+        assert np.abs(fd - g[idx]).max() <= 1e-8
 
 
 def test_bf16_rounding_known_values():
@@ -83,11 +84,12 @@ d = json.loads(sys.stdin.read())
 p = torch.tensor(d["p"], dtype=torch.float64, requires_grad=True)
 X = torch.tensor(d["X"], dtype=torch.float64)
 y = torch.tensor(d["y"], dtype=torch.float64)
+t = torch.tensor(d["t"], dtype=torch.float64)
 n_env = d["n_env"]
 mask = torch.zeros(64, 48, dtype=torch.float64); mask[:, :n_env] = 1
 W1 = p[:3072].reshape(64, 48) * mask
 b1 = p[3072:3136]; W2 = p[3136:3200]; b2 = p[3200]
-z = torch.relu(X @ W1.T + b1) @ W2 + b2
+z = (torch.relu(X @ W1.T + b1) @ W2 + b2) * t
 loss = torch.nn.functional.binary_cross_entropy_with_logits(z, y)
 loss.backward()
 xb = torch.tensor(d["xb"], dtype=torch.float32).to(torch.bfloat16).to(torch.float32)
@@ -101,14 +103,16 @@ def test_loss_gradient_and_bf16_vs_torch():
     X = np.where(rng.random((64, 48)) < 0.5, -1.0, 1.0)
     X[:, 42:] = 0
     y = rng.random(64)
+    t = rng.uniform(0.5, 5.0, 64)
     xb = rng.normal(0, 1, 1000).astype(np.float32)
-    inp = json.dumps({"p": p.tolist(), "X": X.tolist(), "y": y.tolist(), "n_env": 42, "xb": xb.tolist()})
+    inp = json.dumps({"p": p.tolist(), "X": X.tolist(), "y": y.tolist(), "t": t.tolist(), "n_env": 42,
+                      "xb": xb.tolist()})
     env = dict(os.environ, HIP_VISIBLE_DEVICES="")
     r = subprocess.run([sys.executable, "-c", _TORCH_SCRIPT], input=inp, capture_output=True, text=True,
                        timeout=600, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
-    loss, g = M.loss_and_grad(p, X, y, 42)
+    loss, g = M.loss_and_grad(p, X, y, 42, t)
     assert abs(loss - out["loss"]) <= 1e-12
     assert np.abs(g - np.array(out["grad"])).max() <= 1e-12
     assert np.array_equal(M.bf16_round(xb), np.array(out["xb"], dtype=np.float32))
