@@ -64,6 +64,9 @@ struct clexmc_handle {
   int *d_sample_count = nullptr;
   int sample_cap = 0;
   int block_size = 0;
+  int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
+  bool last_packed = false;
+  int global_swaps = 1;             // canonical serial random-pair swaps per pass
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   // learned proposal
@@ -301,9 +304,11 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
 // Independent-set partition computed by the library (clexmc_add_supercell with
 // n_classes == 0): periodic colouring by residues of the cell coordinates when
 // one exists (equal class sizes), otherwise a greedy colouring; plus
-// canonical cross-class pair phases (pairs (s, s+v) for random translations
-// v, greedily kept so that no cluster contains sites of two pairs and the two
-// sites of a pair share no cluster).  Selection never looks at occupations.
+// canonical cross-class pair phases (pairs of s on sublattice b and t on
+// sublattice b2, with the same number of occupants, in the cell translated by
+// a random v; greedily kept so that no cluster contains sites of two pairs
+// and the two sites of a pair share no cluster).  Selection never looks at
+// occupations.
 // ---------------------------------------------------------------------------
 static uint64_t splitmix64(uint64_t &x) {
   uint64_t z = (x += 0x9E3779B97F4A7C15ull);
@@ -431,11 +436,13 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
   while ((int)pb.size() - 1 < n_pair_req && tries < 64 * std::max(1, n_pair_req)) {
     ++tries;
     int b = var[splitmix64(st) % var.size()];
+    int b2 = var[splitmix64(st) % var.size()];
     int v[3] = {(int)(splitmix64(st) % L[0]), (int)(splitmix64(st) % L[1]), (int)(splitmix64(st) % L[2])};
-    if (v[0] == 0 && v[1] == 0 && v[2] == 0) continue;
+    if (h->n_occ[b2] != h->n_occ[b]) continue;
+    if (b2 == b && v[0] == 0 && v[1] == 0 && v[2] == 0) continue;
     bool bad = false;
     for (auto const &o : offs[b])
-      if (o[0] == b && ((o[1] - v[0]) % L[0]) == 0 && ((o[2] - v[1]) % L[1]) == 0 && ((o[3] - v[2]) % L[2]) == 0) {
+      if (o[0] == b2 && ((o[1] - v[0]) % L[0]) == 0 && ((o[2] - v[1]) % L[1]) == 0 && ((o[3] - v[2]) % L[2]) == 0) {
         bad = true;
         break;
       }
@@ -446,15 +453,16 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
     size_t before = ps.size();
     for (long c : order) {
       int i = (int)(c % L[0]), j = (int)((c / L[0]) % L[1]), k = (int)(c / ((long)L[0] * L[1]));
-      long s = lin(b, i, j, k), t = lin(b, i + v[0], j + v[1], k + v[2]);
+      long s = lin(b, i, j, k), t = lin(b2, i + v[0], j + v[1], k + v[2]);
       if (blocked[s] || blocked[t]) continue;
       ps.push_back((int)s);
       ps.push_back((int)t);
       blocked[s] = blocked[t] = 1;
       for (long x : {s, t}) {
+        const int bx = (int)(x / nc);
         long cx = x % nc;
         int ii = (int)(cx % L[0]), jj = (int)((cx / L[0]) % L[1]), kk = (int)(cx / ((long)L[0] * L[1]));
-        for (auto const &o : offs[b]) blocked[lin(o[0], ii + o[1], jj + o[2], kk + o[3])] = 1;
+        for (auto const &o : offs[bx]) blocked[lin(o[0], ii + o[1], jj + o[2], kk + o[3])] = 1;
       }
     }
     if (ps.size() > before) pb.push_back((int)(ps.size() / 2));
@@ -670,9 +678,9 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       int b, i, j, k, b2, i2, j2, k2;
       decode(ls, b, i, j, k);
       decode(lt, b2, i2, j2, k2);
-      ARG_CHECK(b == b2, "pair sites must share a sublattice");
+      ARG_CHECK(h->n_occ[b] == h->n_occ[b2] && h->n_occ[b] >= 2, "pair sites need the same occupant count");
       pair_addr.push_back(make_int2(pad_addr(b, i, j, k), pad_addr(b2, i2, j2, k2)));
-      pair_sub.push_back(b);
+      pair_sub.push_back(b | (b2 << 8));
       images(ls, img);
       for (int a : img) pair_halo.push_back(make_int2(pad_addr(b, i, j, k), a));
       images(lt, img);
@@ -919,6 +927,20 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads) {
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(per_pass >= 0 && per_pass <= 65536, "global swaps per pass out of range");
+  h->global_swaps = per_pass;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(mode >= 0 && mode <= 2, "packing must be 0 (automatic), 1 (int8) or 2 (2-bit)");
+  h->packing = mode;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
@@ -935,13 +957,21 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   int threads = h->block_size;
   if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
   const int n_tab = h->binary ? 0 : h->n_flower_tables;
-  size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, threads, max_class);
   int max_lds = 0;
   HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-  if ((long long)lds > (long long)max_lds - 64) {
+  // int8 image when it fits (or is forced), else 2 bits per site (<= 4 occupants)
+  bool packed = h->packing == 2;
+  size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
+  if (h->packing == 0 && (long long)lds > (long long)max_lds - 64 && h->n_occ_max <= 4) {
+    packed = true;
+    lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
+  }
+  ARG_CHECK(!packed || h->n_occ_max <= 4, "2-bit packed images need at most 4 occupants per site");
+  if (h->proposal == 0 && (long long)lds > (long long)max_lds - 64) {
     h->err = "chain image does not fit in LDS (" + std::to_string(lds) + " > " + std::to_string(max_lds) + " bytes)";
     return CLEXMC_ERR_UNSUPPORTED;
   }
+  h->last_packed = packed;
   // sample capacity (append)
   std::vector<int> cnt(h->n_chains);
   HIP_TRY(hipMemcpy(cnt.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
@@ -982,6 +1012,7 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   a.rounds = std::max(1, canonical_rounds);
   a.n_orbits = h->n_orbits;
   a.n_tab = n_tab;
+  a.global_swaps = h->global_swaps;
   if (h->proposal != 0) {
     if (!h->binary) {
       h->err = "the learned proposal is implemented for binary (+-1) cluster expansions";
@@ -1000,13 +1031,40 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
     HIP_TRY(launch_learned(a, la, kind, weights, h->n_chains, threads, lds, h->stream));
   } else {
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    HIP_TRY(launch_metropolis(a, h->binary != 0, kind, h->n_chains, threads, lds, h->stream));
+    HIP_TRY(launch_metropolis(a, h->binary != 0, kind, packed, h->n_chains, threads, lds, h->stream));
   }
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_reanchor_energies(clexmc_handle *h, double *max_drift) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  std::vector<double> corr((size_t)h->n_chains * h->n_corr);
+  clexmc_status st = run_corr(h, 0, h->n_chains, corr.data());
+  if (st != CLEXMC_OK) return st;
+  std::vector<ChainState> state(h->n_chains);
+  HIP_TRY(hipMemcpy(state.data(), h->d_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToHost));
+  double drift = 0.0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    const double *cc = corr.data() + (size_t)c * h->n_corr;
+    double E = 0.0;
+    for (int i = 0; i < h->n_corr; ++i) E += h->eci[i] * cc[i];
+    if (h->binary) {
+      E = 0.0;
+      for (int a = 0; a < h->n_orbits; ++a)
+        E += h->bin_w[a] * (double)llround(cc[h->orbit_func_begin[a]] * h->orbit_mult[a]);
+    }
+    drift = std::max(drift, std::fabs(state[c].E - E));
+    state[c].E = E;
+  }
+  HIP_TRY(hipMemcpy(h->d_state, state.data(), sizeof(ChainState) * h->n_chains, hipMemcpyHostToDevice));
+  if (max_drift) *max_drift = drift;
   return CLEXMC_OK;
 }
 

@@ -104,7 +104,7 @@ struct DevSupercell {
   int n_pair_phases;
   const int *pair_begin;   // [n_pair_phases+1]
   const int2 *pair_addr;   // padded addresses of both sites
-  const int *pair_sublat;  // [n_pairs] sublattice of both (equal)
+  const int *pair_sublat;  // [n_pairs] sublattices of the two sites: b | (b2 << 8)
   const int *pair_halo_begin; // [n_pair_phases+1]
   const int2 *pair_halo;
   int max_class;           // largest class size

@@ -23,6 +23,7 @@ struct RunArgs {
   int rounds;
   int n_orbits;
   int n_tab;           // general path: flower table floats staged in LDS
+  int global_swaps;    // canonical: serial swaps of uniformly random site pairs per pass
 };
 
 struct DeltaArgs {
@@ -100,11 +101,11 @@ hipError_t launch_mlp_train_step(const TrainArgs &a, hipStream_t st);
 hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);
 hipError_t launch_mlp_publish(const float *p, MlpDev *m, int n_env, float eps, hipStream_t st);
 
-hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, int n_chains, int threads,
+hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
                              size_t lds, hipStream_t st);
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
-size_t metropolis_lds_bytes(int pad_bytes, int n_tab, bool binary, int threads, int max_class);
+size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed);
 
 }  // namespace clexmc

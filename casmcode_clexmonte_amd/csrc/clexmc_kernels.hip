@@ -1,7 +1,9 @@
 // gfx950 kernels of the clexmc Metropolis hot path.
 //
-//  metropolis_kernel   one workgroup per chain; the chain's int8 occupation
-//                      lives in LDS as a halo-padded image; sites are updated
+//  metropolis_kernel   one workgroup per chain; the chain's occupation lives
+//                      in LDS as a halo-padded image (int8 per site, or 2 bits
+//                      per site packed 16 to a 32-bit word when the int8 image
+//                      does not fit: up to 4 occupants); sites are updated
 //                      class by class (independent sets: no cluster contains
 //                      two sites of one class), so all proposals of a class
 //                      are evaluated and applied concurrently and exactly.
@@ -17,6 +19,8 @@
 //  philox_kernel       device Philox for known-answer checks
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "clexmc_device.h"
 #include "clexmc_kernels.h"
@@ -66,6 +70,42 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS occupation image accessors.  A site is written only by the lane that
+// owns it in the current phase (sites of a class share no cluster), but with
+// packing 16 sites share a word, so packed writes are atomic XORs of the
+// changed bits (LDS atomics); the other sites' bits are untouched.
+// ---------------------------------------------------------------------------
+struct ImgU8 {
+  uint8_t *p;
+  __device__ __forceinline__ int get(int a) const { return p[a]; }
+  __device__ __forceinline__ void set(int a, int old, int nw) const {
+    (void)old;
+    p[a] = (uint8_t)nw;
+  }
+  __device__ __forceinline__ void copy(int dst, int src) const { p[dst] = p[src]; }
+};
+
+struct ImgP2 {
+  uint32_t *w;
+  __device__ __forceinline__ int get(int a) const {
+    return (int)__builtin_amdgcn_ubfe(w[a >> 4], (uint32_t)((a & 15) << 1), 2u);
+  }
+  __device__ __forceinline__ void set(int a, int old, int nw) const {
+    atomicXor(&w[a >> 4], (uint32_t)(old ^ nw) << ((a & 15) << 1));
+  }
+  __device__ __forceinline__ void copy(int dst, int src) const {
+    const int v = get(src), o = get(dst);
+    if (v != o) set(dst, o, v);
+  }
+};
+
+// bytes of the LDS image for pad_sites padded sites, a multiple of 16 so that
+// the arrays placed after it keep their natural (up to 16-byte) alignment
+__host__ __device__ inline int image_bytes(int pad_sites, bool packed) {
+  return packed ? ((pad_sites + 63) / 64) * 16 : ((pad_sites + 15) & ~15);
+}
+
+// ---------------------------------------------------------------------------
 // Binary path: K_a(s) = sum over clusters of orbit a containing s of
 // prod phi(other sites) = (-1)^(k-1) * (n_a - 2 * sum XOR(other bits)).
 // The site program lists the padded offsets of the other sites per orbit,
@@ -73,8 +113,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // an even number of copies of the site XORs to 0, an odd number contributes
 // the site's own bit, removed below.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float binary_field(const uint8_t *img, int pa, int o, cptr<BinSub> bs,
-                                              cptr<int> off, int n_slots, int (&K)[kMaxSlots]) {
+template <class IMG>
+__device__ __forceinline__ float binary_field(const IMG &img, int pa, int o, cptr<BinSub> bs, cptr<int> off,
+                                              int n_slots, int (&K)[kMaxSlots]) {
   float F = 0.0f;
 #pragma unroll
   for (int j = 0; j < kMaxSlots; ++j) {
@@ -90,30 +131,30 @@ __device__ __forceinline__ float binary_field(const uint8_t *img, int pa, int o,
 #pragma unroll 2
           for (int e = 0; e < len; e += 4) {
             const int q = q0 + e;
-            const int b0 = img[pa + off[q]];
-            const int b1 = img[pa + off[q + 1]];
-            const int b2 = img[pa + off[q + 2]];
-            const int b3 = img[pa + off[q + 3]];
+            const int b0 = img.get(pa + off[q]);
+            const int b1 = img.get(pa + off[q + 1]);
+            const int b2 = img.get(pa + off[q + 2]);
+            const int b3 = img.get(pa + off[q + 3]);
             X += (b0 + b1) + (b2 + b3);
           }
         } else if (ko == 2) {
 #pragma unroll 2
           for (int e = 0; e < len; e += 4) {
             const int q = q0 + 2 * e;
-            const int b0 = img[pa + off[q]];
-            const int b1 = img[pa + off[q + 1]];
-            const int b2 = img[pa + off[q + 2]];
-            const int b3 = img[pa + off[q + 3]];
-            const int b4 = img[pa + off[q + 4]];
-            const int b5 = img[pa + off[q + 5]];
-            const int b6 = img[pa + off[q + 6]];
-            const int b7 = img[pa + off[q + 7]];
+            const int b0 = img.get(pa + off[q]);
+            const int b1 = img.get(pa + off[q + 1]);
+            const int b2 = img.get(pa + off[q + 2]);
+            const int b3 = img.get(pa + off[q + 3]);
+            const int b4 = img.get(pa + off[q + 4]);
+            const int b5 = img.get(pa + off[q + 5]);
+            const int b6 = img.get(pa + off[q + 6]);
+            const int b7 = img.get(pa + off[q + 7]);
             X += ((b0 ^ b1) + (b2 ^ b3)) + ((b4 ^ b5) + (b6 ^ b7));
           }
         } else if (ko >= 3) {
           for (int e = 0; e < len; ++e) {
             int x = 0;
-            for (int q = q0 + e * ko; q < q0 + (e + 1) * ko; ++q) x ^= img[pa + off[q]];
+            for (int q = q0 + e * ko; q < q0 + (e + 1) * ko; ++q) x ^= img.get(pa + off[q]);
             X += x;
           }
         }
@@ -128,7 +169,8 @@ __device__ __forceinline__ float binary_field(const uint8_t *img, int pa, int o,
 }
 
 // General path: G_f(s) = sum_e V_e[tau_others][f] from the flower tables.
-__device__ __forceinline__ void general_field(const uint8_t *img, int pa, cptr<GenSub> gs, cptr<int> prog,
+template <class IMG>
+__device__ __forceinline__ void general_field(const IMG &img, int pa, cptr<GenSub> gs, cptr<int> prog,
                                               const float *tab, float (&G)[kMaxFunc]) {
   G[0] = G[1] = G[2] = 0.0f;
   const int nf = gs->nf;
@@ -140,7 +182,7 @@ __device__ __forceinline__ void general_field(const uint8_t *img, int pa, cptr<G
     q += 2;
     int idx = 0;
     for (int i = 0; i < k; ++i) {
-      idx += img[pa + prog[q]] * prog[q + 1];
+      idx += img.get(pa + prog[q]) * prog[q + 1];
       q += 2;
     }
     const float *V = tab + toff + idx * nf;
@@ -165,7 +207,7 @@ __device__ __forceinline__ bool metropolis_accept(float dE, float beta, float u)
 // ---------------------------------------------------------------------------
 // Persistent Metropolis kernel
 // ---------------------------------------------------------------------------
-template <bool BIN, int KIND>
+template <bool BIN, int KIND, bool PACK>
 __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int chain = blockIdx.x;
@@ -186,8 +228,11 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
   const int n_slots = sys->n_slots;
 
   // ---- LDS layout ----
-  uint8_t *img = (uint8_t *)smem;
-  size_t off_b = (size_t)((pad_bytes + 15) & ~15);
+  using Img = typename std::conditional<PACK, ImgP2, ImgU8>::type;
+  Img img;
+  if constexpr (PACK) img.w = (uint32_t *)smem;
+  else img.p = (uint8_t *)smem;
+  size_t off_b = (size_t)image_bytes(pad_bytes, PACK);
   float *tabL = (float *)(smem + off_b);
   const int n_tab = BIN ? 0 : args.n_tab;
   off_b += (size_t)((n_tab * 4 + 15) & ~15);
@@ -198,12 +243,13 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
   off_b += 16 * sizeof(double);
   unsigned long long *maxDraw = (unsigned long long *)(smem + off_b);
   off_b += 16;
-  float *cF = (float *)(smem + off_b);                   // [kMaxFunc][max_class]
-  int8_t *cK = (int8_t *)(cF + kMaxFunc * max_class);    // [kMaxSlots][max_class]
+  const int nfs = BIN ? 1 : sys->n_func_max;              // field values kept per class site
+  float *cF = (float *)(smem + off_b);                   // [nfs][max_class]
+  int8_t *cK = (int8_t *)(cF + nfs * max_class);         // [kMaxSlots][max_class] (binary)
 
   // ---- load the padded image (and general tables) ----
   const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
-  for (int p = tid; p < pad_bytes; p += nthr) {
+  auto padded_occ = [&](int p) -> int {
     int b = p / S;
     int r = p - b * S;
     int pk = r / P12;
@@ -213,7 +259,21 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
     int i = ((pi - h) % L1 + L1) % L1;
     int j = ((pj - h) % L2 + L2) % L2;
     int k = ((pk - h) % L3 + L3) % L3;
-    img[p] = (uint8_t)occG[b * n_cells + i + L1 * (j + L2 * k)];
+    return occG[b * n_cells + i + L1 * (j + L2 * k)];
+  };
+  if constexpr (PACK) {
+    // each thread assembles whole words (16 padded sites)
+    const int n_words = (pad_bytes + 15) / 16;
+    for (int wd = tid; wd < n_words; wd += nthr) {
+      uint32_t word = 0u;
+      for (int q = 0; q < 16; ++q) {
+        const int p = wd * 16 + q;
+        if (p < pad_bytes) word |= (uint32_t)(padded_occ(p) & 3) << (2 * q);
+      }
+      img.w[wd] = word;
+    }
+  } else {
+    for (int p = tid; p < pad_bytes; p += nthr) img.p[p] = (uint8_t)padded_occ(p);
   }
   if (!BIN) {
     const float *__restrict__ tg = sys->flower_tables;
@@ -256,7 +316,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
         // ---- semi-grand canonical: every site of the class proposes a change
         for (int t = cb + tid; t < ce; t += nthr) {
           const int pa = sc->class_addr[t];
-          const int o = img[pa];
+          const int o = img.get(pa);
           const float u = rng.next();
           if (BIN) {
             int K[kMaxSlots];
@@ -271,7 +331,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
             accN[1] += sk / 2;
             accN[0] -= sk / 2;
             accAcc += acc ? 1 : 0;
-            if (acc) img[pa] = (uint8_t)(1 - o);
+            if (acc) img.set(pa, o, 1 - o);
           } else {
             float G[kMaxFunc];
             general_field(img, pa, gs, gen_prog, tabL, G);
@@ -285,7 +345,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
             const bool acc = metropolis_accept(dE, beta, u);
             accAtt += 1;
             if (acc) {
-              img[pa] = (uint8_t)nw;
+              img.set(pa, o, nw);
               accAcc += 1;
               accE += (double)dEf;
 #pragma unroll
@@ -301,15 +361,15 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
           const int pa = sc->class_addr[cb + t];
           if (BIN) {
             int K[kMaxSlots];
-            cF[t] = binary_field(img, pa, img[pa], bs, bin_off, n_slots, K);
+            cF[t] = binary_field(img, pa, img.get(pa), bs, bin_off, n_slots, K);
 #pragma unroll
             for (int a = 0; a < kMaxSlots; ++a) cK[a * max_class + t] = (int8_t)K[a];
           } else {
             float G[kMaxFunc];
             general_field(img, pa, gs, gen_prog, tabL, G);
             cF[t] = G[0];
-            cF[max_class + t] = G[1];
-            cF[2 * max_class + t] = G[2];
+            if (nfs > 1) cF[max_class + t] = G[1];
+            if (nfs > 2) cF[2 * max_class + t] = G[2];
           }
         }
         __syncthreads();
@@ -321,7 +381,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
             if (q < 0) q += n;
             if (p >= q) continue;  // q == p: unpaired; p > q: the partner leads
             const int pa = sc->class_addr[cb + p], qa = sc->class_addr[cb + q];
-            const int op = img[pa], oq = img[qa];
+            const int op = img.get(pa), oq = img.get(qa);
             if (op == oq) continue;
             const float u = rng.next();
             float dE;
@@ -329,15 +389,15 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
               const float dp = op ? -2.0f : 2.0f;
               dE = dp * cF[p] - dp * cF[q];
             } else {
-              float Gp[kMaxFunc] = {cF[p], cF[max_class + p], cF[2 * max_class + p]};
-              float Gq[kMaxFunc] = {cF[q], cF[max_class + q], cF[2 * max_class + q]};
+              float Gp[kMaxFunc] = {cF[p], nfs > 1 ? cF[max_class + p] : 0.f, nfs > 2 ? cF[2 * max_class + p] : 0.f};
+              float Gq[kMaxFunc] = {cF[q], nfs > 1 ? cF[max_class + q] : 0.f, nfs > 2 ? cF[2 * max_class + q] : 0.f};
               dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
             }
             const bool acc = metropolis_accept(dE, beta, u);
             accAtt += 1;
             if (acc) {
-              img[pa] = (uint8_t)oq;
-              img[qa] = (uint8_t)op;
+              img.set(pa, op, oq);
+              img.set(qa, oq, op);
               accAcc += 1;
               if (BIN) {
                 const int sp = op ? -2 : 2;
@@ -355,7 +415,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
       // ---- halo copies of this class
       for (int t = sc->halo_begin[c] + tid; t < sc->halo_begin[c + 1]; t += nthr) {
         const int2 hc = sc->halo[t];
-        img[hc.y] = img[hc.x];
+        img.copy(hc.y, hc.x);
       }
       __syncthreads();
     }
@@ -367,30 +427,29 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
         const int2 pq = sc->pair_addr[t];
-        const int b = sc->pair_sublat[t];
-        const int op = img[pq.x], oq = img[pq.y];
+        const int bpq = sc->pair_sublat[t];
+        const int bp = bpq & 0xFF, bq = bpq >> 8;
+        const int op = img.get(pq.x), oq = img.get(pq.y);
         if (op == oq) continue;
         const float u = rng.next();
         float dE;
         int Kp[kMaxSlots], Kq[kMaxSlots];
         if (BIN) {
-          cptr<BinSub> bs = cst(sc->bin_sub + b);
-          const float Fp = binary_field(img, pq.x, op, bs, bin_off, n_slots, Kp);
-          const float Fq = binary_field(img, pq.y, oq, bs, bin_off, n_slots, Kq);
+          const float Fp = binary_field(img, pq.x, op, cst(sc->bin_sub + bp), bin_off, n_slots, Kp);
+          const float Fq = binary_field(img, pq.y, oq, cst(sc->bin_sub + bq), bin_off, n_slots, Kq);
           const float dp = op ? -2.0f : 2.0f;
           dE = dp * Fp - dp * Fq;
         } else {
-          cptr<GenSub> gs = cst(sc->gen_sub + b);
           float Gp[kMaxFunc], Gq[kMaxFunc];
-          general_field(img, pq.x, gs, gen_prog, tabL, Gp);
-          general_field(img, pq.y, gs, gen_prog, tabL, Gq);
-          dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
+          general_field(img, pq.x, cst(sc->gen_sub + bp), gen_prog, tabL, Gp);
+          general_field(img, pq.y, cst(sc->gen_sub + bq), gen_prog, tabL, Gq);
+          dE = general_dE(sys, bp, op, oq, Gp) + general_dE(sys, bq, oq, op, Gq);
         }
         const bool acc = metropolis_accept(dE, beta, u);
         accAtt += 1;
         if (acc) {
-          img[pq.x] = (uint8_t)oq;
-          img[pq.y] = (uint8_t)op;
+          img.set(pq.x, op, oq);
+          img.set(pq.y, oq, op);
           accAcc += 1;
           if (BIN) {
             const int sp = op ? -2 : 2;
@@ -404,7 +463,83 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
       __syncthreads();
       for (int t = sc->pair_halo_begin[ph] + tid; t < sc->pair_halo_begin[ph + 1]; t += nthr) {
         const int2 hc = sc->pair_halo[t];
-        img[hc.y] = img[hc.x];
+        img.copy(hc.y, hc.x);
+      }
+      __syncthreads();
+    }
+
+    // ---- canonical global swaps: uniformly random site pairs anywhere in the
+    // supercell (the two sites may share clusters), evaluated serially by one
+    // lane as two sequential flips on the image (every periodic copy of the
+    // first site is updated before the second site's field is taken).  These
+    // are the reference's unrestricted pair moves; they make every supercell
+    // ergodic, however small, for the canonical ensemble.
+    if (KIND == 0 && args.global_swaps > 0) {
+      if (tid == 0) {
+        const int n_var = sc->class_begin[n_classes];
+        auto set_all = [&](int a, int old, int nw) {
+          const int b = a / S;
+          const int r = a - b * S;
+          const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
+          const int P2 = P12 / P1, P3 = S / P12;
+          const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
+          for (int dk = -m3; dk <= m3; ++dk) {
+            const int qk = pk + dk * L3;
+            if (qk < 0 || qk >= P3) continue;
+            for (int dj = -m2; dj <= m2; ++dj) {
+              const int qj = pj + dj * L2;
+              if (qj < 0 || qj >= P2) continue;
+              for (int di = -m1; di <= m1; ++di) {
+                const int qi = pi + di * L1;
+                if (qi < 0 || qi >= P1) continue;
+                img.set(b * S + qk * P12 + qj * P1 + qi, old, nw);
+              }
+            }
+          }
+        };
+        for (int g = 0; g < args.global_swaps; ++g) {
+          const float u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
+          const int is = min((int)(u1 * (float)n_var), n_var - 1);
+          const int it = min((int)(u2 * (float)n_var), n_var - 1);
+          const int sa = sc->class_addr[is], ta = sc->class_addr[it];
+          if (sa == ta) continue;
+          const int bsa = sa / S, bta = ta / S;
+          if (sys->n_occ[bsa] != sys->n_occ[bta]) continue;
+          const int os = img.get(sa), ot = img.get(ta);
+          if (os == ot) continue;
+          float dE1, dE2;
+          int Ks[kMaxSlots], Kt[kMaxSlots];
+          if (BIN) {
+            dE1 = (os ? -2.0f : 2.0f) * binary_field(img, sa, os, cst(sc->bin_sub + bsa), bin_off, n_slots, Ks);
+          } else {
+            float G[kMaxFunc];
+            general_field(img, sa, cst(sc->gen_sub + bsa), gen_prog, tabL, G);
+            dE1 = general_dE(sys, bsa, os, ot, G);
+          }
+          set_all(sa, os, ot);
+          if (BIN) {
+            dE2 = (ot ? -2.0f : 2.0f) * binary_field(img, ta, ot, cst(sc->bin_sub + bta), bin_off, n_slots, Kt);
+          } else {
+            float G[kMaxFunc];
+            general_field(img, ta, cst(sc->gen_sub + bta), gen_prog, tabL, G);
+            dE2 = general_dE(sys, bta, ot, os, G);
+          }
+          const float dE = dE1 + dE2;
+          accAtt += 1;
+          if (metropolis_accept(dE, beta, u3)) {
+            set_all(ta, ot, os);
+            accAcc += 1;
+            if (BIN) {
+              const int ss = os ? -2 : 2, st = ot ? -2 : 2;
+#pragma unroll
+              for (int a = 0; a < kMaxSlots; ++a) accC[a] += ss * Ks[a] + st * Kt[a];
+            } else {
+              accE += (double)dE;
+            }
+          } else {
+            set_all(sa, ot, os);
+          }
+        }
       }
       __syncthreads();
     }
@@ -499,7 +634,7 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
       int i = cc % L1;
       int j = (cc / L1) % L2;
       int k = cc / (L1 * L2);
-      occW[l] = (int8_t)img[b * S + (k + h) * P12 + (j + h) * P1 + (i + h)];
+      occW[l] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
     }
   }
   // all lanes advance by the same bound so the next launch starts fresh
@@ -618,23 +753,29 @@ __global__ void philox_kernel(int n, const uint4 *ctr, const uint2 *key, uint4 *
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-template <bool BIN, int KIND>
+template <bool BIN, int KIND, bool PACK>
 static hipError_t launch_metro(const RunArgs &args, int n_chains, int threads, size_t lds, hipStream_t st) {
-  auto kern = metropolis_kernel<BIN, KIND>;
+  auto kern = metropolis_kernel<BIN, KIND, PACK>;
   hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args);
   return hipGetLastError();
 }
 
-hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, int n_chains, int threads,
+hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
                              size_t lds, hipStream_t st) {
-  if (binary) {
-    return kind == 0 ? launch_metro<true, 0>(args, n_chains, threads, lds, st)
-                     : launch_metro<true, 1>(args, n_chains, threads, lds, st);
+  if (packed) {
+    if (binary)
+      return kind == 0 ? launch_metro<true, 0, true>(args, n_chains, threads, lds, st)
+                       : launch_metro<true, 1, true>(args, n_chains, threads, lds, st);
+    return kind == 0 ? launch_metro<false, 0, true>(args, n_chains, threads, lds, st)
+                     : launch_metro<false, 1, true>(args, n_chains, threads, lds, st);
   }
-  return kind == 0 ? launch_metro<false, 0>(args, n_chains, threads, lds, st)
-                   : launch_metro<false, 1>(args, n_chains, threads, lds, st);
+  if (binary)
+    return kind == 0 ? launch_metro<true, 0, false>(args, n_chains, threads, lds, st)
+                     : launch_metro<true, 1, false>(args, n_chains, threads, lds, st);
+  return kind == 0 ? launch_metro<false, 0, false>(args, n_chains, threads, lds, st)
+                   : launch_metro<false, 1, false>(args, n_chains, threads, lds, st);
 }
 
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st) {
@@ -654,13 +795,13 @@ hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, 
   return hipGetLastError();
 }
 
-size_t metropolis_lds_bytes(int pad_bytes, int n_tab, bool binary, int threads, int max_class) {
-  (void)threads;
-  size_t img = (size_t)((pad_bytes + 15) & ~15);
+size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed) {
+  size_t img = (size_t)image_bytes(pad_sites, packed);
   size_t tab = binary ? 0 : (size_t)((n_tab * 4 + 15) & ~15);
   const int n_islot = (binary ? kMaxSlots : 0) + kMaxOcc + 2;
   size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16;
-  size_t scratch = (size_t)kMaxFunc * max_class * sizeof(float) + (size_t)kMaxSlots * max_class;
+  const int nfs = binary ? 1 : n_func_max;
+  size_t scratch = (size_t)nfs * max_class * sizeof(float) + (binary ? (size_t)kMaxSlots * max_class : 0);
   return img + tab + red + scratch + 64;
 }
 

@@ -437,15 +437,14 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
         const int2 pq = sc->pair_addr[t];
-        const int b = sc->pair_sublat[t];
+        const int bpq = sc->pair_sublat[t];
         const int op = img[pq.x], oq = img[pq.y];
         if (op == oq) continue;
         const uint4 ur = philox4x32_10(make_uint4((uint32_t)tid, (uint32_t)draw, (uint32_t)(draw >> 32) | 0x80000000u,
                                                  (uint32_t)chain), k0, k1);
         int Kp[kMaxSlots], Kq[kMaxSlots];
-        cptr<BinSub> bs = cst(sc->bin_sub + b);
-        const float Fp = binary_field_l(img, pq.x, op, bs, bin_off, n_slots, Kp);
-        const float Fq = binary_field_l(img, pq.y, oq, bs, bin_off, n_slots, Kq);
+        const float Fp = binary_field_l(img, pq.x, op, cst(sc->bin_sub + (bpq & 0xFF)), bin_off, n_slots, Kp);
+        const float Fq = binary_field_l(img, pq.y, oq, cst(sc->bin_sub + (bpq >> 8)), bin_off, n_slots, Kq);
         const float dp = op ? -2.0f : 2.0f;
         const float dE = dp * Fp - dp * Fq;
         const bool acc = dE <= 0.0f || u01(ur.x) < __expf(-beta * dE);

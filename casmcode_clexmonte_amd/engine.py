@@ -71,7 +71,8 @@ def lib():
                      "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device",
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
-                     "clexmc_wave_pick_device"]:
+                     "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
+                     "clexmc_reanchor_energies"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -88,7 +89,8 @@ def exported_symbols() -> List[str]:
             "clexmc_philox_device", "clexmc_last_error", "clexmc_destroy", "clexmc_version",
             "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
             "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
-            "clexmc_wave_pick_device"]
+            "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
+            "clexmc_reanchor_energies"]
 
 
 def _p(a, t):
@@ -102,10 +104,13 @@ class ClexmcError(RuntimeError):
 def pair_phases(flat: FlatCE, sc: Supercell, n_phases: int, seed: int = 0,
                 max_tries: int = 64) -> List[np.ndarray]:
     """Canonical cross-class mixing phases: each phase is a set of disjoint
-    site pairs (s, s+v) for a random lattice translation v, chosen greedily so
-    that no cluster contains sites of two different pairs and the two sites
-    of a pair share no cluster.  Selection depends on positions only, never
-    on occupations, so every pair move is a symmetric proposal."""
+    site pairs (s on sublattice b, t on sublattice b2 in the cell translated
+    by v) for random b, b2 (with the same number of allowed occupants, so
+    that swaps can also move atoms between sublattices) and a random lattice
+    translation v, chosen greedily so that no cluster contains sites of two
+    different pairs and the two sites of a pair share no cluster.  Selection
+    depends on positions only, never on occupations, so every pair move is a
+    symmetric proposal."""
     prim = sc.prim
     offs = interaction_offsets(flat)
     rng = np.random.default_rng(seed)
@@ -135,13 +140,16 @@ def pair_phases(flat: FlatCE, sc: Supercell, n_phases: int, seed: int = 0,
     while len(phases) < n_phases and tries < max_tries * max(1, n_phases):
         tries += 1
         b = variable[int(rng.integers(len(variable)))]
+        b2 = variable[int(rng.integers(len(variable)))]
+        if len(prim.occupants[b2]) != len(prim.occupants[b]):
+            continue
         v = (int(rng.integers(L1)), int(rng.integers(L2)), int(rng.integers(L3)))
-        if v == (0, 0, 0):
+        if b2 == b and v == (0, 0, 0):
             continue
         # the two sites of a pair must not interact
         bad = False
-        for (b2, di, dj, dk) in offs[b]:
-            if b2 == b and (di - v[0]) % L1 == 0 and (dj - v[1]) % L2 == 0 and (dk - v[2]) % L3 == 0:
+        for (bo, di, dj, dk) in offs[b]:
+            if bo == b2 and (di - v[0]) % L1 == 0 and (dj - v[1]) % L2 == 0 and (dk - v[2]) % L3 == 0:
                 bad = True
                 break
         if bad:
@@ -152,13 +160,13 @@ def pair_phases(flat: FlatCE, sc: Supercell, n_phases: int, seed: int = 0,
         for c in order:
             s = b * nc + int(c)
             i, j, k = c % L1, (c // L1) % L2, c // (L1 * L2)
-            t = sc.linear_index(b, i + v[0], j + v[1], k + v[2])
+            t = sc.linear_index(b2, i + v[0], j + v[1], k + v[2])
             if blocked[s] or blocked[t]:
                 continue
             pairs.append((s, t))
             blocked[s] = blocked[t] = True
             blocked[nbr[b][s - b * nc]] = True
-            blocked[nbr[b][t - b * nc]] = True
+            blocked[nbr[b2][t - b2 * nc]] = True
         if pairs:
             phases.append(np.array(pairs, dtype=np.int32))
     return phases
@@ -430,3 +438,15 @@ class Engine:
         self._check(lib().clexmc_wave_pick_device(self.h, C.c_int(v.shape[0]), _p(v, f32p), _p(u, f32p),
                                                   _p(scan, f32p), _p(pick, i32p)), "clexmc_wave_pick_device")
         return scan, pick
+
+    def set_image_packing(self, mode: str):
+        code = {"auto": 0, "int8": 1, "packed2": 2}[mode]
+        self._check(lib().clexmc_set_image_packing(self.h, C.c_int(code)), "clexmc_set_image_packing")
+
+    def set_global_swaps(self, per_pass: int):
+        self._check(lib().clexmc_set_global_swaps(self.h, C.c_int(per_pass)), "clexmc_set_global_swaps")
+
+    def reanchor_energies(self) -> float:
+        d = C.c_double()
+        self._check(lib().clexmc_reanchor_energies(self.h, C.byref(d)), "clexmc_reanchor_energies")
+        return d.value

@@ -19,7 +19,7 @@ from typing import List, Tuple
 
 import numpy as np
 
-from .system import (BasisSet, ClusterExpansion, FlatCE, Prim, Supercell, fcc_prim, KB)
+from .system import (BasisSet, ClusterExpansion, FlatCE, Prim, Supercell, bcc_prim, fcc_prim, hcp_prim, KB)
 
 FCC_ECI_SEED = 20240611
 FCC_NN = 1.0 / math.sqrt(2.0)
@@ -142,3 +142,92 @@ def l12_occupation(sc: Supercell) -> np.ndarray:
         if np.allclose(r, np.round(r), atol=1e-9):
             occ[l] = 1
     return occ
+
+
+def random_occupation_counts(counts, seed: int) -> np.ndarray:
+    """Shuffled occupation with exactly counts[o] sites of occupant o."""
+    rng = np.random.default_rng(seed)
+    occ = np.concatenate([np.full(int(n), o, dtype=np.int8) for o, n in enumerate(counts)])
+    rng.shuffle(occ)
+    return occ
+
+
+def composition_counts(n_sites: int, x) -> List[int]:
+    """Integer occupant counts closest to fractions x (summing to n_sites)."""
+    c = [int(round(xi * n_sites)) for xi in x[:-1]]
+    return c + [n_sites - sum(c)]
+
+
+# ---------------------------------------------------------------------------
+# Ternary HCP workload: uneven supercells in one launch
+# ---------------------------------------------------------------------------
+HCP_ECI_SEED = 20240613
+HCP_SUPERCELLS = [(16, 16, 10), (16, 16, 12), (20, 20, 12), (24, 24, 16),
+                  (32, 32, 20), (36, 36, 24), (40, 40, 24), (48, 48, 32)]
+HCP_COMPOSITION = (0.4, 0.3, 0.3)
+
+
+def hcp_ternary_basis() -> BasisSet:
+    """Ternary (A, B, C) HCP, ideal c/a, 2 sites per primitive cell,
+    Chebyshev basis (2 site functions); pairs to the 6th neighbour shell
+    (2a), triplets with edges within the 3rd shell (c = 1.633a),
+    quadruplets with edges within the 2nd shell (sqrt(2)a): 99 functions."""
+    prim = hcp_prim(1.0, occupants=("A", "B", "C"))
+    return BasisSet(prim, {2: 2.01, 3: 1.64, 4: 1.42}, "chebychev")
+
+
+def hcp_ternary_ce(seed: int = HCP_ECI_SEED) -> ClusterExpansion:
+    bs = hcp_ternary_basis()
+    return ClusterExpansion(bs, seeded_eci(bs, seed, sigma=0.05, nn=1.0))
+
+
+@dataclass
+class MixedBatch:
+    dims: List[Tuple[int, int, int]]   # supercell of each chain
+    temperatures: np.ndarray
+    occ: List[np.ndarray]
+    seeds: np.ndarray
+
+
+def hcp_canonical_batch(prim: Prim, n_chains: int = 64, T_range=(600.0, 1800.0), seed: int = 13) -> MixedBatch:
+    """Chains cycle through HCP_SUPERCELLS (16x16x10 ... 48x48x32, 5,120 to
+    147,456 sites), canonical at x = (0.4, 0.3, 0.3), temperatures
+    linspace(600, 1800) K."""
+    dims, occs = [], []
+    for c in range(n_chains):
+        d = HCP_SUPERCELLS[c % len(HCP_SUPERCELLS)]
+        n = prim.n_sublat * d[0] * d[1] * d[2]
+        dims.append(d)
+        occs.append(random_occupation_counts(composition_counts(n, HCP_COMPOSITION), seed * 1000003 + c))
+    temps = np.linspace(T_range[0], T_range[1], n_chains)
+    return MixedBatch(dims, temps, occs, (np.uint64(seed) << np.uint64(32)) + np.arange(n_chains, dtype=np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# Quaternary BCC workload: 64^3 = 262,144 sites per chain
+# ---------------------------------------------------------------------------
+BCC_ECI_SEED = 20240614
+BCC_NN = math.sqrt(3.0) / 2.0
+
+
+def bcc_quaternary_basis() -> BasisSet:
+    """Quaternary (A, B, C, D) BCC, 1 site per primitive cell, Chebyshev
+    basis (3 site functions); pairs to the 8th neighbour shell (sqrt(5)a,
+    112 neighbours), triplets with edges within the 3rd shell (sqrt(2)a),
+    quadruplets with edges within the 2nd shell (a): 137 functions."""
+    prim = bcc_prim(1.0, ("A", "B", "C", "D"))
+    return BasisSet(prim, {2: 2.24, 3: 1.42, 4: 1.01}, "chebychev")
+
+
+def bcc_quaternary_ce(seed: int = BCC_ECI_SEED) -> ClusterExpansion:
+    bs = bcc_quaternary_basis()
+    return ClusterExpansion(bs, seeded_eci(bs, seed, sigma=0.05, nn=BCC_NN))
+
+
+def bcc_equiatomic_batch(sc: Supercell, n_chains: int = 32, T_range=(300.0, 2000.0), seed: int = 17) -> ChainBatch:
+    """Canonical equiatomic chains, temperatures linspace(300, 2000) K."""
+    counts = composition_counts(sc.n_sites, (0.25, 0.25, 0.25, 0.25))
+    occs = [random_occupation_counts(counts, seed * 1000003 + c) for c in range(n_chains)]
+    temps = np.linspace(T_range[0], T_range[1], n_chains)
+    return ChainBatch(temps, np.zeros((n_chains, 4)), occs,
+                      (np.uint64(seed) << np.uint64(32)) + np.arange(n_chains, dtype=np.uint64), np.zeros(n_chains))

@@ -107,7 +107,9 @@ typedef struct {
   const int *class_begin;   /* [n_classes+1] */
   const int *class_sites;   /* linear site indices */
   /* canonical cross-class mixing: phases of disjoint, mutually
-   * non-interacting site pairs (optional; n_pair_phases may be 0) */
+   * non-interacting site pairs (optional; n_pair_phases may be 0); the two
+   * sites of a pair may lie on different sublattices with the same number
+   * of occupants (occupant indices must denote the same species) */
   int n_pair_phases;
   const int *pair_phase_begin; /* [n_pair_phases+1] in pairs */
   const int *pair_sites;       /* [n_pairs][2] */
@@ -172,12 +174,30 @@ clexmc_status clexmc_read_state(clexmc_handle *h, int chain, int8_t *occ, double
  * (tracked incrementally, int64 [n_orbits]). */
 clexmc_status clexmc_read_binary_counts(clexmc_handle *h, int chain, long long *counts_out);
 
+/* Replaces every chain's FP64 running energy (the sum of FP32 dEs on the
+ * table path) by a full FP64 recompute of its correlations and returns the
+ * largest |running - recomputed| (eV per supercell).  The binary path's
+ * energies come from exact integer counts and show zero drift. */
+clexmc_status clexmc_reanchor_energies(clexmc_handle *h, double *max_drift);
+
 /* Kernel timing of the last clexmc_run_passes (HIP events on the handle's
  * stream, milliseconds). */
 clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
 
 /* Tuning knobs (block size of the Metropolis kernel; 0 = automatic). */
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
+
+/* Canonical runs: serial swaps of uniformly random site pairs per pass
+ * (default 1), beside the parallel independent-set matchings and pair
+ * phases.  The pair may share clusters (the reference's unrestricted
+ * OccSwap proposal, CanonicalCalculator.cc:78-83); these moves keep every
+ * supercell ergodic, including ones too small for non-interacting pairs. */
+clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
+
+/* LDS image format of the Metropolis kernel: 0 automatic (int8 per site when
+ * the largest padded supercell fits, else 2 bits per site), 1 int8, 2 packed
+ * 2-bit (at most 4 occupants per site). */
+clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode);
 
 /* ---- learned proposal (no counterpart in the reference: it plugs a
  * proposal into the Metropolis loop occupation_metropolis_v2,

@@ -108,15 +108,18 @@ def test_correlations_vs_oracle(fcc_bin, which):
     eng.close()
 
 
+@pytest.mark.parametrize("packing", ["int8", "packed2"])
 @pytest.mark.parametrize("kind", ["semigrand_canonical", "canonical"])
-def test_binary_run_exact_counts(fcc_bin, kind):
+def test_binary_run_exact_counts(fcc_bin, kind, packing):
     """After a Metropolis run, the exact int64 correlation counts tracked on
     the device equal the oracle's full recompute of the final occupation, and
-    the occupant counts are exact (conserved for canonical)."""
-    _print_case(f"binary run exact counts {kind}")
+    the occupant counts are exact (conserved for canonical); int8 and 2-bit
+    packed LDS images."""
+    _print_case(f"binary run exact counts {kind} {packing}")
     flat = fcc_bin
     sc = Supercell(flat.ce.prim, (12, 12, 12))
     eng = Engine(flat)
+    eng.set_image_packing(packing)
     sid = eng.add_supercell(sc, n_pair_phases=8, seed=1)
     occs = [random_occupation(sc.n_sites, 0.5, s) for s in range(4)]
     chem = [[0.0, mu] for mu in (-0.05, 0.0, 0.05, 0.1)]
@@ -141,14 +144,17 @@ def test_binary_run_exact_counts(fcc_bin, kind):
     eng.close()
 
 
+@pytest.mark.parametrize("packing", ["int8", "packed2"])
 @pytest.mark.parametrize("kind", ["semigrand_canonical", "canonical"])
-def test_general_path_run_energy_tracking(kind):
+def test_general_path_run_energy_tracking(kind, packing):
     """Ternary (table path): FP64 running energy of FP32 dEs versus the
-    oracle's energy of the final occupation; composition exact."""
-    _print_case(f"general run {kind}")
+    oracle's energy of the final occupation; composition exact; int8 and
+    2-bit packed LDS images."""
+    _print_case(f"general run {kind} {packing}")
     flat = ternary_fcc_flat()
     sc = Supercell(flat.ce.prim, (9, 9, 9))
     eng = Engine(flat)
+    eng.set_image_packing(packing)
     sid = eng.add_supercell(sc, n_pair_phases=4, seed=2)
     rng = np.random.default_rng(8)
     occs = [rng.integers(0, 3, sc.n_sites).astype(np.int8) for _ in range(2)]

@@ -86,33 +86,34 @@ def test_canonical_exact_enumeration():
 def test_canonical_workload_vs_oracle():
     """12x12x12 FCC, x_B = 0.5, T = 600 K (the reference's CPU-runnable
     canonical case): GPU chains versus the oracle's restatement of the
-    reference loop."""
+    reference loop (8 independent oracle chains; both sides 1,000
+    equilibration + 2,000 sampled passes)."""
     print("[case] canonical 12^3 vs oracle", flush=True)
+    from concurrent.futures import ThreadPoolExecutor
     ce = fcc_binary_ce()
     flat = FlatCE(ce)
     sc = Supercell(ce.prim, (12, 12, 12))
     T = 600.0
     nc = sc.n_cells
-    # oracle: 2 chains, 300 equilibration + 1200 sampled passes
     osys = O.OracleSystem(flat, sc.dims)
-    o_means, o_vars = [], []
-    for c in range(2):
-        r = osys.run(random_occupation(sc.n_sites, 0.5, 40 + c), "canonical", T, 300, 1200, 77 + c,
+
+    def orc(c):
+        r = osys.run(random_occupation(sc.n_sites, 0.5, 40 + c), "canonical", T, 1000, 2000, 77 + c,
                      full_recompute_samplers=False)
-        m, e = mean_and_error(r["E"] / nc)
-        o_means.append(m)
-        o_vars.append(e * e)
-    o_mean = float(np.mean(o_means))
-    o_err = float(np.sqrt(np.sum(o_vars)) / len(o_means))
-    # GPU: 32 chains, 300 + 1200 passes (canonical matching rounds: 2)
+        return float(np.mean(r["E"] / nc))
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        o_chain = np.array(list(ex.map(orc, range(8))))
+    o_mean = float(o_chain.mean())
+    o_err = float(o_chain.std(ddof=1) / np.sqrt(len(o_chain)))
     nch = 32
     eng = Engine(flat)
     sid = eng.add_supercell(sc, n_pair_phases=16, seed=3)
     occs = [random_occupation(sc.n_sites, 0.5, 500 + c) for c in range(nch)]
     eng.load_chains([sid] * nch, occs, [T] * nch, 3000 + np.arange(nch))
-    eng.run_passes("canonical", 1500, 1, canonical_rounds=2)
+    eng.run_passes("canonical", 3000, 1, canonical_rounds=2)
     E, cnt = eng.read_samples()
-    per_chain = E[:, 300:].mean(1) / nc
+    per_chain = E[:, 1000:].mean(1) / nc
     g_mean = float(per_chain.mean())
     g_err = float(per_chain.std(ddof=1) / np.sqrt(nch))
     tol = NSIG * np.sqrt(g_err ** 2 + o_err ** 2)
