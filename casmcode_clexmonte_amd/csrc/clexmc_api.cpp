@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,9 @@ struct clexmc_handle {
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
+  hipGraph_t graph = nullptr;       // captured block of passes
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_replays_left = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   // learned proposal
@@ -727,7 +731,10 @@ static clexmc_status sync_supercells(clexmc_handle *h) {
   return CLEXMC_OK;
 }
 
+static void graph_release(clexmc_handle *h);
+
 static void free_chains(clexmc_handle *h) {
+  graph_release(h);
   hipFree(h->d_params);
   hipFree(h->d_state);
   hipFree(h->d_occ);
@@ -941,43 +948,22 @@ clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode) {
   return CLEXMC_OK;
 }
 
-clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
-  if (!h) return CLEXMC_ERR_ARG;
-  ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
-  ARG_CHECK(n_passes >= 0 && sample_period >= 1, "bad n_passes / sample_period");
-  ARG_CHECK(h->n_chains > 0, "no chains loaded");
-  if (n_passes == 0) return CLEXMC_OK;
-  HIP_TRY(hipSetDevice(h->device));
-  int max_pad = 0, max_class = 1;
-  for (int c = 0; c < h->n_chains; ++c) {
-    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
-    max_pad = std::max(max_pad, d.pad_bytes);
-    max_class = std::max(max_class, d.max_class);
-  }
-  int threads = h->block_size;
-  if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
-  const int n_tab = h->binary ? 0 : h->n_flower_tables;
-  int max_lds = 0;
-  HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-  // int8 image when it fits (or is forced), else 2 bits per site (<= 4 occupants)
-  bool packed = h->packing == 2;
-  size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
-  if (h->packing == 0 && (long long)lds > (long long)max_lds - 64 && h->n_occ_max <= 4) {
-    packed = true;
-    lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
-  }
-  ARG_CHECK(!packed || h->n_occ_max <= 4, "2-bit packed images need at most 4 occupants per site");
-  if (h->proposal == 0 && (long long)lds > (long long)max_lds - 64) {
-    h->err = "chain image does not fit in LDS (" + std::to_string(lds) + " > " + std::to_string(max_lds) + " bytes)";
-    return CLEXMC_ERR_UNSUPPORTED;
-  }
-  h->last_packed = packed;
-  // sample capacity (append)
+// Launch plan of the Metropolis (or grouped-proposal) kernel for the loaded
+// chains: block size, LDS image format and size, kernel arguments.
+struct RunPlan {
+  int kind = 0, threads = 64, weights = 0;
+  bool learned = false, packed = false;
+  size_t lds = 0;
+  RunArgs a;
+  LearnArgs la;
+};
+
+static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
   std::vector<int> cnt(h->n_chains);
   HIP_TRY(hipMemcpy(cnt.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
   int used = 0;
   for (int v : cnt) used = std::max(used, v);
-  const int need = used + n_passes / sample_period + 2;
+  const int need = used + n_new_samples + 2;
   if (need > h->sample_cap) {
     const int cap = std::max(need, 2 * h->sample_cap);
     double *nE = nullptr;
@@ -996,7 +982,53 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
     h->d_sampN = nN;
     h->sample_cap = cap;
   }
-  RunArgs a;
+  return CLEXMC_OK;
+}
+
+// threads_override / packing_override: 0 = the handle's setting
+static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
+                              int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0) {
+  int max_pad = 0, max_class = 1;
+  for (int c = 0; c < h->n_chains; ++c) {
+    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
+    max_pad = std::max(max_pad, d.pad_bytes);
+    max_class = std::max(max_class, d.max_class);
+  }
+  int threads = threads_override ? threads_override : h->block_size;
+  if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+  const int packing = packing_override ? packing_override : h->packing;
+  const int n_tab = h->binary ? 0 : h->n_flower_tables;
+  int max_lds = 0;
+  HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  // int8 image when it fits (or is forced), else 2 bits per site (<= 4 occupants)
+  bool packed = packing == 2;
+  size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
+  if (packing == 0 && (long long)lds > (long long)max_lds - 64 && h->n_occ_max <= 4) {
+    packed = true;
+    lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
+  }
+  ARG_CHECK(!packed || h->n_occ_max <= 4, "2-bit packed images need at most 4 occupants per site");
+  P.learned = h->proposal != 0;
+  if (P.learned) {
+    if (!h->binary) {
+      h->err = "the learned proposal is implemented for binary (+-1) cluster expansions";
+      return CLEXMC_ERR_UNSUPPORTED;
+    }
+    packed = false;
+    lds = learned_lds_bytes(max_pad);
+  }
+  if ((long long)lds > (long long)max_lds - 64) {
+    h->err = "chain image does not fit in LDS (" + std::to_string(lds) + " > " + std::to_string(max_lds) + " bytes)";
+    return CLEXMC_ERR_UNSUPPORTED;
+  }
+  clexmc_status st = grow_samples(h, total_passes / sample_period);
+  if (st != CLEXMC_OK) return st;
+  P.kind = kind;
+  P.threads = threads;
+  P.packed = packed;
+  P.lds = lds;
+  P.weights = h->proposal == 1 ? 1 : (h->proposal == 3 ? 2 : 0);
+  RunArgs &a = P.a;
   a.sys = h->d_sys;
   a.scs = h->d_scs;
   a.params = h->d_params;
@@ -1013,31 +1045,200 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   a.n_orbits = h->n_orbits;
   a.n_tab = n_tab;
   a.global_swaps = h->global_swaps;
-  if (h->proposal != 0) {
-    if (!h->binary) {
-      h->err = "the learned proposal is implemented for binary (+-1) cluster expansions";
-      return CLEXMC_ERR_UNSUPPORTED;
-    }
-    lds = learned_lds_bytes(max_pad);
-    if ((long long)lds > (long long)max_lds - 64) {
-      h->err = "chain image does not fit in LDS";
-      return CLEXMC_ERR_UNSUPPORTED;
-    }
-    LearnArgs la;
-    la.mlp = h->d_mlp;
-    la.rounds_per_group = h->rounds_per_group;
-    HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    const int weights = h->proposal == 1 ? 1 : (h->proposal == 3 ? 2 : 0);
-    HIP_TRY(launch_learned(a, la, kind, weights, h->n_chains, threads, lds, h->stream));
-  } else {
-    HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    HIP_TRY(launch_metropolis(a, h->binary != 0, kind, packed, h->n_chains, threads, lds, h->stream));
-  }
+  P.la.mlp = h->d_mlp;
+  P.la.rounds_per_group = h->rounds_per_group;
+  return CLEXMC_OK;
+}
+
+// attribute setting (not a stream operation) and the launch, separately, so
+// that a launch can be recorded into a graph
+static hipError_t plan_attributes(const RunPlan &P, clexmc_handle *h) {
+  if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, false);
+  return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, false);
+}
+
+static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h) {
+  if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, true);
+  return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, true);
+}
+
+clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
+  ARG_CHECK(n_passes >= 0 && sample_period >= 1, "bad n_passes / sample_period");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  if (n_passes == 0) return CLEXMC_OK;
+  HIP_TRY(hipSetDevice(h->device));
+  RunPlan P;
+  clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, P);
+  if (st != CLEXMC_OK) return st;
+  h->last_packed = P.packed;
+  HIP_TRY(plan_attributes(P, h));
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(plan_launch(P, h));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
+  return CLEXMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// HIP graph of a block of passes: captured once, replayed many times.  The
+// kernel reads each chain's pass and draw counters from device memory, so
+// replays continue the chains exactly like successive clexmc_run_passes calls
+// of the same block length.  The sample buffer is sized for max_replays at
+// capture time (the graph holds its address).
+// ---------------------------------------------------------------------------
+static void graph_release(clexmc_handle *h) {
+  if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->graph_replays_left = 0;
+}
+
+clexmc_status clexmc_graph_capture(clexmc_handle *h, int kind, int passes_per_replay, int max_replays,
+                                   int sample_period, int canonical_rounds) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
+  ARG_CHECK(passes_per_replay >= 1 && max_replays >= 1 && sample_period >= 1, "bad graph arguments");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  graph_release(h);
+  RunPlan P;
+  clexmc_status st = plan_run(h, kind, passes_per_replay, sample_period, canonical_rounds,
+                              passes_per_replay * max_replays, P);
+  if (st != CLEXMC_OK) return st;
+  HIP_TRY(plan_attributes(P, h));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = plan_launch(P, h);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    h->err = std::string("graph capture: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  h->graph = g;
+  e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    graph_release(h);
+    h->err = std::string("graph instantiate: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  h->graph_replays_left = max_replays;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_graph_replay(clexmc_handle *h, int n_replays) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->graph_exec != nullptr, "no captured graph");
+  ARG_CHECK(n_replays >= 1 && n_replays <= h->graph_replays_left, "replays beyond the captured sample capacity");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < n_replays; ++r) HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->graph_replays_left -= n_replays;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_graph_release(clexmc_handle *h) {
+  if (!h) return CLEXMC_ERR_ARG;
+  graph_release(h);
+  return CLEXMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times short trial launches over block sizes (and LDS image
+// formats where the int8 image fits) on the loaded chains, restoring the
+// chains' device state after every trial, and keeps the fastest setting.
+// Results are cached per (kind, proposal, CE shape, chain count, largest
+// padded supercell, largest class) for the life of the process.
+// ---------------------------------------------------------------------------
+static std::map<std::string, std::pair<int, int>> g_tune_cache;
+
+clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int canonical_rounds, int *threads_out,
+                              int *packing_out, double *ms_per_pass_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
+  ARG_CHECK(trial_passes >= 1, "trial_passes >= 1 required");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  HIP_TRY(hipSetDevice(h->device));
+  int max_pad = 0, max_class = 1;
+  long long occ_bytes = 0;
+  for (int c = 0; c < h->n_chains; ++c) {
+    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
+    max_pad = std::max(max_pad, d.pad_bytes);
+    max_class = std::max(max_class, d.max_class);
+    occ_bytes += d.n_sites;
+  }
+  const std::string key = std::to_string(kind) + "/" + std::to_string(h->proposal) + "/" + std::to_string(h->n_corr) +
+                          "/" + std::to_string(h->n_sublat) + "/" + std::to_string(h->n_chains) + "/" +
+                          std::to_string(max_pad) + "/" + std::to_string(max_class) + "/" +
+                          std::to_string(trial_passes);
+  auto it = g_tune_cache.find(key);
+  double best_ms = -1.0;
+  int best_t = 0, best_p = 0;
+  if (it != g_tune_cache.end()) {
+    best_t = it->second.first;
+    best_p = it->second.second;
+  } else {
+    // snapshot of the chains' device state
+    DevBufs B;
+    int8_t *s_occ;
+    ChainState *s_state;
+    long long *s_binC;
+    int *s_cnt;
+    HIP_TRY(B.alloc(&s_occ, (size_t)occ_bytes));
+    HIP_TRY(B.alloc(&s_state, (size_t)h->n_chains));
+    HIP_TRY(B.alloc(&s_binC, (size_t)h->n_chains * h->n_orbits));
+    HIP_TRY(B.alloc(&s_cnt, (size_t)h->n_chains));
+    HIP_TRY(hipMemcpy(s_occ, h->d_occ, (size_t)occ_bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(s_state, h->d_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(s_binC, h->d_binC, sizeof(long long) * h->n_chains * h->n_orbits, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(s_cnt, h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToDevice));
+    const int t_max = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+    for (int packing = 1; packing <= 2; ++packing) {
+      for (int t = 64; t <= t_max; t *= 2) {
+        RunPlan P;
+        if (plan_run(h, kind, trial_passes, trial_passes + 1, canonical_rounds, trial_passes, P, t, packing) !=
+            CLEXMC_OK)
+          continue;
+        if (packing == 2 && (P.learned || !P.packed)) continue;
+        HIP_TRY(plan_attributes(P, h));
+        HIP_TRY(hipEventRecord(h->ev0, h->stream));
+        HIP_TRY(plan_launch(P, h));
+        HIP_TRY(hipEventRecord(h->ev1, h->stream));
+        HIP_TRY(hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        HIP_TRY(hipMemcpy(h->d_occ, s_occ, (size_t)occ_bytes, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(h->d_state, s_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(h->d_binC, s_binC, sizeof(long long) * h->n_chains * h->n_orbits, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(h->d_sample_count, s_cnt, sizeof(int) * h->n_chains, hipMemcpyDeviceToDevice));
+        if (best_ms < 0.0 || ms < best_ms) {
+          best_ms = ms;
+          best_t = t;
+          best_p = packing;
+        }
+      }
+    }
+    ARG_CHECK(best_t > 0, "no launch configuration fits");
+    g_tune_cache[key] = std::make_pair(best_t, best_p);
+  }
+  h->block_size = best_t;
+  h->packing = best_p;
+  if (threads_out) *threads_out = best_t;
+  if (packing_out) *packing_out = best_p;
+  if (ms_per_pass_out) *ms_per_pass_out = best_ms < 0.0 ? -1.0 : best_ms / trial_passes;
   return CLEXMC_OK;
 }
 

@@ -91,7 +91,7 @@ struct AdamArgs {
 };
 
 hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
-                          int threads, size_t lds, hipStream_t st);
+                          int threads, size_t lds, hipStream_t st, bool launch);
 size_t learned_lds_bytes(int pad_bytes);
 hipError_t launch_mlp_infer(const MlpDev *mlp, int n, const int8_t *X, float *out, hipStream_t st);
 hipError_t launch_wave_pick(int n_rows, const float *v, const float *u, float *scan_out, int *pick_out,
@@ -102,7 +102,7 @@ hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);
 hipError_t launch_mlp_publish(const float *p, MlpDev *m, int n_env, float eps, hipStream_t st);
 
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
-                             size_t lds, hipStream_t st);
+                             size_t lds, hipStream_t st, bool launch);
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);

@@ -753,29 +753,31 @@ __global__ void philox_kernel(int n, const uint4 *ctr, const uint2 *key, uint4 *
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+// launch == false: only set the kernel's dynamic LDS limit (not a stream
+// operation, so it stays outside graph capture)
 template <bool BIN, int KIND, bool PACK>
-static hipError_t launch_metro(const RunArgs &args, int n_chains, int threads, size_t lds, hipStream_t st) {
+static hipError_t launch_metro(const RunArgs &args, int n_chains, int threads, size_t lds, hipStream_t st,
+                               bool launch) {
   auto kern = metropolis_kernel<BIN, KIND, PACK>;
-  hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
+  if (!launch) return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args);
   return hipGetLastError();
 }
 
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
-                             size_t lds, hipStream_t st) {
+                             size_t lds, hipStream_t st, bool launch) {
   if (packed) {
     if (binary)
-      return kind == 0 ? launch_metro<true, 0, true>(args, n_chains, threads, lds, st)
-                       : launch_metro<true, 1, true>(args, n_chains, threads, lds, st);
-    return kind == 0 ? launch_metro<false, 0, true>(args, n_chains, threads, lds, st)
-                     : launch_metro<false, 1, true>(args, n_chains, threads, lds, st);
+      return kind == 0 ? launch_metro<true, 0, true>(args, n_chains, threads, lds, st, launch)
+                       : launch_metro<true, 1, true>(args, n_chains, threads, lds, st, launch);
+    return kind == 0 ? launch_metro<false, 0, true>(args, n_chains, threads, lds, st, launch)
+                     : launch_metro<false, 1, true>(args, n_chains, threads, lds, st, launch);
   }
   if (binary)
-    return kind == 0 ? launch_metro<true, 0, false>(args, n_chains, threads, lds, st)
-                     : launch_metro<true, 1, false>(args, n_chains, threads, lds, st);
-  return kind == 0 ? launch_metro<false, 0, false>(args, n_chains, threads, lds, st)
-                   : launch_metro<false, 1, false>(args, n_chains, threads, lds, st);
+    return kind == 0 ? launch_metro<true, 0, false>(args, n_chains, threads, lds, st, launch)
+                     : launch_metro<true, 1, false>(args, n_chains, threads, lds, st, launch);
+  return kind == 0 ? launch_metro<false, 0, false>(args, n_chains, threads, lds, st, launch)
+                   : launch_metro<false, 1, false>(args, n_chains, threads, lds, st, launch);
 }
 
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st) {

@@ -772,24 +772,23 @@ __global__ void mlp_publish_kernel(const float *p, MlpDev *m, int n_env, float e
 // ---------------------------------------------------------------------------
 template <int KIND, int WEIGHTS>
 static hipError_t launch_l(const RunArgs &args, const LearnArgs &la, int n_chains, int threads, size_t lds,
-                           hipStream_t st) {
+                           hipStream_t st, bool launch) {
   auto kern = learned_kernel<KIND, WEIGHTS>;
-  hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
+  if (!launch) return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args, la);
   return hipGetLastError();
 }
 
 hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
-                          int threads, size_t lds, hipStream_t st) {
+                          int threads, size_t lds, hipStream_t st, bool launch) {
   if (kind == 0) {
-    if (weights == 1) return launch_l<0, 1>(args, la, n_chains, threads, lds, st);
-    if (weights == 2) return launch_l<0, 2>(args, la, n_chains, threads, lds, st);
-    return launch_l<0, 0>(args, la, n_chains, threads, lds, st);
+    if (weights == 1) return launch_l<0, 1>(args, la, n_chains, threads, lds, st, launch);
+    if (weights == 2) return launch_l<0, 2>(args, la, n_chains, threads, lds, st, launch);
+    return launch_l<0, 0>(args, la, n_chains, threads, lds, st, launch);
   }
-  if (weights == 1) return launch_l<1, 1>(args, la, n_chains, threads, lds, st);
-  if (weights == 2) return launch_l<1, 2>(args, la, n_chains, threads, lds, st);
-  return launch_l<1, 0>(args, la, n_chains, threads, lds, st);
+  if (weights == 1) return launch_l<1, 1>(args, la, n_chains, threads, lds, st, launch);
+  if (weights == 2) return launch_l<1, 2>(args, la, n_chains, threads, lds, st, launch);
+  return launch_l<1, 0>(args, la, n_chains, threads, lds, st, launch);
 }
 
 size_t learned_lds_bytes(int pad_bytes) {

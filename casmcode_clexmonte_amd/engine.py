@@ -72,7 +72,8 @@ def lib():
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
-                     "clexmc_reanchor_energies"]:
+                     "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
+                     "clexmc_graph_release", "clexmc_autotune"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -90,7 +91,8 @@ def exported_symbols() -> List[str]:
             "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
             "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
-            "clexmc_reanchor_energies"]
+            "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
+            "clexmc_graph_release", "clexmc_autotune"]
 
 
 def _p(a, t):
@@ -450,3 +452,30 @@ class Engine:
         d = C.c_double()
         self._check(lib().clexmc_reanchor_energies(self.h, C.byref(d)), "clexmc_reanchor_energies")
         return d.value
+
+    # -- HIP graph of a block of passes ------------------------------------------
+    def graph_capture(self, kind: str, passes_per_replay: int, max_replays: int, sample_period: int = 1,
+                      canonical_rounds: int = 2):
+        k = 0 if kind == "canonical" else 1
+        self._check(lib().clexmc_graph_capture(self.h, C.c_int(k), C.c_int(passes_per_replay), C.c_int(max_replays),
+                                               C.c_int(sample_period), C.c_int(canonical_rounds)),
+                    "clexmc_graph_capture")
+
+    def graph_replay(self, n_replays: int = 1) -> float:
+        self._check(lib().clexmc_graph_replay(self.h, C.c_int(n_replays)), "clexmc_graph_replay")
+        ms = C.c_double()
+        lib().clexmc_last_run_ms(self.h, C.byref(ms))
+        return ms.value
+
+    def graph_release(self):
+        self._check(lib().clexmc_graph_release(self.h), "clexmc_graph_release")
+
+    def autotune(self, kind: str, trial_passes: int = 20, canonical_rounds: int = 2):
+        k = 0 if kind == "canonical" else 1
+        t = C.c_int()
+        p = C.c_int()
+        ms = C.c_double()
+        self._check(lib().clexmc_autotune(self.h, C.c_int(k), C.c_int(trial_passes), C.c_int(canonical_rounds),
+                                          C.byref(t), C.byref(p), C.byref(ms)), "clexmc_autotune")
+        return {"threads": t.value, "packing": {1: "int8", 2: "packed2"}.get(p.value, "auto"),
+                "ms_per_pass": ms.value}

@@ -174,6 +174,27 @@ clexmc_status clexmc_read_state(clexmc_handle *h, int chain, int8_t *occ, double
  * (tracked incrementally, int64 [n_orbits]). */
 clexmc_status clexmc_read_binary_counts(clexmc_handle *h, int chain, long long *counts_out);
 
+/* HIP graph of one block of passes (the persistent Metropolis launch),
+ * captured once and replayed: each replay continues every chain exactly as a
+ * clexmc_run_passes call of passes_per_replay passes would.  Samples for up
+ * to max_replays replays are preallocated at capture.  The graph is released
+ * by clexmc_graph_release, clexmc_load_chains or clexmc_destroy.  (No
+ * reference counterpart: the reference's run loop is host code,
+ * occupation_metropolis.hh:72-123.) */
+clexmc_status clexmc_graph_capture(clexmc_handle *h, int kind, int passes_per_replay, int max_replays,
+                                   int sample_period, int canonical_rounds);
+clexmc_status clexmc_graph_replay(clexmc_handle *h, int n_replays);
+clexmc_status clexmc_graph_release(clexmc_handle *h);
+
+/* Autotuner: times trial_passes-pass launches of the loaded chains over
+ * block sizes (64..512) and LDS image formats (int8 where it fits, 2-bit
+ * packed), restores the chains' device state after each trial, and sets the
+ * fastest as the handle's block size and packing.  Cached per (kind,
+ * proposal, CE shape, chain count, largest padded supercell, largest class).
+ * Outputs may be NULL; ms_per_pass is -1 when the cache answered. */
+clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int canonical_rounds, int *threads_out,
+                              int *packing_out, double *ms_per_pass_out);
+
 /* Replaces every chain's FP64 running energy (the sum of FP32 dEs on the
  * table path) by a full FP64 recompute of its correlations and returns the
  * largest |running - recomputed| (eV per supercell).  The binary path's

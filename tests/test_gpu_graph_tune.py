@@ -1,0 +1,101 @@
+"""HIP-graph replays and the autotuner.
+
+* A captured block of passes replayed k times leaves every chain bit-for-bit
+  where k eager clexmc_run_passes calls of the same block leave it
+  (occupations, running energies, counters, samples), checked after every
+  replay with eager reads and synchronisation in between.
+* The autotuner restores the chains' state after its trials, picks a valid
+  configuration, answers a repeated query from its cache, and a run with the
+  tuned configuration equals a run with that configuration set by hand.
+"""
+import numpy as np
+import pytest
+
+from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.system import FlatCE, Supercell
+from casmcode_clexmonte_amd.workloads import (composition_counts, fcc_binary_ce, hcp_ternary_ce, random_occupation,
+                                              random_occupation_counts)
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def _setup(which):
+    if which == "binary_sgc":
+        ce = fcc_binary_ce()
+        flat = FlatCE(ce)
+        sc = Supercell(ce.prim, (12, 12, 12))
+        occs = [random_occupation(sc.n_sites, 0.5, 10 + c, exact=False) for c in range(8)]
+        chem = [[0.0, mu] for mu in np.linspace(-0.1, 0.1, 8)]
+        return flat, sc, occs, np.linspace(300, 1500, 8), chem, "semigrand_canonical", 0
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (8, 8, 6))
+    occs = [random_occupation_counts(composition_counts(sc.n_sites, (0.4, 0.3, 0.3)), 20 + c) for c in range(4)]
+    return flat, sc, occs, np.linspace(600, 1800, 4), None, "canonical", 4
+
+
+def _engine(flat, sc, occs, temps, chem, n_phases):
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=n_phases, seed=2)
+    eng.load_chains([sid] * len(occs), occs, temps, 500 + np.arange(len(occs)), chem=chem)
+    return eng
+
+
+def _states(eng, n):
+    return [eng.read_state(c) for c in range(n)]
+
+
+def _same(sa, sb):
+    for a, b in zip(sa, sb):
+        assert np.array_equal(a["occ"], b["occ"])
+        assert a["E"] == b["E"]
+        assert a["n_attempt"] == b["n_attempt"] and a["n_accept"] == b["n_accept"]
+
+
+@pytest.mark.parametrize("which", ["binary_sgc", "ternary_canonical"])
+def test_graph_replays_equal_eager_blocks(which):
+    _print_case(f"graph replays vs eager {which}")
+    flat, sc, occs, temps, chem, kind, n_phases = _setup(which)
+    eager = _engine(flat, sc, occs, temps, chem, n_phases)
+    graph = _engine(flat, sc, occs, temps, chem, n_phases)
+    block, reps = 10, 4
+    graph.graph_capture(kind, block, reps, sample_period=1, canonical_rounds=2)
+    for r in range(reps):
+        eager.run_passes(kind, block, 1, canonical_rounds=2)
+        graph.graph_replay(1)
+        _same(_states(eager, len(occs)), _states(graph, len(occs)))
+        Ee, ce = eager.read_samples()
+        Eg, cg = graph.read_samples()
+        assert Ee.shape == Eg.shape == (len(occs), block * (r + 1))
+        assert np.array_equal(Ee, Eg) and np.array_equal(ce, cg)
+    with pytest.raises(Exception):
+        graph.graph_replay(1)        # beyond the captured sample capacity
+    graph.graph_release()
+    eager.close()
+    graph.close()
+
+
+@pytest.mark.parametrize("which", ["binary_sgc", "ternary_canonical"])
+def test_autotune_restores_state_and_matches_manual_setting(which):
+    _print_case(f"autotune {which}")
+    flat, sc, occs, temps, chem, kind, n_phases = _setup(which)
+    tuned = _engine(flat, sc, occs, temps, chem, n_phases)
+    before = _states(tuned, len(occs))
+    cfg = tuned.autotune(kind, trial_passes=5)
+    print(cfg)
+    assert cfg["threads"] in (64, 128, 256, 512) and cfg["packing"] in ("int8", "packed2")
+    assert cfg["ms_per_pass"] > 0
+    _same(before, _states(tuned, len(occs)))
+    assert tuned.autotune(kind, trial_passes=5)["ms_per_pass"] == -1.0    # cached
+    manual = _engine(flat, sc, occs, temps, chem, n_phases)
+    manual.set_block_size(cfg["threads"])
+    manual.set_image_packing(cfg["packing"])
+    tuned.run_passes(kind, 15, 1, canonical_rounds=2)
+    manual.run_passes(kind, 15, 1, canonical_rounds=2)
+    _same(_states(tuned, len(occs)), _states(manual, len(occs)))
+    tuned.close()
+    manual.close()
