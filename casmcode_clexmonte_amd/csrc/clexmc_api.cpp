@@ -19,6 +19,7 @@
 #include "../../include/clexmc.h"
 #include "clexmc_device.h"
 #include "clexmc_kernels.h"
+#include "clexmc_jit.h"
 
 using namespace clexmc;
 
@@ -27,6 +28,13 @@ namespace {
 struct HostSupercell {
   DevSupercell dev;  // device pointers filled in
   std::vector<void *> allocs;
+  // binary site programs (host copies, for run-time specialisation)
+  std::vector<BinSub> bin_sub;
+  std::vector<int> bin_off;
+  // specialised kernels: 0 not tried, 1 loaded, -1 unavailable
+  int jit_state = 0;
+  hipModule_t jit_mod = nullptr;
+  hipFunction_t jit_fn[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 }  // namespace
@@ -67,7 +75,11 @@ struct clexmc_handle {
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
+  int last_specialized = 0, last_threads = 0;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
+  int n_slots = 0;                  // binary path: orbits with >= 1 site
+  int specialize = 1;               // run-time specialised binary kernels (1 when possible, 0 never)
+  std::string jit_log;              // last specialisation compiler log
   hipGraph_t graph = nullptr;       // captured block of passes
   hipGraphExec_t graph_exec = nullptr;
   int graph_replays_left = 0;
@@ -251,6 +263,7 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
       h->bin_w[a] = s->bin_orbit_weight[a];
       if (s->orbit_k[a] >= 1) d.slot_orbit[d.n_slots++] = a;
     }
+    h->n_slots = d.n_slots;
   }
   // environment of the learned proposal: sites of the three shortest pair orbits
   {
@@ -702,6 +715,8 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, class_addr.data(), class_addr.size(), (int **)&d.class_addr));
   HIP_TRY(upload(h, A, bin_sub.data(), bin_sub.size(), (BinSub **)&d.bin_sub));
   HIP_TRY(upload(h, A, bin_off.data(), bin_off.size(), (int **)&d.bin_off));
+  hs.bin_sub = bin_sub;
+  hs.bin_off = bin_off;
   HIP_TRY(upload(h, A, gen_sub.data(), gen_sub.size(), (GenSub **)&d.gen_sub));
   HIP_TRY(upload(h, A, gen_prog.data(), gen_prog.size(), (int **)&d.gen_prog));
   HIP_TRY(upload(h, A, env_off.data(), env_off.size(), (int **)&d.env_off));
@@ -941,6 +956,29 @@ clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass) {
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_set_specialization(clexmc_handle *h, int mode) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(mode == 0 || mode == 1, "specialization must be 0 (off) or 1 (when possible)");
+  h->specialize = mode;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_specialization_log(clexmc_handle *h, char *buf, int cap) {
+  if (!h || !buf || cap < 1) return CLEXMC_ERR_ARG;
+  const int n = std::min(cap - 1, (int)h->jit_log.size());
+  std::memcpy(buf, h->jit_log.data(), (size_t)n);
+  buf[n] = '\0';
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_last_run_info(clexmc_handle *h, int *specialized, int *packed, int *threads) {
+  if (!h) return CLEXMC_ERR_ARG;
+  if (specialized) *specialized = h->last_specialized;
+  if (packed) *packed = h->last_packed ? 1 : 0;
+  if (threads) *threads = h->last_threads;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(mode >= 0 && mode <= 2, "packing must be 0 (automatic), 1 (int8) or 2 (2-bit)");
@@ -954,9 +992,39 @@ struct RunPlan {
   int kind = 0, threads = 64, weights = 0;
   bool learned = false, packed = false;
   size_t lds = 0;
+  hipFunction_t jit = nullptr;  // specialised kernel (binary path, one supercell shape)
   RunArgs a;
   LearnArgs la;
 };
+
+// The specialised kernel for (supercell, kind, packing), compiled on first
+// use; nullptr when specialisation is off or unavailable.
+static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed) {
+  HostSupercell &hs = h->scs[sc_id];
+  if (hs.jit_state == 0) {
+    hs.jit_state = -1;
+    std::vector<int> variable(h->n_sublat);
+    for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
+    const std::string src = jit_source(h->n_sublat, variable, hs.bin_sub, hs.bin_off, h->n_slots);
+    hipModule_t mod = nullptr;
+    const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
+    if (je != hipSuccess) h->jit_log += std::string("\n[compile/load: ") + hipGetErrorString(je) + "]";
+    if (je == hipSuccess) {
+      bool ok = true;
+      for (int k = 0; k < 2; ++k)
+        for (int p = 0; p < 2; ++p)
+          ok = ok && hipModuleGetFunction(&hs.jit_fn[k][p], mod, jit_kernel_name(k, p != 0)) == hipSuccess;
+      if (ok) {
+        hs.jit_mod = mod;
+        hs.jit_state = 1;
+      } else {
+        h->jit_log += "\n[kernel lookup failed]";
+        hipModuleUnload(mod);
+      }
+    }
+  }
+  return hs.jit_state == 1 ? hs.jit_fn[kind ? 1 : 0][packed ? 1 : 0] : nullptr;
+}
 
 static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
   std::vector<int> cnt(h->n_chains);
@@ -1047,17 +1115,32 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.global_swaps = h->global_swaps;
   P.la.mlp = h->d_mlp;
   P.la.rounds_per_group = h->rounds_per_group;
+  // specialised binary kernel when every chain uses the same supercell (its
+  // dynamic LDS stays within the default 64 KiB launch limit)
+  P.jit = nullptr;
+  if (h->specialize && h->binary && !P.learned && lds <= 65536) {
+    bool one_shape = true;
+    for (int c = 1; c < h->n_chains; ++c) one_shape = one_shape && h->chain_sc[c] == h->chain_sc[0];
+    if (one_shape) P.jit = jit_kernel(h, h->chain_sc[0], kind, packed);
+  }
   return CLEXMC_OK;
 }
 
 // attribute setting (not a stream operation) and the launch, separately, so
 // that a launch can be recorded into a graph
 static hipError_t plan_attributes(const RunPlan &P, clexmc_handle *h) {
+  if (P.jit) return hipSuccess;
   if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, false);
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, false);
 }
 
 static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h) {
+  if (P.jit) {
+    RunArgs args = P.a;
+    void *params[] = {(void *)&args};
+    return hipModuleLaunchKernel(P.jit, (unsigned)h->n_chains, 1, 1, (unsigned)P.threads, 1, 1, (unsigned)P.lds,
+                                 h->stream, params, nullptr);
+  }
   if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, true);
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, true);
 }
@@ -1073,6 +1156,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, P);
   if (st != CLEXMC_OK) return st;
   h->last_packed = P.packed;
+  h->last_specialized = P.jit ? 1 : 0;
+  h->last_threads = P.threads;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   HIP_TRY(plan_launch(P, h));
@@ -1617,7 +1702,10 @@ void clexmc_destroy(clexmc_handle *h) {
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
   free_chains(h);
-  for (auto &s : h->scs) free_all(s.allocs);
+  for (auto &s : h->scs) {
+    free_all(s.allocs);
+    if (s.jit_mod) hipModuleUnload(s.jit_mod);
+  }
   free_all(h->sys_allocs);
   if (h->d_scs) hipFree(h->d_scs);
   if (h->ev0) hipEventDestroy(h->ev0);

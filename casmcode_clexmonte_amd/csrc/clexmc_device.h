@@ -14,7 +14,9 @@
 // Read-only tables are read through the constant address space so that the
 // uniform per-cluster offsets become scalar (SMEM) loads.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace clexmc {

@@ -1,0 +1,100 @@
+// Run-time specialisation of the binary Metropolis kernel (see clexmc_jit.h).
+#include "clexmc_jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <sstream>
+
+#include "clexmc_jit_src.inc"  // kJitHeaderNames / kJitHeaderTexts: the device headers as text
+
+namespace clexmc {
+
+static std::string hexf(float v) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%af", (double)v);
+  return buf;
+}
+
+std::string jit_source(int n_sublat, const std::vector<int> &variable, const std::vector<BinSub> &subs,
+                       const std::vector<int> &bin_off, int n_slots) {
+  std::ostringstream o;
+  o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\nstruct JitField {\n"
+    << "  template <class IMG>\n"
+    << "  __device__ __forceinline__ static float bin(const FieldCtx &, const IMG &img, int b, int pa, int o,\n"
+    << "                                              int (&K)[kMaxSlots]) {\n"
+    << "    (void)o;\n";
+  for (int j = 0; j < kMaxSlots; ++j) o << "    K[" << j << "] = 0;\n";
+  o << "    float F = 0.0f;\n";
+  for (int b = 0; b < n_sublat; ++b) {
+    if (!variable[b]) continue;
+    const BinSub &B = subs[b];
+    int minoff = 0;
+    for (int j = 0; j < n_slots; ++j) {
+      const int n = B.slot_n[j], ko = B.slot_k[j];
+      for (int e = 0; e < n * ko; ++e) minoff = std::min(minoff, bin_off[B.slot_begin[j] + e]);
+    }
+    // base + (off - minoff) == pa + off, with every immediate offset >= 0
+    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    for (int j = 0; j < n_slots; ++j) {
+      const int n = B.slot_n[j], ko = B.slot_k[j];
+      if (n <= 0) continue;
+      o << "      {\n        int X = 0;\n";
+      for (int e = 0; e < n && ko > 0; ++e) {
+        o << "        X += ";
+        for (int i = 0; i < ko; ++i) {
+          if (i) o << " ^ ";
+          o << "img.get(base + " << (bin_off[B.slot_begin[j] + e * ko + i] - minoff) << ")";
+        }
+        o << ";\n";
+      }
+      if (ko & 1)
+        o << "        const int Kj = 2 * X - " << n << ";\n";
+      else
+        o << "        const int Kj = " << n << " - 2 * X;\n";
+      o << "        K[" << j << "] = Kj;\n        F = __builtin_fmaf(" << hexf(B.slot_w[j]) << ", (float)Kj, F);\n"
+        << "      }\n";
+    }
+    o << "      return F;\n    }\n";
+  }
+  o << "    return F;\n  }\n};\n}  // namespace clexmc\n";
+  for (int kind = 0; kind < 2; ++kind)
+    for (int pk = 0; pk < 2; ++pk)
+      o << "extern \"C\" __global__ __launch_bounds__(512) void " << jit_kernel_name(kind, pk != 0)
+        << "(clexmc::RunArgs a) { clexmc::metropolis_body<true, " << kind << ", " << (pk ? "true" : "false")
+        << ", clexmc::JitField>(a); }\n";
+  return o.str();
+}
+
+const char *jit_kernel_name(int kind, bool packed) {
+  static const char *names[2][2] = {{"clexmc_jit_c_u8", "clexmc_jit_c_p2"}, {"clexmc_jit_s_u8", "clexmc_jit_s_p2"}};
+  return names[kind ? 1 : 0][packed ? 1 : 0];
+}
+
+hipError_t jit_compile_load(const std::string &src, hipModule_t *mod, std::string *log) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "clexmc_jit.hip", kJitNumHeaders, kJitHeaderTexts,
+                          kJitHeaderNames) != HIPRTC_SUCCESS)
+    return hipErrorInvalidValue;
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  if (log && ls > 1) {
+    log->assign(ls, '\0');
+    hiprtcGetProgramLog(prog, &(*log)[0]);
+  }
+  if (rc != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return hipErrorInvalidImage;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> code(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return hipModuleLoadData(mod, code.data());
+}
+
+}  // namespace clexmc

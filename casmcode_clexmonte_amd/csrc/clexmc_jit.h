@@ -1,0 +1,27 @@
+// Run-time specialisation of the binary Metropolis kernel to one supercell
+// shape: the site programs (padded LDS offsets of every other site of every
+// cluster, per orbit) become compile-time constants, so each neighbour read
+// is one LDS load with an immediate offset and no per-read address or
+// scalar-load work.  Compiled with hipRTC for gfx950 on first use.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "clexmc_device.h"
+
+namespace clexmc {
+
+// Source of the specialised kernels for the binary site programs of one
+// supercell (subs[n_sublat], offsets as laid out by the library).
+std::string jit_source(int n_sublat, const std::vector<int> &variable, const std::vector<BinSub> &subs,
+                       const std::vector<int> &bin_off, int n_slots);
+
+// Kernel name for (kind, packed) in a module built from jit_source.
+const char *jit_kernel_name(int kind, bool packed);
+
+// Compiles `src` (hipRTC, gfx950) and loads it as a module.
+hipError_t jit_compile_load(const std::string &src, hipModule_t *mod, std::string *log);
+
+}  // namespace clexmc

@@ -1,7 +1,9 @@
 // Kernel argument blocks and launch wrappers (internal to the library).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#endif
 
 #include "clexmc_device.h"
 
@@ -90,6 +92,7 @@ struct AdamArgs {
   float lr, b1m, b2m, b1t, b2t;  // b1t = b1m^t, b2t = b2m^t
 };
 
+#ifndef __HIPCC_RTC__
 hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
                           int threads, size_t lds, hipStream_t st, bool launch);
 size_t learned_lds_bytes(int pad_bytes);
@@ -107,5 +110,7 @@ hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
 size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed);
+
+#endif  // __HIPCC_RTC__
 
 }  // namespace clexmc

@@ -1,0 +1,654 @@
+// Device code of the persistent Metropolis kernel, shared by the library's
+// ahead-of-time build (generic site programs) and the run-time compiled
+// kernels specialised to one supercell shape (constant site offsets).
+#pragma once
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "clexmc_device.h"
+#include "clexmc_kernels.h"
+
+namespace clexmc {
+
+// ---------------------------------------------------------------------------
+// Per-lane uniform stream: one Philox block yields four uniforms.
+// Counter = (lane, draw_lo, draw_hi, chain); key = chain seed.
+// ---------------------------------------------------------------------------
+struct LaneRng {
+  uint4 buf;
+  int left;
+  unsigned long long draw;
+  uint32_t lane, chain, k0, k1;
+
+  __device__ __forceinline__ float next() {
+    if (left == 0) {
+      buf = philox4x32_10(make_uint4(lane, (uint32_t)draw, (uint32_t)(draw >> 32), chain), k0, k1);
+      ++draw;
+      left = 4;
+    }
+    uint32_t x = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
+    --left;
+    return u01(x);
+  }
+};
+
+// block-uniform random integer in [0, n) for (pass, phase)
+__device__ __forceinline__ uint32_t uniform_int(uint32_t n, unsigned long long pass, uint32_t phase,
+                                                uint32_t chain, uint32_t k0, uint32_t k1) {
+  uint4 r = philox4x32_10(make_uint4(0xFFFFFFFFu, (uint32_t)pass, phase, chain ^ 0x9E3779B9u), k0, k1);
+  unsigned long long x = ((unsigned long long)r.x << 32) | r.y;
+  return n ? (uint32_t)(x % n) : 0u;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// LDS occupation image accessors.  A site is written only by the lane that
+// owns it in the current phase (sites of a class share no cluster), but with
+// packing 16 sites share a word, so packed writes are atomic XORs of the
+// changed bits (LDS atomics); the other sites' bits are untouched.
+// ---------------------------------------------------------------------------
+struct ImgU8 {
+  uint8_t *p;
+  __device__ __forceinline__ int get(int a) const { return p[a]; }
+  __device__ __forceinline__ void set(int a, int old, int nw) const {
+    (void)old;
+    p[a] = (uint8_t)nw;
+  }
+  __device__ __forceinline__ void copy(int dst, int src) const { p[dst] = p[src]; }
+};
+
+struct ImgP2 {
+  uint32_t *w;
+  __device__ __forceinline__ int get(int a) const {
+    return (int)__builtin_amdgcn_ubfe(w[a >> 4], (uint32_t)((a & 15) << 1), 2u);
+  }
+  __device__ __forceinline__ void set(int a, int old, int nw) const {
+    atomicXor(&w[a >> 4], (uint32_t)(old ^ nw) << ((a & 15) << 1));
+  }
+  __device__ __forceinline__ void copy(int dst, int src) const {
+    const int v = get(src), o = get(dst);
+    if (v != o) set(dst, o, v);
+  }
+};
+
+// bytes of the LDS image for pad_sites padded sites, a multiple of 16 so that
+// the arrays placed after it keep their natural (up to 16-byte) alignment
+__host__ __device__ inline int image_bytes(int pad_sites, bool packed) {
+  return packed ? ((pad_sites + 63) / 64) * 16 : ((pad_sites + 15) & ~15);
+}
+
+// ---------------------------------------------------------------------------
+// Binary path: K_a(s) = sum over clusters of orbit a containing s of
+// prod phi(other sites) = (-1)^(k-1) * (n_a - 2 * sum XOR(other bits)).
+// The site program lists the padded offsets of the other sites per orbit,
+// padded to multiples of 4 clusters with offset-0 entries (the site itself):
+// an even number of copies of the site XORs to 0, an odd number contributes
+// the site's own bit, removed below.
+// ---------------------------------------------------------------------------
+template <class IMG>
+__device__ __forceinline__ float binary_field(const IMG &img, int pa, int o, cptr<BinSub> bs, cptr<int> off,
+                                              int n_slots, int (&K)[kMaxSlots]) {
+  float F = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kMaxSlots; ++j) {
+    K[j] = 0;
+    if (j < n_slots) {
+      const int n = bs->slot_n[j];
+      if (n > 0) {
+        const int ko = bs->slot_k[j];
+        const int q0 = bs->slot_begin[j];
+        const int len = bs->slot_len[j];
+        int X = 0;
+        if (ko == 1) {
+#pragma unroll 2
+          for (int e = 0; e < len; e += 4) {
+            const int q = q0 + e;
+            const int b0 = img.get(pa + off[q]);
+            const int b1 = img.get(pa + off[q + 1]);
+            const int b2 = img.get(pa + off[q + 2]);
+            const int b3 = img.get(pa + off[q + 3]);
+            X += (b0 + b1) + (b2 + b3);
+          }
+        } else if (ko == 2) {
+#pragma unroll 2
+          for (int e = 0; e < len; e += 4) {
+            const int q = q0 + 2 * e;
+            const int b0 = img.get(pa + off[q]);
+            const int b1 = img.get(pa + off[q + 1]);
+            const int b2 = img.get(pa + off[q + 2]);
+            const int b3 = img.get(pa + off[q + 3]);
+            const int b4 = img.get(pa + off[q + 4]);
+            const int b5 = img.get(pa + off[q + 5]);
+            const int b6 = img.get(pa + off[q + 6]);
+            const int b7 = img.get(pa + off[q + 7]);
+            X += ((b0 ^ b1) + (b2 ^ b3)) + ((b4 ^ b5) + (b6 ^ b7));
+          }
+        } else if (ko >= 3) {
+          for (int e = 0; e < len; ++e) {
+            int x = 0;
+            for (int q = q0 + e * ko; q < q0 + (e + 1) * ko; ++q) x ^= img.get(pa + off[q]);
+            X += x;
+          }
+        }
+        if (ko & 1) X -= bs->slot_dummy[j] * o;
+        const int Kj = (ko & 1) ? (2 * X - n) : (n - 2 * X);
+        K[j] = Kj;
+        F = __builtin_fmaf(bs->slot_w[j], (float)Kj, F);
+      }
+    }
+  }
+  return F;
+}
+
+// General path: G_f(s) = sum_e V_e[tau_others][f] from the flower tables.
+template <class IMG>
+__device__ __forceinline__ void general_field(const IMG &img, int pa, cptr<GenSub> gs, cptr<int> prog,
+                                              const float *tab, float (&G)[kMaxFunc]) {
+  G[0] = G[1] = G[2] = 0.0f;
+  const int nf = gs->nf;
+  const int n = gs->n_entries;
+  int q = gs->begin;
+  for (int e = 0; e < n; ++e) {
+    const int k = prog[q];
+    const int toff = prog[q + 1];
+    q += 2;
+    int idx = 0;
+    for (int i = 0; i < k; ++i) {
+      idx += img.get(pa + prog[q]) * prog[q + 1];
+      q += 2;
+    }
+    const float *V = tab + toff + idx * nf;
+    G[0] += V[0];
+    if (nf > 1) G[1] += V[1];
+    if (nf > 2) G[2] += V[2];
+  }
+}
+
+__device__ __forceinline__ float general_dE(cptr<DevSystem> sys, int b, int o, int nw, const float (&G)[kMaxFunc]) {
+  const int nf = sys->n_func[b];
+  float d = (sys->site_func[b][0][nw] - sys->site_func[b][0][o]) * G[0];
+  if (nf > 1) d += (sys->site_func[b][1][nw] - sys->site_func[b][1][o]) * G[1];
+  if (nf > 2) d += (sys->site_func[b][2][nw] - sys->site_func[b][2][o]) * G[2];
+  return d;
+}
+
+__device__ __forceinline__ bool metropolis_accept(float dE, float beta, float u) {
+  return dE <= 0.0f || u < __expf(-beta * dE);
+}
+
+// Binary field policy.  FieldCtx carries what the runtime site programs
+// need; GenericField interprets them, a specialised policy (compiled at run
+// time for one supercell shape) has the offsets built in and ignores it.
+struct FieldCtx {
+  cptr<DevSupercell> sc;
+  cptr<int> bin_off;
+  int n_slots;
+};
+
+struct GenericField {
+  template <class IMG>
+  __device__ __forceinline__ static float bin(const FieldCtx &c, const IMG &img, int b, int pa, int o,
+                                              int (&K)[kMaxSlots]) {
+    return binary_field(img, pa, o, cst(c.sc->bin_sub + b), c.bin_off, c.n_slots, K);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Persistent Metropolis kernel
+// ---------------------------------------------------------------------------
+template <bool BIN, int KIND, bool PACK, class FIELD>
+__device__ __forceinline__ void metropolis_body(const RunArgs &args) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int chain = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int n_waves = nthr >> 6;
+  cptr<DevSystem> sys = cst(args.sys);
+  cptr<ChainParams> cp = cst(args.params + chain);
+  ChainState *cs = args.state + chain;
+  cptr<DevSupercell> sc = cst(args.scs + cp->sc);
+
+  const int L1 = sc->L1, L2 = sc->L2, L3 = sc->L3, h = sc->h;
+  const int P1 = sc->P1, P12 = sc->P1 * sc->P2, S = sc->S;
+  const int n_cells = sc->n_cells, n_sites = sc->n_sites, pad_bytes = sc->pad_bytes;
+  const int n_classes = sc->n_classes, max_class = sc->max_class;
+  const int n_slots = sys->n_slots;
+
+  // ---- LDS layout ----
+  using Img = typename std::conditional<PACK, ImgP2, ImgU8>::type;
+  Img img;
+  if constexpr (PACK) img.w = (uint32_t *)smem;
+  else img.p = (uint8_t *)smem;
+  size_t off_b = (size_t)image_bytes(pad_bytes, PACK);
+  float *tabL = (float *)(smem + off_b);
+  const int n_tab = BIN ? 0 : args.n_tab;
+  off_b += (size_t)((n_tab * 4 + 15) & ~15);
+  const int n_islot = (BIN ? kMaxSlots : 0) + kMaxOcc + 2;
+  int *redI = (int *)(smem + off_b);                     // [n_islot][n_waves]
+  off_b += (size_t)n_islot * 16 * sizeof(int);
+  double *redD = (double *)(smem + off_b);               // [n_waves]
+  off_b += 16 * sizeof(double);
+  unsigned long long *maxDraw = (unsigned long long *)(smem + off_b);
+  off_b += 16;
+  const int nfs = BIN ? 1 : sys->n_func_max;              // field values kept per class site
+  float *cF = (float *)(smem + off_b);                   // [nfs][max_class]
+  int8_t *cK = (int8_t *)(cF + nfs * max_class);         // [kMaxSlots][max_class] (binary)
+
+  // ---- load the padded image (and general tables) ----
+  const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
+  auto padded_occ = [&](int p) -> int {
+    int b = p / S;
+    int r = p - b * S;
+    int pk = r / P12;
+    int r2 = r - pk * P12;
+    int pj = r2 / P1;
+    int pi = r2 - pj * P1;
+    int i = ((pi - h) % L1 + L1) % L1;
+    int j = ((pj - h) % L2 + L2) % L2;
+    int k = ((pk - h) % L3 + L3) % L3;
+    return occG[b * n_cells + i + L1 * (j + L2 * k)];
+  };
+  if constexpr (PACK) {
+    // each thread assembles whole words (16 padded sites)
+    const int n_words = (pad_bytes + 15) / 16;
+    for (int wd = tid; wd < n_words; wd += nthr) {
+      uint32_t word = 0u;
+      for (int q = 0; q < 16; ++q) {
+        const int p = wd * 16 + q;
+        if (p < pad_bytes) word |= (uint32_t)(padded_occ(p) & 3) << (2 * q);
+      }
+      img.w[wd] = word;
+    }
+  } else {
+    for (int p = tid; p < pad_bytes; p += nthr) img.p[p] = (uint8_t)padded_occ(p);
+  }
+  if (!BIN) {
+    const float *__restrict__ tg = sys->flower_tables;
+    for (int p = tid; p < n_tab; p += nthr) tabL[p] = tg[p];
+  }
+  if (tid == 0) *maxDraw = 0ull;
+  __syncthreads();
+
+  const float beta = cp->beta;
+  LaneRng rng;
+  rng.left = 0;
+  rng.draw = cs->draw_base;
+  rng.lane = (uint32_t)tid;
+  rng.chain = (uint32_t)chain;
+  rng.k0 = cp->key0;
+  rng.k1 = cp->key1;
+
+  // per-lane accumulators
+  int accC[kMaxSlots];
+#pragma unroll
+  for (int a = 0; a < kMaxSlots; ++a) accC[a] = 0;
+  int accN[kMaxOcc] = {0, 0, 0, 0};
+  int accAtt = 0, accAcc = 0;
+  double accE = 0.0;
+
+  const unsigned long long pass0 = cs->pass_base;
+  const int rounds = args.rounds < 1 ? 1 : args.rounds;
+  cptr<int> bin_off = cst(sc->bin_off);
+  cptr<int> gen_prog = cst(sc->gen_prog);
+  const FieldCtx fctx{sc, bin_off, n_slots};
+
+  for (int pass = 0; pass < args.n_passes; ++pass) {
+    const unsigned long long gpass = pass0 + (unsigned long long)pass;
+    for (int c = 0; c < n_classes; ++c) {
+      const int b = sc->class_sublat[c];
+      const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
+      cptr<BinSub> bs = cst(sc->bin_sub + b);
+      cptr<GenSub> gs = cst(sc->gen_sub + b);
+      const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
+      if (KIND == 1) {
+        // ---- semi-grand canonical: every site of the class proposes a change
+        for (int t = cb + tid; t < ce; t += nthr) {
+          const int pa = sc->class_addr[t];
+          const int o = img.get(pa);
+          const float u = rng.next();
+          if (BIN) {
+            int K[kMaxSlots];
+            const float F = FIELD::bin(fctx, img, b, pa, o, K);
+            const float dphi = o ? -2.0f : 2.0f;
+            const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
+            const bool acc = metropolis_accept(dE, beta, u);
+            accAtt += 1;
+            const int sk = acc ? (o ? -2 : 2) : 0;
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+            accN[1] += sk / 2;
+            accN[0] -= sk / 2;
+            accAcc += acc ? 1 : 0;
+            if (acc) img.set(pa, o, 1 - o);
+          } else {
+            float G[kMaxFunc];
+            general_field(img, pa, gs, gen_prog, tabL, G);
+            const int no = sys->n_occ[b];
+            const float u2 = rng.next();
+            int r = (int)(u2 * (float)(no - 1));
+            if (r > no - 2) r = no - 2;
+            const int nw = r >= o ? r + 1 : r;
+            const float dEf = general_dE(sys, b, o, nw, G);
+            const float dE = dEf - (cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o]);
+            const bool acc = metropolis_accept(dE, beta, u);
+            accAtt += 1;
+            if (acc) {
+              img.set(pa, o, nw);
+              accAcc += 1;
+              accE += (double)dEf;
+#pragma unroll
+              for (int q = 0; q < kMaxOcc; ++q) accN[q] += (q == nw) - (q == o);
+            }
+          }
+        }
+        __syncthreads();
+      } else {
+        // ---- canonical: fields of all class sites, then random matchings
+        const int n = ce - cb;
+        for (int t = tid; t < n; t += nthr) {
+          const int pa = sc->class_addr[cb + t];
+          if (BIN) {
+            int K[kMaxSlots];
+            cF[t] = FIELD::bin(fctx, img, b, pa, img.get(pa), K);
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) cK[a * max_class + t] = (int8_t)K[a];
+          } else {
+            float G[kMaxFunc];
+            general_field(img, pa, gs, gen_prog, tabL, G);
+            cF[t] = G[0];
+            if (nfs > 1) cF[max_class + t] = G[1];
+            if (nfs > 2) cF[2 * max_class + t] = G[2];
+          }
+        }
+        __syncthreads();
+        for (int rd = 0; rd < rounds; ++rd) {
+          const uint32_t r = uniform_int((uint32_t)n, gpass, (uint32_t)(c * rounds + rd), (uint32_t)chain,
+                                         cp->key0, cp->key1);
+          for (int p = tid; p < n; p += nthr) {
+            int q = (int)r - p;
+            if (q < 0) q += n;
+            if (p >= q) continue;  // q == p: unpaired; p > q: the partner leads
+            const int pa = sc->class_addr[cb + p], qa = sc->class_addr[cb + q];
+            const int op = img.get(pa), oq = img.get(qa);
+            if (op == oq) continue;
+            const float u = rng.next();
+            float dE;
+            if (BIN) {
+              const float dp = op ? -2.0f : 2.0f;
+              dE = dp * cF[p] - dp * cF[q];
+            } else {
+              float Gp[kMaxFunc] = {cF[p], nfs > 1 ? cF[max_class + p] : 0.f, nfs > 2 ? cF[2 * max_class + p] : 0.f};
+              float Gq[kMaxFunc] = {cF[q], nfs > 1 ? cF[max_class + q] : 0.f, nfs > 2 ? cF[2 * max_class + q] : 0.f};
+              dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
+            }
+            const bool acc = metropolis_accept(dE, beta, u);
+            accAtt += 1;
+            if (acc) {
+              img.set(pa, op, oq);
+              img.set(qa, oq, op);
+              accAcc += 1;
+              if (BIN) {
+                const int sp = op ? -2 : 2;
+#pragma unroll
+                for (int a = 0; a < kMaxSlots; ++a)
+                  accC[a] += sp * ((int)cK[a * max_class + p] - (int)cK[a * max_class + q]);
+              } else {
+                accE += (double)dE;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // ---- halo copies of this class
+      for (int t = sc->halo_begin[c] + tid; t < sc->halo_begin[c + 1]; t += nthr) {
+        const int2 hc = sc->halo[t];
+        img.copy(hc.y, hc.x);
+      }
+      __syncthreads();
+    }
+
+    // ---- canonical cross-class mixing: one randomly chosen pair phase
+    if (KIND == 0 && sc->n_pair_phases > 0) {
+      const uint32_t ph = uniform_int((uint32_t)sc->n_pair_phases, gpass, 0xFFFFFFF0u, (uint32_t)chain,
+                                      cp->key0, cp->key1);
+      const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
+      for (int t = pb + tid; t < pe; t += nthr) {
+        const int2 pq = sc->pair_addr[t];
+        const int bpq = sc->pair_sublat[t];
+        const int bp = bpq & 0xFF, bq = bpq >> 8;
+        const int op = img.get(pq.x), oq = img.get(pq.y);
+        if (op == oq) continue;
+        const float u = rng.next();
+        float dE;
+        int Kp[kMaxSlots], Kq[kMaxSlots];
+        if (BIN) {
+          const float Fp = FIELD::bin(fctx, img, bp, pq.x, op, Kp);
+          const float Fq = FIELD::bin(fctx, img, bq, pq.y, oq, Kq);
+          const float dp = op ? -2.0f : 2.0f;
+          dE = dp * Fp - dp * Fq;
+        } else {
+          float Gp[kMaxFunc], Gq[kMaxFunc];
+          general_field(img, pq.x, cst(sc->gen_sub + bp), gen_prog, tabL, Gp);
+          general_field(img, pq.y, cst(sc->gen_sub + bq), gen_prog, tabL, Gq);
+          dE = general_dE(sys, bp, op, oq, Gp) + general_dE(sys, bq, oq, op, Gq);
+        }
+        const bool acc = metropolis_accept(dE, beta, u);
+        accAtt += 1;
+        if (acc) {
+          img.set(pq.x, op, oq);
+          img.set(pq.y, oq, op);
+          accAcc += 1;
+          if (BIN) {
+            const int sp = op ? -2 : 2;
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sp * (Kp[a] - Kq[a]);
+          } else {
+            accE += (double)dE;
+          }
+        }
+      }
+      __syncthreads();
+      for (int t = sc->pair_halo_begin[ph] + tid; t < sc->pair_halo_begin[ph + 1]; t += nthr) {
+        const int2 hc = sc->pair_halo[t];
+        img.copy(hc.y, hc.x);
+      }
+      __syncthreads();
+    }
+
+    // ---- canonical global swaps: uniformly random site pairs anywhere in the
+    // supercell (the two sites may share clusters), evaluated serially by one
+    // lane as two sequential flips on the image (every periodic copy of the
+    // first site is updated before the second site's field is taken).  These
+    // are the reference's unrestricted pair moves; they make every supercell
+    // ergodic, however small, for the canonical ensemble.
+    if (KIND == 0 && args.global_swaps > 0) {
+      if (tid == 0) {
+        const int n_var = sc->class_begin[n_classes];
+        auto set_all = [&](int a, int old, int nw) {
+          const int b = a / S;
+          const int r = a - b * S;
+          const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
+          const int P2 = P12 / P1, P3 = S / P12;
+          const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
+          for (int dk = -m3; dk <= m3; ++dk) {
+            const int qk = pk + dk * L3;
+            if (qk < 0 || qk >= P3) continue;
+            for (int dj = -m2; dj <= m2; ++dj) {
+              const int qj = pj + dj * L2;
+              if (qj < 0 || qj >= P2) continue;
+              for (int di = -m1; di <= m1; ++di) {
+                const int qi = pi + di * L1;
+                if (qi < 0 || qi >= P1) continue;
+                img.set(b * S + qk * P12 + qj * P1 + qi, old, nw);
+              }
+            }
+          }
+        };
+        for (int g = 0; g < args.global_swaps; ++g) {
+          const float u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
+          const int is = min((int)(u1 * (float)n_var), n_var - 1);
+          const int it = min((int)(u2 * (float)n_var), n_var - 1);
+          const int sa = sc->class_addr[is], ta = sc->class_addr[it];
+          if (sa == ta) continue;
+          const int bsa = sa / S, bta = ta / S;
+          if (sys->n_occ[bsa] != sys->n_occ[bta]) continue;
+          const int os = img.get(sa), ot = img.get(ta);
+          if (os == ot) continue;
+          float dE1, dE2;
+          int Ks[kMaxSlots], Kt[kMaxSlots];
+          if (BIN) {
+            dE1 = (os ? -2.0f : 2.0f) * FIELD::bin(fctx, img, bsa, sa, os, Ks);
+          } else {
+            float G[kMaxFunc];
+            general_field(img, sa, cst(sc->gen_sub + bsa), gen_prog, tabL, G);
+            dE1 = general_dE(sys, bsa, os, ot, G);
+          }
+          set_all(sa, os, ot);
+          if (BIN) {
+            dE2 = (ot ? -2.0f : 2.0f) * FIELD::bin(fctx, img, bta, ta, ot, Kt);
+          } else {
+            float G[kMaxFunc];
+            general_field(img, ta, cst(sc->gen_sub + bta), gen_prog, tabL, G);
+            dE2 = general_dE(sys, bta, ot, os, G);
+          }
+          const float dE = dE1 + dE2;
+          accAtt += 1;
+          if (metropolis_accept(dE, beta, u3)) {
+            set_all(ta, ot, os);
+            accAcc += 1;
+            if (BIN) {
+              const int ss = os ? -2 : 2, st = ot ? -2 : 2;
+#pragma unroll
+              for (int a = 0; a < kMaxSlots; ++a) accC[a] += ss * Ks[a] + st * Kt[a];
+            } else {
+              accE += (double)dE;
+            }
+          } else {
+            set_all(sa, ot, os);
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- reduction of lane accumulators and sampling
+    const bool sample_due = ((gpass + 1) % (unsigned long long)args.sample_period) == 0ull;
+    const bool flush = sample_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);
+    if (flush) {
+      int s = 0;
+      if (BIN) {
+#pragma unroll
+        for (int a = 0; a < kMaxSlots; ++a) {
+          const int v = wave_sum(accC[a]);
+          if (lane == 0) redI[(s + a) * n_waves + wave] = v;
+          accC[a] = 0;
+        }
+        s += kMaxSlots;
+      }
+#pragma unroll
+      for (int o = 0; o < kMaxOcc; ++o) {
+        const int v = wave_sum(accN[o]);
+        if (lane == 0) redI[(s + o) * n_waves + wave] = v;
+        accN[o] = 0;
+      }
+      s += kMaxOcc;
+      {
+        const int v0 = wave_sum(accAtt);
+        const int v1 = wave_sum(accAcc);
+        if (lane == 0) {
+          redI[s * n_waves + wave] = v0;
+          redI[(s + 1) * n_waves + wave] = v1;
+        }
+        accAtt = accAcc = 0;
+      }
+      if (!BIN) {
+        const double v = wave_sum(accE);
+        if (lane == 0) redD[wave] = v;
+        accE = 0.0;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int sumI[kMaxSlots + kMaxOcc + 2];
+        for (int q = 0; q < n_islot; ++q) {
+          int acc = 0;
+          for (int w = 0; w < n_waves; ++w) acc += redI[q * n_waves + w];
+          sumI[q] = acc;
+        }
+        int s2 = 0;
+        double E;
+        if (BIN) {
+          long long *C = args.binC + (long long)chain * args.n_orbits;
+          for (int j = 0; j < n_slots; ++j) C[sys->slot_orbit[j]] += sumI[j];
+          E = 0.0;
+          for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)C[a];
+          s2 = kMaxSlots;
+        } else {
+          double dsum = 0.0;
+          for (int w = 0; w < n_waves; ++w) dsum += redD[w];
+          E = cs->E + dsum;
+        }
+        cs->E = E;
+        for (int o = 0; o < kMaxOcc; ++o) cs->counts[o] += sumI[s2 + o];
+        cs->n_attempt += sumI[s2 + kMaxOcc];
+        cs->n_accept += sumI[s2 + kMaxOcc + 1];
+        if (sample_due) {
+          const int idx = args.sample_count[chain];
+          if (idx < args.sample_cap) {
+            const long long o = (long long)chain * args.sample_cap + idx;
+            args.sampE[o] = E;
+            for (int q = 0; q < kMaxOcc; ++q) args.sampN[o * kMaxOcc + q] = (int)cs->counts[q];
+            args.sample_count[chain] = idx + 1;
+          }
+          const double n1 = (double)cs->counts[1];
+          cs->sumE += E;
+          cs->sumE2 += E * E;
+          cs->sumN1 += n1;
+          cs->sumN12 += n1 * n1;
+          cs->sumEN1 += E * n1;
+          cs->n_samples_acc += 1;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- write back occupation and RNG state
+  {
+    int8_t *occW = args.occ + cp->occ_offset;
+    for (int l = tid; l < n_sites; l += nthr) {
+      int b = l / n_cells;
+      int cc = l - b * n_cells;
+      int i = cc % L1;
+      int j = (cc / L1) % L2;
+      int k = cc / (L1 * L2);
+      occW[l] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
+    }
+  }
+  // all lanes advance by the same bound so the next launch starts fresh
+  atomicMax(maxDraw, rng.draw);
+  __syncthreads();
+  if (tid == 0) {
+    cs->draw_base = *maxDraw + 1ull;
+    cs->pass_base = pass0 + (unsigned long long)args.n_passes;
+  }
+}
+
+}  // namespace clexmc

@@ -73,7 +73,8 @@ def lib():
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
-                     "clexmc_graph_release", "clexmc_autotune"]:
+                     "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
+                     "clexmc_last_run_info", "clexmc_specialization_log"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -92,7 +93,8 @@ def exported_symbols() -> List[str]:
             "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
-            "clexmc_graph_release", "clexmc_autotune"]
+            "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
+            "clexmc_last_run_info", "clexmc_specialization_log"]
 
 
 def _p(a, t):
@@ -479,3 +481,16 @@ class Engine:
                                           C.byref(t), C.byref(p), C.byref(ms)), "clexmc_autotune")
         return {"threads": t.value, "packing": {1: "int8", 2: "packed2"}.get(p.value, "auto"),
                 "ms_per_pass": ms.value}
+
+    def set_specialization(self, on: bool):
+        self._check(lib().clexmc_set_specialization(self.h, C.c_int(1 if on else 0)), "clexmc_set_specialization")
+
+    def last_run_info(self):
+        sp, pk, th = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().clexmc_last_run_info(self.h, C.byref(sp), C.byref(pk), C.byref(th)), "clexmc_last_run_info")
+        return {"specialized": bool(sp.value), "packed": bool(pk.value), "threads": th.value}
+
+    def specialization_log(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        self._check(lib().clexmc_specialization_log(self.h, buf, C.c_int(1 << 16)), "clexmc_specialization_log")
+        return buf.value.decode(errors="replace")

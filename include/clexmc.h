@@ -215,6 +215,20 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
  * supercell ergodic, including ones too small for non-interacting pairs. */
 clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
 
+/* Run-time specialisation (hipRTC, gfx950) of the binary Metropolis kernel
+ * to the loaded supercell: constant site offsets instead of interpreted site
+ * programs; used when every chain shares one supercell.  1 (default): when
+ * possible, 0: never.  Results are bit-identical either way. */
+clexmc_status clexmc_set_specialization(clexmc_handle *h, int mode);
+
+/* Compiler log of the last specialisation attempt (NUL-terminated, at most
+ * cap bytes). */
+clexmc_status clexmc_specialization_log(clexmc_handle *h, char *buf, int cap);
+
+/* What the last clexmc_run_passes launched: specialised kernel (0/1), 2-bit
+ * packed image (0/1), block size. */
+clexmc_status clexmc_last_run_info(clexmc_handle *h, int *specialized, int *packed, int *threads);
+
 /* LDS image format of the Metropolis kernel: 0 automatic (int8 per site when
  * the largest padded supercell fits, else 2 bits per site), 1 int8, 2 packed
  * 2-bit (at most 4 occupants per site). */

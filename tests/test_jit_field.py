@@ -1,0 +1,99 @@
+"""CPU check of the run-time specialised binary field: the straight-line
+code the library generates for a site program (constant offsets) must give
+the same integer cluster-product counts K and the same FP32 field
+F = fma(w_j, K_j, F) as the interpreted site program, on random images.
+
+The generated JitField is compiled for the host with g++ and compared with
+the interpreted semantics (the generic kernel's rule: per orbit slot,
+X = sum over clusters of the XOR of the other sites, K = 2X - n for an odd
+number of other sites, n - 2X for an even number)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from casmcode_clexmonte_amd.system import FlatCE
+from casmcode_clexmonte_amd.workloads import fcc_binary_ce, fcc_ordering_ce
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DUMP = os.path.join(ROOT, "tests", "cpp", "jit_source_dump")
+
+HOST = r'''
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <vector>
+#define __device__
+#define __forceinline__ inline
+constexpr int kMaxSlots = 8;
+struct FieldCtx {};
+struct HostImg { const uint8_t *p; int get(int a) const { return p[a]; } };
+JITFIELD
+int main(int, char **argv) {
+  std::ifstream in(argv[1]); int ns; in >> ns;
+  std::vector<int> ko(ns), n(ns), beg(ns); std::vector<float> w(ns); std::vector<int> off;
+  for (int j = 0; j < ns; ++j) { double ww; in >> ko[j] >> n[j] >> ww; w[j] = (float)ww; beg[j] = (int)off.size();
+    for (int e = 0; e < n[j] * ko[j]; ++e) { int x; in >> x; off.push_back(x); } }
+  const int P = PADDIM, H = HALO, L = P - 2 * H; std::vector<uint8_t> img(P * P * P); srand(7);
+  int bad = 0;
+  for (int trial = 0; trial < 3000; ++trial) {
+    for (auto &v : img) v = (uint8_t)(rand() & 1);
+    const int pa = (H + rand() % L) + P * (H + rand() % L) + P * P * (H + rand() % L);
+    int Kg[8] = {0}; float Fg = 0.f;
+    for (int j = 0; j < ns; ++j) { if (n[j] <= 0) continue; int X = 0;
+      for (int e = 0; e < n[j]; ++e) { int x = 0; for (int i = 0; i < ko[j]; ++i) x ^= img[pa + off[beg[j] + e * ko[j] + i]]; X += x; }
+      const int K = (ko[j] & 1) ? (2 * X - n[j]) : (n[j] - 2 * X); Kg[j] = K; Fg = std::fma(w[j], (float)K, Fg); }
+    int Kj[8]; HostImg im{img.data()}; FieldCtx c;
+    const float Fj = JitField::bin(c, im, 0, pa, img[pa], Kj);
+    for (int j = 0; j < 8; ++j) bad += Kj[j] != Kg[j];
+    bad += Fj != Fg;
+  }
+  printf("%d\n", bad); return 0;
+}
+'''
+
+
+def _program(flat, L):
+    """The library's binary site program for sublattice 0 of an L^3 FCC
+    supercell: per orbit slot (orbits with >= 1 site, in orbit order), the
+    padded offsets of the other sites of each cluster containing the site."""
+    h = int(flat.halo)
+    P = L + 2 * h
+    lines = []
+    slots = [a for a in range(flat.n_orbits) if flat.orbit_k[a] >= 1]
+    w = np.asarray(flat.bin_orbit_weight)
+    for a in slots:
+        ko = int(flat.orbit_k[a]) - 1
+        offs, n = [], 0
+        for e in range(flat.flower_begin[0], flat.flower_begin[1]):
+            if flat.entry_orbit[e] != a:
+                continue
+            n += 1
+            for q in range(flat.entry_site_begin[e], flat.entry_site_begin[e + 1]):
+                b2, di, dj, dk = flat.entry_sites[q]
+                offs.append(int(b2 * P ** 3 + dk * P * P + dj * P + di))
+        lines.append(f"{ko} {n} {float(w[a])!r} " + " ".join(map(str, offs)))
+    return P, h, f"{len(slots)}\n" + "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("ce", [fcc_binary_ce, fcc_ordering_ce])
+def test_generated_field_matches_interpreted_program(ce, tmp_path):
+    if not os.path.exists(DUMP):
+        pytest.skip("tests/cpp/jit_source_dump not built (run __graft_entry__.build())")
+    flat = FlatCE(ce())
+    P, h, prog = _program(flat, 12)
+    pf = tmp_path / "prog.txt"
+    pf.write_text(prog)
+    src = subprocess.run([DUMP, str(pf)], capture_output=True, text=True, check=True).stdout
+    i = src.index("struct JitField {")
+    j = src.index("};\n}  // namespace clexmc")
+    host = HOST.replace("JITFIELD", src[i:j + 2]).replace("PADDIM", str(P)).replace("HALO", str(h))
+    cpp = tmp_path / "check.cpp"
+    cpp.write_text(host)
+    exe = tmp_path / "check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(exe), str(cpp)], check=True, capture_output=True)
+    r = subprocess.run([str(exe), str(pf)], capture_output=True, text=True, check=True)
+    assert int(r.stdout.strip()) == 0
