@@ -144,6 +144,103 @@ def learned_comparison(args):
             **res}
 
 
+def _run_timed(eng, kind, n_passes, n_chains):
+    a0 = sum(eng.read_state(c)["n_attempt"] for c in range(n_chains))
+    t0 = time.perf_counter()
+    kernel_ms = eng.run_passes(kind, n_passes, 1, canonical_rounds=2)
+    wall = time.perf_counter() - t0
+    a1 = sum(eng.read_state(c)["n_attempt"] for c in range(n_chains))
+    return (a1 - a0) / wall, wall, kernel_ms
+
+
+def workload_c1(args):
+    """The reference's CPU-runnable case: FCC 12^3, canonical, x_B = 0.5,
+    T = 600 K, one chain, 1,000 + 5,000 passes, sampled every pass; the CPU
+    oracle (one core) timed on the first 1,000 passes of the same chain."""
+    from casmcode_clexmonte_amd.engine import Engine
+    from casmcode_clexmonte_amd.system import FlatCE, Supercell
+    from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    occ = random_occupation(sc.n_sites, 0.5, 2024)
+    eng = Engine(flat, device=0)
+    sid = eng.add_supercell(sc, n_pair_phases=16, seed=3)
+    eng.load_chains([sid], [occ], [600.0], [11])
+    t0 = time.perf_counter()
+    eng.run_passes("canonical", 1000, 1, canonical_rounds=2)
+    rate, wall, kms = _run_timed(eng, "canonical", 5000, 1)
+    total = time.perf_counter() - t0
+    E, _ = eng.read_samples()
+    e = E[0, 1000:] / sc.n_cells
+    tau = tau_int(e)
+    eng.close()
+    out = {"workload": "fcc_binary_12x12x12_canonical_xB0.5_T600_1chain", "site_attempts_per_s": rate,
+           "passes_per_s": 5000 / wall, "run_s_1000_plus_5000_passes": total,
+           "E_per_cell_mean": float(e.mean()), "E_per_cell_err": float(e.std() * np.sqrt(2 * tau / len(e))),
+           "tau_int_E_passes": tau}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # CPU restatement of the reference loop (timed baseline)
+        osys = O.OracleSystem(flat, sc.dims)
+        t0 = time.perf_counter()
+        r = osys.run(occ, "canonical", 600.0, 0, 1000, 11, full_recompute_samplers=True)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": r["n_attempt"] / el, "unit": "site-attempts/s", "cores": 1, "kind": "port",
+                               "sample": f"1,000 passes of the same chain, {el:.1f} s"}
+        out["speedup_vs_cpu_1core"] = rate / out["cpu_baseline"]["value"]
+    return out
+
+
+def workload_c3(args):
+    """Ternary HCP, 64 canonical chains on uneven supercells 16x16x10 ...
+    48x48x32 (5,120 ... 147,456 sites) in one persistent launch, autotuned."""
+    from casmcode_clexmonte_amd.engine import Engine
+    from casmcode_clexmonte_amd.system import FlatCE, Supercell
+    from casmcode_clexmonte_amd.workloads import hcp_canonical_batch, hcp_ternary_ce
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    b = hcp_canonical_batch(ce.prim)
+    eng = Engine(flat, device=0)
+    ids = {}
+    for d in b.dims:
+        if d not in ids:
+            ids[d] = eng.add_supercell(Supercell(ce.prim, d), n_pair_phases=8, seed=len(ids))
+    n = len(b.dims)
+    eng.load_chains([ids[d] for d in b.dims], b.occ, b.temperatures, b.seeds)
+    tune = eng.autotune("canonical", trial_passes=5)
+    rate, wall, kms = _run_timed(eng, "canonical", args.extra_passes, n)
+    drift = eng.reanchor_energies()
+    sites = sum(len(o) for o in b.occ)
+    eng.close()
+    return {"workload": "hcp_ternary_canonical_64chains_16x16x10-48x48x32", "site_attempts_per_s": rate,
+            "sites_total": sites, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
+            "autotune": tune, "max_energy_drift_eV": drift}
+
+
+def workload_c5(args):
+    """Quaternary BCC 64^3 (262,144 sites), 32 canonical equiatomic chains,
+    T = 300..2000 K, autotuned (2-bit packed LDS image)."""
+    from casmcode_clexmonte_amd.engine import Engine
+    from casmcode_clexmonte_amd.system import FlatCE, Supercell
+    from casmcode_clexmonte_amd.workloads import bcc_equiatomic_batch, bcc_quaternary_ce
+    ce = bcc_quaternary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (64, 64, 64))
+    b = bcc_equiatomic_batch(sc)
+    eng = Engine(flat, device=0)
+    sid = eng.add_supercell(sc, n_pair_phases=8, seed=5)
+    n = len(b.occ)
+    eng.load_chains([sid] * n, b.occ, b.temperatures, b.seeds)
+    tune = eng.autotune("canonical", trial_passes=3)
+    rate, wall, kms = _run_timed(eng, "canonical", args.extra_passes, n)
+    drift = eng.reanchor_energies()
+    eng.close()
+    return {"workload": "bcc_quaternary_64x64x64_canonical_equiatomic_32chains_T300-2000",
+            "site_attempts_per_s": rate, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
+            "autotune": tune, "max_energy_drift_eV": drift}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +251,8 @@ def main():
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-passes", type=int, default=40)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal ESS comparison")
+    ap.add_argument("--no-extra", action="store_true", help="skip the single-chain, HCP and BCC workloads")
+    ap.add_argument("--extra-passes", type=int, default=200)
     ap.add_argument("--c4-size", type=int, default=32)
     ap.add_argument("--c4-chains", type=int, default=128)
     ap.add_argument("--c4-equil", type=int, default=1000)
@@ -241,6 +340,9 @@ def main():
     eng.close()
     if not args.no_learned:
         out["learned_vs_plain"] = learned_comparison(args)
+    if not args.no_extra:
+        out["workloads"] = {"single_chain": workload_c1(args), "hcp_ternary_uneven": workload_c3(args),
+                            "bcc_quaternary_64": workload_c5(args)}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_rate, r = cpu_baseline(flat, sc, batch, args.cpu_chains, args.cpu_passes, threads)
