@@ -31,8 +31,11 @@ struct HostSupercell {
   // binary site programs (host copies, for run-time specialisation)
   std::vector<BinSub> bin_sub;
   std::vector<int> bin_off;
+  std::vector<GenSub> gen_sub;
+  std::vector<int> gen_prog;
   // specialised kernels: 0 not tried, 1 loaded, -1 unavailable
   int jit_state = 0;
+  size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
   hipModule_t jit_mod = nullptr;
   hipFunction_t jit_fn[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
@@ -717,6 +720,8 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, bin_off.data(), bin_off.size(), (int **)&d.bin_off));
   hs.bin_sub = bin_sub;
   hs.bin_off = bin_off;
+  hs.gen_sub = gen_sub;
+  hs.gen_prog = gen_prog;
   HIP_TRY(upload(h, A, gen_sub.data(), gen_sub.size(), (GenSub **)&d.gen_sub));
   HIP_TRY(upload(h, A, gen_prog.data(), gen_prog.size(), (int **)&d.gen_prog));
   HIP_TRY(upload(h, A, env_off.data(), env_off.size(), (int **)&d.env_off));
@@ -1005,7 +1010,8 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
     hs.jit_state = -1;
     std::vector<int> variable(h->n_sublat);
     for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
-    const std::string src = jit_source(h->n_sublat, variable, hs.bin_sub, hs.bin_off, h->n_slots);
+    const std::string src = jit_source(h->n_sublat, variable, h->binary != 0, hs.bin_sub, hs.bin_off, h->n_slots,
+                                       hs.gen_sub, hs.gen_prog);
     hipModule_t mod = nullptr;
     const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
     if (je != hipSuccess) h->jit_log += std::string("\n[compile/load: ") + hipGetErrorString(je) + "]";
@@ -1115,13 +1121,13 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.global_swaps = h->global_swaps;
   P.la.mlp = h->d_mlp;
   P.la.rounds_per_group = h->rounds_per_group;
-  // specialised binary kernel when every chain uses the same supercell (its
-  // dynamic LDS stays within the default 64 KiB launch limit)
+  // specialised kernel when every chain uses the same supercell
   P.jit = nullptr;
-  if (h->specialize && h->binary && !P.learned && lds <= 65536) {
+  if (h->specialize && !P.learned) {
     bool one_shape = true;
     for (int c = 1; c < h->n_chains; ++c) one_shape = one_shape && h->chain_sc[c] == h->chain_sc[0];
-    if (one_shape) P.jit = jit_kernel(h, h->chain_sc[0], kind, packed);
+    const size_t fail = h->scs[h->chain_sc[0]].jit_lds_fail;
+    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[0], kind, packed);
   }
   return CLEXMC_OK;
 }
@@ -1160,7 +1166,18 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   h->last_threads = P.threads;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(plan_launch(P, h));
+  hipError_t le = plan_launch(P, h);
+  if (le != hipSuccess && P.jit) {
+    // the runtime refused the specialised launch (dynamic LDS of a module
+    // kernel above its limit): remember the size, run the generic kernel
+    (void)hipGetLastError();
+    h->scs[h->chain_sc[0]].jit_lds_fail = P.lds;
+    P.jit = nullptr;
+    h->last_specialized = 0;
+    HIP_TRY(plan_attributes(P, h));
+    le = plan_launch(P, h);
+  }
+  HIP_TRY(le);
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -1198,11 +1215,23 @@ clexmc_status clexmc_graph_capture(clexmc_handle *h, int kind, int passes_per_re
   if (st != CLEXMC_OK) return st;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = plan_launch(P, h);
+  hipError_t e = hipSuccess;
   hipGraph_t g = nullptr;
-  hipError_t e2 = hipStreamEndCapture(h->stream, &g);
-  if (e == hipSuccess) e = e2;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    e = plan_launch(P, h);
+    g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess || !P.jit) break;
+    // specialised launch refused: capture the generic kernel instead
+    (void)hipGetLastError();
+    if (g) hipGraphDestroy(g);
+    g = nullptr;
+    h->scs[h->chain_sc[0]].jit_lds_fail = P.lds;
+    P.jit = nullptr;
+    HIP_TRY(plan_attributes(P, h));
+  }
   if (e != hipSuccess) {
     if (g) hipGraphDestroy(g);
     h->err = std::string("graph capture: ") + hipGetErrorString(e);

@@ -16,17 +16,18 @@ static std::string hexf(float v) {
   return buf;
 }
 
-std::string jit_source(int n_sublat, const std::vector<int> &variable, const std::vector<BinSub> &subs,
-                       const std::vector<int> &bin_off, int n_slots) {
+std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
+                       const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
+                       const std::vector<int> &gen_prog) {
   std::ostringstream o;
   o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\nstruct JitField {\n"
     << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static float bin(const FieldCtx &, const IMG &img, int b, int pa, int o,\n"
     << "                                              int (&K)[kMaxSlots]) {\n"
-    << "    (void)o;\n";
+    << "    (void)img; (void)b; (void)pa; (void)o;\n";
   for (int j = 0; j < kMaxSlots; ++j) o << "    K[" << j << "] = 0;\n";
   o << "    float F = 0.0f;\n";
-  for (int b = 0; b < n_sublat; ++b) {
+  for (int b = 0; binary && b < n_sublat; ++b) {
     if (!variable[b]) continue;
     const BinSub &B = subs[b];
     int minoff = 0;
@@ -57,12 +58,45 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, const std
     }
     o << "      return F;\n    }\n";
   }
-  o << "    return F;\n  }\n};\n}  // namespace clexmc\n";
+  o << "    return F;\n  }\n";
+  // general table path: same entry order and accumulation as general_field
+  o << "  template <class IMG>\n"
+    << "  __device__ __forceinline__ static void gen(const FieldCtx &, const IMG &img, int b, int pa, const float *tab,\n"
+    << "                                             float (&G)[kMaxFunc]) {\n"
+    << "    (void)img; (void)b; (void)pa; (void)tab;\n"
+    << "    G[0] = G[1] = G[2] = 0.0f;\n";
+  for (int b = 0; !binary && b < n_sublat; ++b) {
+    if (!variable[b]) continue;
+    const GenSub &S = gsubs[b];
+    int minoff = 0;
+    {
+      int q = S.begin;
+      for (int e = 0; e < S.n_entries; ++e) {
+        const int k = gen_prog[q];
+        q += 2;
+        for (int i = 0; i < k; ++i, q += 2) minoff = std::min(minoff, gen_prog[q]);
+      }
+    }
+    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    int q = S.begin;
+    for (int e = 0; e < S.n_entries; ++e) {
+      const int k = gen_prog[q], toff = gen_prog[q + 1];
+      q += 2;
+      o << "      {\n        const int idx = 0";
+      for (int i = 0; i < k; ++i, q += 2)
+        o << " + img.get(base + " << (gen_prog[q] - minoff) << ") * " << gen_prog[q + 1];
+      o << ";\n        const float *V = tab + " << toff << " + idx * " << S.nf << ";\n";
+      for (int f = 0; f < S.nf && f < kMaxFunc; ++f) o << "        G[" << f << "] += V[" << f << "];\n";
+      o << "      }\n";
+    }
+    o << "      return;\n    }\n";
+  }
+  o << "  }\n};\n}  // namespace clexmc\n";
   for (int kind = 0; kind < 2; ++kind)
     for (int pk = 0; pk < 2; ++pk)
       o << "extern \"C\" __global__ __launch_bounds__(512) void " << jit_kernel_name(kind, pk != 0)
-        << "(clexmc::RunArgs a) { clexmc::metropolis_body<true, " << kind << ", " << (pk ? "true" : "false")
-        << ", clexmc::JitField>(a); }\n";
+        << "(clexmc::RunArgs a) { clexmc::metropolis_body<" << (binary ? "true" : "false") << ", " << kind << ", "
+        << (pk ? "true" : "false") << ", clexmc::JitField>(a); }\n";
   return o.str();
 }
 

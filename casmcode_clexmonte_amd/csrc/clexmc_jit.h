@@ -13,10 +13,13 @@
 
 namespace clexmc {
 
-// Source of the specialised kernels for the binary site programs of one
-// supercell (subs[n_sublat], offsets as laid out by the library).
-std::string jit_source(int n_sublat, const std::vector<int> &variable, const std::vector<BinSub> &subs,
-                       const std::vector<int> &bin_off, int n_slots);
+// Source of the specialised kernels for the site programs of one supercell:
+// the binary programs (subs[n_sublat], offsets as laid out by the library)
+// when `binary`, else the general table programs (gsubs[n_sublat], gen_prog:
+// per cluster k, table offset, then k (offset, radix multiplier) pairs).
+std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
+                       const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
+                       const std::vector<int> &gen_prog);
 
 // Kernel name for (kind, packed) in a module built from jit_source.
 const char *jit_kernel_name(int kind, bool packed);

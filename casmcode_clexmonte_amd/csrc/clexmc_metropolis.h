@@ -197,6 +197,7 @@ __device__ __forceinline__ bool metropolis_accept(float dE, float beta, float u)
 struct FieldCtx {
   cptr<DevSupercell> sc;
   cptr<int> bin_off;
+  cptr<int> gen_prog;
   int n_slots;
 };
 
@@ -205,6 +206,11 @@ struct GenericField {
   __device__ __forceinline__ static float bin(const FieldCtx &c, const IMG &img, int b, int pa, int o,
                                               int (&K)[kMaxSlots]) {
     return binary_field(img, pa, o, cst(c.sc->bin_sub + b), c.bin_off, c.n_slots, K);
+  }
+  template <class IMG>
+  __device__ __forceinline__ static void gen(const FieldCtx &c, const IMG &img, int b, int pa, const float *tab,
+                                             float (&G)[kMaxFunc]) {
+    general_field(img, pa, cst(c.sc->gen_sub + b), c.gen_prog, tab, G);
   }
 };
 
@@ -307,7 +313,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   const int rounds = args.rounds < 1 ? 1 : args.rounds;
   cptr<int> bin_off = cst(sc->bin_off);
   cptr<int> gen_prog = cst(sc->gen_prog);
-  const FieldCtx fctx{sc, bin_off, n_slots};
+  const FieldCtx fctx{sc, bin_off, gen_prog, n_slots};
 
   for (int pass = 0; pass < args.n_passes; ++pass) {
     const unsigned long long gpass = pass0 + (unsigned long long)pass;
@@ -339,7 +345,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             if (acc) img.set(pa, o, 1 - o);
           } else {
             float G[kMaxFunc];
-            general_field(img, pa, gs, gen_prog, tabL, G);
+            FIELD::gen(fctx, img, b, pa, tabL, G);
             const int no = sys->n_occ[b];
             const float u2 = rng.next();
             int r = (int)(u2 * (float)(no - 1));
@@ -371,7 +377,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             for (int a = 0; a < kMaxSlots; ++a) cK[a * max_class + t] = (int8_t)K[a];
           } else {
             float G[kMaxFunc];
-            general_field(img, pa, gs, gen_prog, tabL, G);
+            FIELD::gen(fctx, img, b, pa, tabL, G);
             cF[t] = G[0];
             if (nfs > 1) cF[max_class + t] = G[1];
             if (nfs > 2) cF[2 * max_class + t] = G[2];
@@ -446,8 +452,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           dE = dp * Fp - dp * Fq;
         } else {
           float Gp[kMaxFunc], Gq[kMaxFunc];
-          general_field(img, pq.x, cst(sc->gen_sub + bp), gen_prog, tabL, Gp);
-          general_field(img, pq.y, cst(sc->gen_sub + bq), gen_prog, tabL, Gq);
+          FIELD::gen(fctx, img, bp, pq.x, tabL, Gp);
+          FIELD::gen(fctx, img, bq, pq.y, tabL, Gq);
           dE = general_dE(sys, bp, op, oq, Gp) + general_dE(sys, bq, oq, op, Gq);
         }
         const bool acc = metropolis_accept(dE, beta, u);
@@ -518,7 +524,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             dE1 = (os ? -2.0f : 2.0f) * FIELD::bin(fctx, img, bsa, sa, os, Ks);
           } else {
             float G[kMaxFunc];
-            general_field(img, sa, cst(sc->gen_sub + bsa), gen_prog, tabL, G);
+            FIELD::gen(fctx, img, bsa, sa, tabL, G);
             dE1 = general_dE(sys, bsa, os, ot, G);
           }
           set_all(sa, os, ot);
@@ -526,7 +532,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             dE2 = (ot ? -2.0f : 2.0f) * FIELD::bin(fctx, img, bta, ta, ot, Kt);
           } else {
             float G[kMaxFunc];
-            general_field(img, ta, cst(sc->gen_sub + bta), gen_prog, tabL, G);
+            FIELD::gen(fctx, img, bta, ta, tabL, G);
             dE2 = general_dE(sys, bta, ot, os, G);
           }
           const float dE = dE1 + dE2;

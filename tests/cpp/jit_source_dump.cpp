@@ -1,34 +1,55 @@
-// Reads a binary site program (text) and prints the run-time specialised
-// kernel source the library generates for it (test helper).
-// Input: n_slots, then per slot: k_others n_clusters weight offsets...
-#include <cstdio>
+// Reads a site program (text) and prints the run-time specialised kernel
+// source the library generates for it (test helper).
+//   binary:  "bin" n_slots, then per slot: k_others n_clusters weight offsets...
+//   general: "gen" nf n_entries, then per entry: k table_offset (offset radix)*k
 #include <fstream>
 #include <iostream>
+#include <string>
 
 #include "../../casmcode_clexmonte_amd/csrc/clexmc_jit.h"
 
 int main(int argc, char **argv) {
   if (argc < 2) return 2;
   std::ifstream in(argv[1]);
-  int ns = 0;
-  in >> ns;
+  std::string mode;
+  in >> mode;
   std::vector<clexmc::BinSub> subs(1);
-  clexmc::BinSub &B = subs[0];
-  std::vector<int> off;
-  for (int j = 0; j < ns; ++j) {
-    int ko = 0, n = 0;
-    double w = 0.0;
-    in >> ko >> n >> w;
-    B.slot_k[j] = ko;
-    B.slot_n[j] = n;
-    B.slot_begin[j] = (int)off.size();
-    B.slot_w[j] = (float)w;
-    for (int e = 0; e < n * ko; ++e) {
-      int x = 0;
-      in >> x;
-      off.push_back(x);
+  std::vector<clexmc::GenSub> gsubs(1);
+  std::vector<int> off, prog;
+  int ns = 0;
+  if (mode == "bin") {
+    in >> ns;
+    clexmc::BinSub &B = subs[0];
+    for (int j = 0; j < ns; ++j) {
+      int ko = 0, n = 0;
+      double w = 0.0;
+      in >> ko >> n >> w;
+      B.slot_k[j] = ko;
+      B.slot_n[j] = n;
+      B.slot_begin[j] = (int)off.size();
+      B.slot_w[j] = (float)w;
+      for (int e = 0; e < n * ko; ++e) {
+        int x = 0;
+        in >> x;
+        off.push_back(x);
+      }
+    }
+  } else {
+    clexmc::GenSub &G = gsubs[0];
+    in >> G.nf >> G.n_entries;
+    G.begin = 0;
+    for (int e = 0; e < G.n_entries; ++e) {
+      int k = 0, toff = 0;
+      in >> k >> toff;
+      prog.push_back(k);
+      prog.push_back(toff);
+      for (int i = 0; i < 2 * k; ++i) {
+        int x = 0;
+        in >> x;
+        prog.push_back(x);
+      }
     }
   }
-  std::cout << clexmc::jit_source(1, {1}, subs, off, ns);
+  std::cout << clexmc::jit_source(1, {1}, mode == "bin", subs, off, ns, gsubs, prog);
   return 0;
 }

@@ -130,3 +130,34 @@ def test_specialized_kernel_bit_identical_to_generic(kind, packing):
     for a, b in zip(c0, c1):
         assert np.array_equal(a, b)
     assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
+
+
+@pytest.mark.parametrize("kind", ["semigrand_canonical", "canonical"])
+@pytest.mark.parametrize("packing", ["int8", "packed2"])
+def test_specialized_general_kernel_bit_identical_to_generic(kind, packing):
+    """Same for the table path: ternary HCP (two sublattices)."""
+    _print_case(f"specialised vs generic general {kind} {packing}")
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (8, 8, 6))
+    rng = np.random.default_rng(4)
+    if kind == "canonical":
+        occs = [random_occupation_counts(composition_counts(sc.n_sites, (0.4, 0.3, 0.3)), 60 + c) for c in range(4)]
+    else:
+        occs = [rng.integers(0, 3, sc.n_sites).astype(np.int8) for _ in range(4)]
+    chem = [[0.0, 0.02 * c, -0.01 * c] * 2 for c in range(4)]
+    temps = np.linspace(600, 1800, 4)
+    runs = []
+    for spec in (False, True):
+        eng = _engine(flat, sc, occs, temps, chem, 4)
+        eng.set_specialization(spec)
+        eng.set_image_packing(packing)
+        eng.run_passes(kind, 8, 1, canonical_rounds=2)
+        info = eng.last_run_info()
+        assert info["specialized"] == spec, (info, eng.specialization_log()[-3000:])
+        E, cnt = eng.read_samples()
+        runs.append((_states(eng, len(occs)), E, cnt))
+        eng.close()
+    (s0, E0, n0), (s1, E1, n1) = runs
+    _same(s0, s1)
+    assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
