@@ -66,6 +66,12 @@ struct clexmc_handle {
   // chains
   int n_chains = 0;
   std::vector<int> chain_sc, chain_off;
+  // chains grouped by supercell (one launch per group when shapes differ)
+  std::vector<std::vector<int>> groups;
+  std::vector<int> group_off;
+  int *d_group_map = nullptr;
+  std::vector<hipStream_t> side;    // side streams of the grouped launches (default priority)
+  std::vector<hipEvent_t> side_ev;
   long long occ_total = 0;
   ChainParams *d_params = nullptr;
   ChainState *d_state = nullptr;
@@ -755,6 +761,10 @@ static void graph_release(clexmc_handle *h);
 
 static void free_chains(clexmc_handle *h) {
   graph_release(h);
+  hipFree(h->d_group_map);
+  h->d_group_map = nullptr;
+  h->groups.clear();
+  h->group_off.clear();
   hipFree(h->d_params);
   hipFree(h->d_state);
   hipFree(h->d_occ);
@@ -854,6 +864,25 @@ clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supe
   }
   HIP_TRY(hipMemcpy(h->d_binC, C.data(), sizeof(long long) * C.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->d_state, state.data(), sizeof(ChainState) * n_chains, hipMemcpyHostToDevice));
+  // chains grouped by supercell, in order of first appearance
+  {
+    std::vector<int> gid(h->scs.size(), -1);
+    for (int c = 0; c < n_chains; ++c) {
+      int &g = gid[h->chain_sc[c]];
+      if (g < 0) {
+        g = (int)h->groups.size();
+        h->groups.emplace_back();
+      }
+      h->groups[g].push_back(c);
+    }
+    std::vector<int> flat;
+    for (auto &g : h->groups) {
+      h->group_off.push_back((int)flat.size());
+      flat.insert(flat.end(), g.begin(), g.end());
+    }
+    HIP_TRY(hipMalloc(&h->d_group_map, sizeof(int) * flat.size()));
+    HIP_TRY(hipMemcpy(h->d_group_map, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+  }
   return CLEXMC_OK;
 }
 
@@ -997,7 +1026,9 @@ struct RunPlan {
   int kind = 0, threads = 64, weights = 0;
   bool learned = false, packed = false;
   size_t lds = 0;
-  hipFunction_t jit = nullptr;  // specialised kernel (binary path, one supercell shape)
+  hipFunction_t jit = nullptr;  // specialised kernel (one supercell shape)
+  int n_blocks = 0;             // workgroups (chains) of this launch
+  int sc_id = 0;                // supercell of a single-shape launch
   RunArgs a;
   LearnArgs la;
 };
@@ -1060,10 +1091,14 @@ static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
 }
 
 // threads_override / packing_override: 0 = the handle's setting
+// subset: chain indices of one supercell group (nullptr: all chains)
 static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
-                              int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0) {
+                              int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0,
+                              const std::vector<int> *subset = nullptr) {
   int max_pad = 0, max_class = 1;
-  for (int c = 0; c < h->n_chains; ++c) {
+  const int n_sel = subset ? (int)subset->size() : h->n_chains;
+  for (int i = 0; i < n_sel; ++i) {
+    const int c = subset ? (*subset)[i] : i;
     const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
     max_pad = std::max(max_pad, d.pad_bytes);
     max_class = std::max(max_class, d.max_class);
@@ -1119,15 +1154,19 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.n_orbits = h->n_orbits;
   a.n_tab = n_tab;
   a.global_swaps = h->global_swaps;
+  a.chain_map = nullptr;
+  P.n_blocks = n_sel;
   P.la.mlp = h->d_mlp;
   P.la.rounds_per_group = h->rounds_per_group;
   // specialised kernel when every chain uses the same supercell
   P.jit = nullptr;
   if (h->specialize && !P.learned) {
+    const int c0 = subset ? (*subset)[0] : 0;
     bool one_shape = true;
-    for (int c = 1; c < h->n_chains; ++c) one_shape = one_shape && h->chain_sc[c] == h->chain_sc[0];
-    const size_t fail = h->scs[h->chain_sc[0]].jit_lds_fail;
-    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[0], kind, packed);
+    for (int i = 1; i < n_sel; ++i) one_shape = one_shape && h->chain_sc[subset ? (*subset)[i] : i] == h->chain_sc[c0];
+    const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
+    P.sc_id = h->chain_sc[c0];
+    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed);
   }
   return CLEXMC_OK;
 }
@@ -1140,15 +1179,78 @@ static hipError_t plan_attributes(const RunPlan &P, clexmc_handle *h) {
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, false);
 }
 
-static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h) {
+static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st = nullptr) {
   if (P.jit) {
     RunArgs args = P.a;
     void *params[] = {(void *)&args};
-    return hipModuleLaunchKernel(P.jit, (unsigned)h->n_chains, 1, 1, (unsigned)P.threads, 1, 1, (unsigned)P.lds,
-                                 h->stream, params, nullptr);
+    return hipModuleLaunchKernel(P.jit, (unsigned)P.n_blocks, 1, 1, (unsigned)P.threads, 1, 1, (unsigned)P.lds,
+                                 st ? st : h->stream, params, nullptr);
   }
-  if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, true);
-  return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, true);
+  if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, P.n_blocks, P.threads, P.lds, h->stream, true);
+  return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, P.n_blocks, P.threads, P.lds, st ? st : h->stream,
+                           true);
+}
+
+// Chains on different supercells: one launch per supercell group, each with
+// the kernel specialised to its shape and its own block size and LDS image,
+// on side streams so that the groups run concurrently; the handle's stream
+// waits for all of them.
+static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
+  const int ng = (int)h->groups.size();
+  const int ns = std::min(ng, 8);
+  while ((int)h->side.size() < ns) {
+    hipStream_t s;
+    hipEvent_t e;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    h->side.push_back(s);
+    h->side_ev.push_back(e);
+  }
+  std::vector<RunPlan> plans(ng);
+  for (int g = 0; g < ng; ++g) {
+    clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, plans[g], 0, 0,
+                                &h->groups[g]);
+    if (st != CLEXMC_OK) return st;
+    plans[g].a.chain_map = h->d_group_map + h->group_off[g];
+    HIP_TRY(plan_attributes(plans[g], h));
+  }
+  // sample buffers may have grown while planning: every plan uses the final ones
+  for (auto &P : plans) {
+    P.a.sampE = h->d_sampE;
+    P.a.sampN = h->d_sampN;
+    P.a.sample_cap = h->sample_cap;
+  }
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (int s = 0; s < ns; ++s) HIP_TRY(hipStreamWaitEvent(h->side[s], h->ev0, 0));
+  bool all_spec = true, any_packed = false;
+  for (int g = 0; g < ng; ++g) {
+    RunPlan &P = plans[g];
+    hipStream_t st = h->side[g % ns];
+    hipError_t le = plan_launch(P, h, st);
+    if (le != hipSuccess && P.jit) {
+      (void)hipGetLastError();
+      h->scs[P.sc_id].jit_lds_fail = P.lds;
+      P.jit = nullptr;
+      HIP_TRY(plan_attributes(P, h));
+      le = plan_launch(P, h, st);
+    }
+    HIP_TRY(le);
+    all_spec = all_spec && P.jit != nullptr;
+    any_packed = any_packed || P.packed;
+  }
+  for (int s = 0; s < ns; ++s) {
+    HIP_TRY(hipEventRecord(h->side_ev[s], h->side[s]));
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->side_ev[s], 0));
+  }
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->last_specialized = all_spec ? 1 : 0;
+  h->last_packed = any_packed;
+  h->last_threads = plans[0].threads;
+  return CLEXMC_OK;
 }
 
 clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
@@ -1158,6 +1260,8 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
   if (n_passes == 0) return CLEXMC_OK;
   HIP_TRY(hipSetDevice(h->device));
+  if (h->specialize && h->proposal == 0 && h->groups.size() > 1)
+    return run_grouped(h, kind, n_passes, sample_period, canonical_rounds);
   RunPlan P;
   clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, P);
   if (st != CLEXMC_OK) return st;
@@ -1739,6 +1843,11 @@ void clexmc_destroy(clexmc_handle *h) {
   if (h->d_scs) hipFree(h->d_scs);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
+  for (auto e : h->side_ev) hipEventDestroy(e);
+  for (auto s : h->side) {
+    hipStreamSynchronize(s);
+    hipStreamDestroy(s);
+  }
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }

@@ -26,6 +26,7 @@ struct RunArgs {
   int n_orbits;
   int n_tab;           // general path: flower table floats staged in LDS
   int global_swaps;    // canonical: serial swaps of uniformly random site pairs per pass
+  const int *chain_map; // workgroup -> chain (nullptr: workgroup i runs chain i)
 };
 
 struct DeltaArgs {

@@ -220,7 +220,7 @@ struct GenericField {
 template <bool BIN, int KIND, bool PACK, class FIELD>
 __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int chain = blockIdx.x;
+  const int chain = args.chain_map ? args.chain_map[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int lane = tid & 63;

@@ -161,3 +161,33 @@ def test_specialized_general_kernel_bit_identical_to_generic(kind, packing):
     (s0, E0, n0), (s1, E1, n1) = runs
     _same(s0, s1)
     assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
+
+
+def test_grouped_shape_launches_bit_identical_to_one_launch():
+    """Chains on uneven supercells: the grouped launches (one specialised
+    kernel per supercell shape, concurrent streams) equal the single generic
+    launch over all chains bit for bit."""
+    _print_case("grouped specialised launches vs one generic launch")
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    shapes = [(6, 6, 4), (8, 8, 5), (10, 10, 6)]
+    occs, dims = [], []
+    for c in range(6):
+        d = shapes[c % 3]
+        dims.append(d)
+        occs.append(random_occupation_counts(composition_counts(2 * d[0] * d[1] * d[2], (0.4, 0.3, 0.3)), 80 + c))
+    temps = np.linspace(700, 1700, 6)
+    runs = []
+    for spec in (False, True):
+        eng = Engine(flat)
+        eng.set_specialization(spec)
+        sids = [eng.add_supercell(Supercell(ce.prim, d), n_pair_phases=4, seed=i) for i, d in enumerate(shapes)]
+        eng.load_chains([sids[c % 3] for c in range(6)], occs, temps, 300 + np.arange(6))
+        eng.run_passes("canonical", 10, 1, canonical_rounds=2)
+        assert eng.last_run_info()["specialized"] == spec, eng.specialization_log()[-2000:]
+        E, cnt = eng.read_samples()
+        runs.append((_states(eng, 6), E, cnt))
+        eng.close()
+    (s0, E0, n0), (s1, E1, n1) = runs
+    _same(s0, s1)
+    assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
