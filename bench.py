@@ -38,7 +38,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9  # 256 CU x 128 B/clk x 2.4 GHz = 78.6 TB/s
+# LDS read peak of MI355X (CDNA4: 160 KB LDS per CU with twice the CDNA3 read
+# bandwidth, 256 B/clk): 256 CU x 256 B/clk x 2.4 GHz = 157.3 TB/s.  bench.py
+# also measures the device's own LDS throughput (16-byte reads and byte gathers).
+LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
@@ -317,6 +320,10 @@ def main():
     ess_total = float(np.sum(E.shape[1] / (2.0 * taus)))
     bytes_per_attempt = 1 + 54  # the site and its 54 interaction neighbours, int8 in LDS
     achieved = attempts * bytes_per_attempt / (kernel_ms * 1e-3) / 1e9
+    # measured LDS read throughput of this device: byte gathers and 16-byte reads
+    lds_u8 = eng.lds_probe("u8")
+    lds_b128 = eng.lds_probe("b128")
+    launch = eng.last_run_info()
     out = {
         "metric": "MC site-attempts/s (FCC binary 24^3 SGC, 256 chains, 16 T x 16 mu)",
         "value": value,
@@ -335,7 +342,10 @@ def main():
                    "cluster_expansion": "FCC Chebyshev binary, pairs<=4NN, triplets<=2NN, 8 correlations"},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-                     "bytes_per_attempt": bytes_per_attempt, "kernel_ms": kernel_ms},
+                     "bytes_per_attempt": bytes_per_attempt, "kernel_ms": kernel_ms,
+                     "measured_lds_byte_gather_GBs": lds_u8, "frac_of_measured_byte_gather": achieved / lds_u8,
+                     "measured_lds_16B_read_GBs": lds_b128},
+        "launch": launch,
         "ess_per_s": ess_total / wall,
         "tau_int_E_median_passes": float(np.median(taus)),
         "acceptance": float(sum(eng.read_state(c)["n_accept"] for c in range(n_chains)) / max(1, att1)),

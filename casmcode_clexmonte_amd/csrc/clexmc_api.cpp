@@ -1460,6 +1460,29 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_lds_probe(clexmc_handle *h, int mode, double *gbps) {
+  if (!h || !gbps) return CLEXMC_ERR_ARG;
+  ARG_CHECK(mode == 0 || mode == 1, "mode must be 0 (byte gathers) or 1 (16-byte reads)");
+  HIP_TRY(hipSetDevice(h->device));
+  int n_cu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, h->device));
+  const int blocks = 4 * n_cu, iters = 4096;
+  DevBufs B;
+  unsigned *sink;
+  HIP_TRY(B.alloc(&sink, (size_t)blocks * 512));
+  HIP_TRY(launch_lds_probe(mode, 16, blocks, sink, h->stream));  // warm-up
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(launch_lds_probe(mode, iters, blocks, sink, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const double per_read = mode == 0 ? 1.0 : 16.0;
+  const double bytes = (double)blocks * 512.0 * iters * 8.0 * per_read;
+  *gbps = bytes / (ms * 1e-3) / 1e9;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_reanchor_energies(clexmc_handle *h, double *max_drift) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(h->n_chains > 0, "no chains loaded");

@@ -170,6 +170,47 @@ hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool pa
                    : launch_metro<false, 1, false>(args, n_chains, threads, lds, st, launch);
 }
 
+// ---------------------------------------------------------------------------
+// LDS read-throughput probe: every lane reads LDS in a loop and folds the
+// values into a checksum (written out so that nothing is optimised away).
+// mode 0: byte gathers, lanes of a wavefront 3 bytes apart (the stride of
+// independent-set class sites along a row) at 8 offsets per iteration;
+// mode 1: 16-byte reads, lanes 16 bytes apart (bank-conflict free).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void lds_probe_kernel(int mode, int iters, unsigned *sink) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int tid = threadIdx.x;
+  const int bytes = 32768;
+  for (int i = tid; i < bytes / 4; i += blockDim.x) ((unsigned *)lds)[i] = (unsigned)(i * 2654435761u);
+  __syncthreads();
+  unsigned acc = 0u;
+  if (mode == 0) {
+    const int base = (tid & 63) * 3 + (tid >> 6) * 211;
+    for (int it = 0; it < iters; ++it) {
+      const int o = (it * 97) & 4095;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc += lds[(base + o + k * 401) & (bytes - 1)];
+    }
+  } else {
+    const uint4 *v = (const uint4 *)lds;
+    const int base = tid & 511;
+    for (int it = 0; it < iters; ++it) {
+      const int o = (it * 64) & 1023;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint4 x = v[(base + o + k * 256) & 2047];
+        acc += x.x ^ x.y ^ x.z ^ x.w;
+      }
+    }
+  }
+  sink[blockIdx.x * blockDim.x + tid] = acc;
+}
+
+hipError_t launch_lds_probe(int mode, int iters, int blocks, unsigned *sink, hipStream_t st) {
+  hipLaunchKernelGGL(lds_probe_kernel, dim3(blocks), dim3(512), 32768, st, mode, iters, sink);
+  return hipGetLastError();
+}
+
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st) {
   const int threads = 128;
   const int blocks = (a.n_prop + threads - 1) / threads;

@@ -74,7 +74,7 @@ def lib():
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
-                     "clexmc_last_run_info", "clexmc_specialization_log"]:
+                     "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -94,7 +94,7 @@ def exported_symbols() -> List[str]:
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
-            "clexmc_last_run_info", "clexmc_specialization_log"]
+            "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe"]
 
 
 def _p(a, t):
@@ -494,3 +494,9 @@ class Engine:
         buf = C.create_string_buffer(1 << 16)
         self._check(lib().clexmc_specialization_log(self.h, buf, C.c_int(1 << 16)), "clexmc_specialization_log")
         return buf.value.decode(errors="replace")
+
+    def lds_probe(self, mode: str) -> float:
+        g = C.c_double()
+        self._check(lib().clexmc_lds_probe(self.h, C.c_int({"u8": 0, "b128": 1}[mode]), C.byref(g)),
+                    "clexmc_lds_probe")
+        return g.value

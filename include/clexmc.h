@@ -195,6 +195,11 @@ clexmc_status clexmc_graph_release(clexmc_handle *h);
 clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int canonical_rounds, int *threads_out,
                               int *packing_out, double *ms_per_pass_out);
 
+/* Measured LDS read throughput of the device (GB/s): mode 0 byte gathers
+ * (the Metropolis kernels' access type), mode 1 bank-conflict-free 16-byte
+ * reads.  A roofline reference, not part of a run. */
+clexmc_status clexmc_lds_probe(clexmc_handle *h, int mode, double *gbps);
+
 /* Replaces every chain's FP64 running energy (the sum of FP32 dEs on the
  * table path) by a full FP64 recompute of its correlations and returns the
  * largest |running - recomputed| (eV per supercell).  The binary path's
