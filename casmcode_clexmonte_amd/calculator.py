@@ -159,9 +159,12 @@ class MonteCalculator:
         if p.n_equilibration_passes > 0:
             self.engine.run_passes(kind, p.n_equilibration_passes, 1, p.canonical_rounds)
             self.engine.clear_samples()
+            # sampling starts from energies recomputed in FP64 (table-path CEs accumulate FP32 dE)
+            self.engine.reanchor_energies()
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
         acc0 = [self.engine.read_state(c)["n_accept"] for c in range(len(states))]
         self.engine.run_passes(kind, p.n_passes, p.sample_period, p.canonical_rounds)
+        self.engine.reanchor_energies()
         elapsed = time.perf_counter() - t0
         E, cnt = self.engine.read_samples()
         out = []

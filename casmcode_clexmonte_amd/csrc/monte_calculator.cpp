@@ -417,10 +417,13 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
     clexmc_last_run_ms(h, &ms);
     kms += ms;
     check(h, clexmc_clear_samples(h), calculator_name + "::run");
+    // sampling starts from energies recomputed in FP64 (table-path CEs accumulate FP32 dE)
+    check(h, clexmc_reanchor_energies(h, nullptr), calculator_name + "::run");
   }
   check(h, clexmc_run_passes(h, kind, (int)P.n_passes, P.sample_period, P.canonical_rounds), calculator_name + "::run");
   clexmc_last_run_ms(h, &ms);
   kms += ms;
+  check(h, clexmc_reanchor_energies(h, nullptr), calculator_name + "::run");
   double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   int ns = 0;
   check(h, clexmc_read_samples(h, &ns, nullptr, nullptr), calculator_name + "::run (samples)");
