@@ -33,11 +33,11 @@ struct HostSupercell {
   std::vector<int> bin_off;
   std::vector<GenSub> gen_sub;
   std::vector<int> gen_prog;
-  // specialised kernels: 0 not tried, 1 loaded, -1 unavailable
-  int jit_state = 0;
+  // specialised kernels per (kind, packed, lanes): 0 not tried, 1 loaded, -1 unavailable
+  int jit_state[2][2][2] = {};
   size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
-  hipModule_t jit_mod = nullptr;
-  hipFunction_t jit_fn[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  std::vector<hipModule_t> jit_mods;
+  hipFunction_t jit_fn[2][2][2] = {};
 };
 
 }  // namespace
@@ -84,9 +84,10 @@ struct clexmc_handle {
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
-  int last_specialized = 0, last_threads = 0;
+  int last_specialized = 0, last_threads = 0, last_lanes = 1;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
   int n_slots = 0;                  // binary path: orbits with >= 1 site
+  int lanes_per_site = 0;           // 0 automatic (1; the autotuner may pick 2), 1, 2
   int specialize = 1;               // run-time specialised binary kernels (1 when possible, 0 never)
   std::string jit_log;              // last specialisation compiler log
   hipGraph_t graph = nullptr;       // captured block of passes
@@ -1013,6 +1014,19 @@ clexmc_status clexmc_last_run_info(clexmc_handle *h, int *specialized, int *pack
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_last_run_lanes(clexmc_handle *h, int *lanes_per_site) {
+  if (!h || !lanes_per_site) return CLEXMC_ERR_ARG;
+  *lanes_per_site = h->last_lanes;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_set_lanes_per_site(clexmc_handle *h, int lanes) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(lanes >= 0 && lanes <= 2, "lanes per site must be 0 (automatic), 1 or 2");
+  h->lanes_per_site = lanes;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(mode >= 0 && mode <= 2, "packing must be 0 (automatic), 1 (int8) or 2 (2-bit)");
@@ -1027,6 +1041,7 @@ struct RunPlan {
   bool learned = false, packed = false;
   size_t lds = 0;
   hipFunction_t jit = nullptr;  // specialised kernel (one supercell shape)
+  int lanes = 1;                // lanes per site in the class sweeps (binary path: 1 or 2)
   int n_blocks = 0;             // workgroups (chains) of this launch
   int sc_id = 0;                // supercell of a single-shape launch
   RunArgs a;
@@ -1035,32 +1050,29 @@ struct RunPlan {
 
 // The specialised kernel for (supercell, kind, packing), compiled on first
 // use; nullptr when specialisation is off or unavailable.
-static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed) {
+static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes) {
   HostSupercell &hs = h->scs[sc_id];
-  if (hs.jit_state == 0) {
-    hs.jit_state = -1;
+  const int k = kind ? 1 : 0, p = packed ? 1 : 0, l = lanes == 2 ? 1 : 0;
+  if (hs.jit_state[k][p][l] == 0) {
+    hs.jit_state[k][p][l] = -1;
     std::vector<int> variable(h->n_sublat);
     for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
     const std::string src = jit_source(h->n_sublat, variable, h->binary != 0, hs.bin_sub, hs.bin_off, h->n_slots,
-                                       hs.gen_sub, hs.gen_prog);
+                                       hs.gen_sub, hs.gen_prog, kind, packed, lanes);
     hipModule_t mod = nullptr;
     const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
     if (je != hipSuccess) h->jit_log += std::string("\n[compile/load: ") + hipGetErrorString(je) + "]";
     if (je == hipSuccess) {
-      bool ok = true;
-      for (int k = 0; k < 2; ++k)
-        for (int p = 0; p < 2; ++p)
-          ok = ok && hipModuleGetFunction(&hs.jit_fn[k][p], mod, jit_kernel_name(k, p != 0)) == hipSuccess;
-      if (ok) {
-        hs.jit_mod = mod;
-        hs.jit_state = 1;
+      if (hipModuleGetFunction(&hs.jit_fn[k][p][l], mod, jit_kernel_name(kind, packed, lanes)) == hipSuccess) {
+        hs.jit_mods.push_back(mod);
+        hs.jit_state[k][p][l] = 1;
       } else {
         h->jit_log += "\n[kernel lookup failed]";
         hipModuleUnload(mod);
       }
     }
   }
-  return hs.jit_state == 1 ? hs.jit_fn[kind ? 1 : 0][packed ? 1 : 0] : nullptr;
+  return hs.jit_state[k][p][l] == 1 ? hs.jit_fn[k][p][l] : nullptr;
 }
 
 static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
@@ -1094,7 +1106,7 @@ static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
 // subset: chain indices of one supercell group (nullptr: all chains)
 static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
                               int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0,
-                              const std::vector<int> *subset = nullptr) {
+                              const std::vector<int> *subset = nullptr, int lanes_override = 0) {
   int max_pad = 0, max_class = 1;
   const int n_sel = subset ? (int)subset->size() : h->n_chains;
   for (int i = 0; i < n_sel; ++i) {
@@ -1133,6 +1145,9 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   clexmc_status st = grow_samples(h, total_passes / sample_period);
   if (st != CLEXMC_OK) return st;
   P.kind = kind;
+  P.lanes = 1;
+  const int lanes_req = lanes_override ? lanes_override : h->lanes_per_site;
+  if (h->binary && !P.learned && lanes_req == 2 && 2 * threads <= 1024) P.lanes = 2;
   P.threads = threads;
   P.packed = packed;
   P.lds = lds;
@@ -1166,7 +1181,7 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     for (int i = 1; i < n_sel; ++i) one_shape = one_shape && h->chain_sc[subset ? (*subset)[i] : i] == h->chain_sc[c0];
     const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
     P.sc_id = h->chain_sc[c0];
-    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed);
+    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes);
   }
   return CLEXMC_OK;
 }
@@ -1175,6 +1190,8 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
 // that a launch can be recorded into a graph
 static hipError_t plan_attributes(const RunPlan &P, clexmc_handle *h) {
   if (P.jit) return hipSuccess;
+  if (P.lanes == 2)
+    return launch_metropolis_split(P.a, P.kind, P.packed, P.n_blocks, P.threads * 2, P.lds, h->stream, false);
   if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, h->n_chains, P.threads, P.lds, h->stream, false);
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, false);
 }
@@ -1183,9 +1200,11 @@ static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st
   if (P.jit) {
     RunArgs args = P.a;
     void *params[] = {(void *)&args};
-    return hipModuleLaunchKernel(P.jit, (unsigned)P.n_blocks, 1, 1, (unsigned)P.threads, 1, 1, (unsigned)P.lds,
-                                 st ? st : h->stream, params, nullptr);
+    return hipModuleLaunchKernel(P.jit, (unsigned)P.n_blocks, 1, 1, (unsigned)(P.threads * P.lanes), 1, 1,
+                                 (unsigned)P.lds, st ? st : h->stream, params, nullptr);
   }
+  if (P.lanes == 2)
+    return launch_metropolis_split(P.a, P.kind, P.packed, P.n_blocks, P.threads * 2, P.lds, st ? st : h->stream, true);
   if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, P.n_blocks, P.threads, P.lds, h->stream, true);
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, P.n_blocks, P.threads, P.lds, st ? st : h->stream,
                            true);
@@ -1250,6 +1269,7 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
   h->last_specialized = all_spec ? 1 : 0;
   h->last_packed = any_packed;
   h->last_threads = plans[0].threads;
+  h->last_lanes = plans[0].lanes;
   return CLEXMC_OK;
 }
 
@@ -1268,6 +1288,7 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   h->last_packed = P.packed;
   h->last_specialized = P.jit ? 1 : 0;
   h->last_threads = P.threads;
+  h->last_lanes = P.lanes;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   hipError_t le = plan_launch(P, h);
@@ -1381,10 +1402,10 @@ clexmc_status clexmc_graph_release(clexmc_handle *h) {
 // Results are cached per (kind, proposal, CE shape, chain count, largest
 // padded supercell, largest class) for the life of the process.
 // ---------------------------------------------------------------------------
-static std::map<std::string, std::pair<int, int>> g_tune_cache;
+static std::map<std::string, std::array<int, 3>> g_tune_cache;  // threads, packing, lanes per site
 
 clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int canonical_rounds, int *threads_out,
-                              int *packing_out, double *ms_per_pass_out) {
+                              int *packing_out, int *lanes_out, double *ms_per_pass_out) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
   ARG_CHECK(trial_passes >= 1, "trial_passes >= 1 required");
@@ -1404,10 +1425,11 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
                           std::to_string(trial_passes);
   auto it = g_tune_cache.find(key);
   double best_ms = -1.0;
-  int best_t = 0, best_p = 0;
+  int best_t = 0, best_p = 0, best_l = 1;
   if (it != g_tune_cache.end()) {
-    best_t = it->second.first;
-    best_p = it->second.second;
+    best_t = it->second[0];
+    best_p = it->second[1];
+    best_l = it->second[2];
   } else {
     // snapshot of the chains' device state
     DevBufs B;
@@ -1424,13 +1446,15 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
     HIP_TRY(hipMemcpy(s_binC, h->d_binC, sizeof(long long) * h->n_chains * h->n_orbits, hipMemcpyDeviceToDevice));
     HIP_TRY(hipMemcpy(s_cnt, h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToDevice));
     const int t_max = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+    for (int lanes = 1; lanes <= (h->binary ? 2 : 1); ++lanes)
     for (int packing = 1; packing <= 2; ++packing) {
       for (int t = 64; t <= t_max; t *= 2) {
         RunPlan P;
-        if (plan_run(h, kind, trial_passes, trial_passes + 1, canonical_rounds, trial_passes, P, t, packing) !=
-            CLEXMC_OK)
+        if (plan_run(h, kind, trial_passes, trial_passes + 1, canonical_rounds, trial_passes, P, t, packing, nullptr,
+                     lanes) != CLEXMC_OK)
           continue;
         if (packing == 2 && (P.learned || !P.packed)) continue;
+        if (P.lanes != lanes) continue;
         HIP_TRY(plan_attributes(P, h));
         HIP_TRY(hipEventRecord(h->ev0, h->stream));
         HIP_TRY(plan_launch(P, h));
@@ -1446,16 +1470,19 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
           best_ms = ms;
           best_t = t;
           best_p = packing;
+          best_l = lanes;
         }
       }
     }
     ARG_CHECK(best_t > 0, "no launch configuration fits");
-    g_tune_cache[key] = std::make_pair(best_t, best_p);
+    g_tune_cache[key] = {best_t, best_p, best_l};
   }
   h->block_size = best_t;
   h->packing = best_p;
+  h->lanes_per_site = best_l;
   if (threads_out) *threads_out = best_t;
   if (packing_out) *packing_out = best_p;
+  if (lanes_out) *lanes_out = best_l;
   if (ms_per_pass_out) *ms_per_pass_out = best_ms < 0.0 ? -1.0 : best_ms / trial_passes;
   return CLEXMC_OK;
 }
@@ -1860,7 +1887,7 @@ void clexmc_destroy(clexmc_handle *h) {
   free_chains(h);
   for (auto &s : h->scs) {
     free_all(s.allocs);
-    if (s.jit_mod) hipModuleUnload(s.jit_mod);
+    for (auto m : s.jit_mods) hipModuleUnload(m);
   }
   free_all(h->sys_allocs);
   if (h->d_scs) hipFree(h->d_scs);

@@ -4,6 +4,8 @@
 #include <hip/hiprtc.h>
 
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <sstream>
 
 #include "clexmc_jit_src.inc"  // kJitHeaderNames / kJitHeaderTexts: the device headers as text
@@ -18,7 +20,7 @@ static std::string hexf(float v) {
 
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog) {
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes) {
   std::ostringstream o;
   o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\nstruct JitField {\n"
     << "  template <class IMG>\n"
@@ -59,6 +61,60 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     o << "      return F;\n    }\n";
   }
   o << "    return F;\n  }\n";
+  // lane-pair partial sums (clusters of each orbit split in halves) and the finish
+  o << "  template <class IMG>\n"
+    << "  __device__ __forceinline__ static void bin_x(const FieldCtx &, const IMG &img, int b, int pa, int o, int part,\n"
+    << "                                               int (&X)[kMaxSlots]) {\n"
+    << "    (void)img; (void)b; (void)pa; (void)o; (void)part;\n";
+  for (int j = 0; j < kMaxSlots; ++j) o << "    X[" << j << "] = 0;\n";
+  for (int b = 0; binary && b < n_sublat; ++b) {
+    if (!variable[b]) continue;
+    const BinSub &B = subs[b];
+    int minoff = 0;
+    for (int j = 0; j < n_slots; ++j)
+      for (int e = 0; e < B.slot_n[j] * B.slot_k[j]; ++e) minoff = std::min(minoff, bin_off[B.slot_begin[j] + e]);
+    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    for (int part = 0; part < 2; ++part) {
+      o << "      if (part == " << part << ") {\n";
+      for (int j = 0; j < n_slots; ++j) {
+        const int n = B.slot_n[j], ko = B.slot_k[j];
+        if (n <= 0 || ko <= 0) continue;
+        const int e0 = part == 0 ? 0 : (n + 1) / 2, e1 = part == 0 ? (n + 1) / 2 : n;
+        for (int e = e0; e < e1; ++e) {
+          o << "        X[" << j << "] += ";
+          for (int i = 0; i < ko; ++i) {
+            if (i) o << " ^ ";
+            o << "img.get(base + " << (bin_off[B.slot_begin[j] + e * ko + i] - minoff) << ")";
+          }
+          o << ";\n";
+        }
+      }
+      o << "      }\n";
+    }
+    o << "      return;\n    }\n";
+  }
+  o << "  }\n";
+  o << "  __device__ __forceinline__ static float bin_finish(const FieldCtx &, int b, const int (&X)[kMaxSlots],\n"
+    << "                                                     int (&K)[kMaxSlots]) {\n"
+    << "    (void)b; (void)X;\n";
+  for (int j = 0; j < kMaxSlots; ++j) o << "    K[" << j << "] = 0;\n";
+  o << "    float F = 0.0f;\n";
+  for (int b = 0; binary && b < n_sublat; ++b) {
+    if (!variable[b]) continue;
+    const BinSub &B = subs[b];
+    o << "    if (b == " << b << ") {\n";
+    for (int j = 0; j < n_slots; ++j) {
+      const int n = B.slot_n[j], ko = B.slot_k[j];
+      if (n <= 0) continue;
+      if (ko & 1)
+        o << "      K[" << j << "] = 2 * X[" << j << "] - " << n << ";\n";
+      else
+        o << "      K[" << j << "] = " << n << " - 2 * X[" << j << "];\n";
+      o << "      F = __builtin_fmaf(" << hexf(B.slot_w[j]) << ", (float)K[" << j << "], F);\n";
+    }
+    o << "      return F;\n    }\n";
+  }
+  o << "    return F;\n  }\n";
   // general table path: same entry order and accumulation as general_field
   o << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static void gen(const FieldCtx &, const IMG &img, int b, int pa, const float *tab,\n"
@@ -92,20 +148,30 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     o << "      return;\n    }\n";
   }
   o << "  }\n};\n}  // namespace clexmc\n";
-  for (int kind = 0; kind < 2; ++kind)
-    for (int pk = 0; pk < 2; ++pk)
-      o << "extern \"C\" __global__ __launch_bounds__(512) void " << jit_kernel_name(kind, pk != 0)
-        << "(clexmc::RunArgs a) { clexmc::metropolis_body<" << (binary ? "true" : "false") << ", " << kind << ", "
-        << (pk ? "true" : "false") << ", clexmc::JitField>(a); }\n";
+  o << "extern \"C\" __global__ __launch_bounds__(" << (lanes == 2 ? 1024 : 512) << ") void "
+    << jit_kernel_name(kind, packed, lanes) << "(clexmc::RunArgs a) { clexmc::metropolis_body<"
+    << (binary ? "true" : "false") << ", " << kind << ", " << (packed ? "true" : "false") << ", clexmc::JitField, "
+    << lanes << ">(a); }\n";
   return o.str();
 }
 
-const char *jit_kernel_name(int kind, bool packed) {
-  static const char *names[2][2] = {{"clexmc_jit_c_u8", "clexmc_jit_c_p2"}, {"clexmc_jit_s_u8", "clexmc_jit_s_p2"}};
-  return names[kind ? 1 : 0][packed ? 1 : 0];
+const char *jit_kernel_name(int kind, bool packed, int lanes_per_site) {
+  static const char *names[2][2][2] = {{{"clexmc_jit_c_u8", "clexmc_jit_c_u8_x2"}, {"clexmc_jit_c_p2", "clexmc_jit_c_p2_x2"}},
+                                       {{"clexmc_jit_s_u8", "clexmc_jit_s_u8_x2"}, {"clexmc_jit_s_p2", "clexmc_jit_s_p2_x2"}}};
+  return names[kind ? 1 : 0][packed ? 1 : 0][lanes_per_site == 2 ? 1 : 0];
 }
 
+// code objects of the sources compiled so far in this process (a handle or
+// supercell with the same site programs and variant reuses them)
+static std::mutex g_code_mutex;
+static std::map<std::string, std::vector<char>> g_code_cache;
+
 hipError_t jit_compile_load(const std::string &src, hipModule_t *mod, std::string *log) {
+  {
+    std::lock_guard<std::mutex> lock(g_code_mutex);
+    auto it = g_code_cache.find(src);
+    if (it != g_code_cache.end()) return hipModuleLoadData(mod, it->second.data());
+  }
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "clexmc_jit.hip", kJitNumHeaders, kJitHeaderTexts,
                           kJitHeaderNames) != HIPRTC_SUCCESS)
@@ -128,7 +194,12 @@ hipError_t jit_compile_load(const std::string &src, hipModule_t *mod, std::strin
   std::vector<char> code(cs);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
-  return hipModuleLoadData(mod, code.data());
+  hipError_t e = hipModuleLoadData(mod, code.data());
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_code_mutex);
+    g_code_cache.emplace(src, std::move(code));
+  }
+  return e;
 }
 
 }  // namespace clexmc

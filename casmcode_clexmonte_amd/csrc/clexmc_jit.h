@@ -17,14 +17,17 @@ namespace clexmc {
 // the binary programs (subs[n_sublat], offsets as laid out by the library)
 // when `binary`, else the general table programs (gsubs[n_sublat], gen_prog:
 // per cluster k, table offset, then k (offset, radix multiplier) pairs).
+// One kernel per (kind, packed, lanes per site) variant.
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog);
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes);
 
-// Kernel name for (kind, packed) in a module built from jit_source.
-const char *jit_kernel_name(int kind, bool packed);
+// Kernel name for (kind, packed, lanes per site) in a module built from
+// jit_source (two lanes per site: binary expansions only).
+const char *jit_kernel_name(int kind, bool packed, int lanes_per_site);
 
-// Compiles `src` (hipRTC, gfx950) and loads it as a module.
+// Compiles `src` (hipRTC, gfx950) and loads it as a module; code objects are
+// cached per source text for the life of the process.
 hipError_t jit_compile_load(const std::string &src, hipModule_t *mod, std::string *log);
 
 }  // namespace clexmc

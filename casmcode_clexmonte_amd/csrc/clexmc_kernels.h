@@ -98,6 +98,8 @@ hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, in
                           int threads, size_t lds, hipStream_t st, bool launch);
 size_t learned_lds_bytes(int pad_bytes);
 hipError_t launch_mlp_infer(const MlpDev *mlp, int n, const int8_t *X, float *out, hipStream_t st);
+hipError_t launch_metropolis_split(const RunArgs &args, int kind, bool packed, int n_chains, int threads, size_t lds,
+                                   hipStream_t st, bool launch);
 hipError_t launch_lds_probe(int mode, int iters, int blocks, unsigned *sink, hipStream_t st);
 hipError_t launch_wave_pick(int n_rows, const float *v, const float *u, float *scan_out, int *pick_out,
                             hipStream_t st);

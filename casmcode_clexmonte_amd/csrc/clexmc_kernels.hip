@@ -36,6 +36,12 @@ __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
   metropolis_body<BIN, KIND, PACK, GenericField>(args);
 }
 
+// binary path, two lanes per site in the class sweeps (up to 1024 threads)
+template <int KIND, bool PACK>
+__global__ __launch_bounds__(1024) void metropolis_split_kernel(RunArgs args) {
+  metropolis_body<true, KIND, PACK, GenericField, 2>(args);
+}
+
 // ---------------------------------------------------------------------------
 // Per-proposal dE in FP32 on unpadded occupation (periodic wrap)
 // ---------------------------------------------------------------------------
@@ -152,6 +158,24 @@ static hipError_t launch_metro(const RunArgs &args, int n_chains, int threads, s
   if (!launch) return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args);
   return hipGetLastError();
+}
+
+template <int KIND, bool PACK>
+static hipError_t launch_split(const RunArgs &args, int n_chains, int threads, size_t lds, hipStream_t st,
+                               bool launch) {
+  auto kern = metropolis_split_kernel<KIND, PACK>;
+  if (!launch) return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(kern, dim3(n_chains), dim3(threads), lds, st, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_metropolis_split(const RunArgs &args, int kind, bool packed, int n_chains, int threads, size_t lds,
+                                   hipStream_t st, bool launch) {
+  if (packed)
+    return kind == 0 ? launch_split<0, true>(args, n_chains, threads, lds, st, launch)
+                     : launch_split<1, true>(args, n_chains, threads, lds, st, launch);
+  return kind == 0 ? launch_split<0, false>(args, n_chains, threads, lds, st, launch)
+                   : launch_split<1, false>(args, n_chains, threads, lds, st, launch);
 }
 
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,

@@ -155,6 +155,59 @@ __device__ __forceinline__ float binary_field(const IMG &img, int pa, int o, cpt
   return F;
 }
 
+// Partial cluster sums of the binary field for lane pairs: part p of nparts
+// takes a contiguous share of each orbit's cluster list (in chunks of 4, the
+// padding unit); part 0 also removes the padding's self copies.  Summing the
+// parts gives binary_field's X exactly.
+template <class IMG>
+__device__ __forceinline__ void binary_partial_x(const IMG &img, int pa, int o, cptr<BinSub> bs, cptr<int> off,
+                                                 int n_slots, int part, int nparts, int (&X)[kMaxSlots]) {
+#pragma unroll
+  for (int j = 0; j < kMaxSlots; ++j) {
+    X[j] = 0;
+    if (j < n_slots) {
+      const int n = bs->slot_n[j];
+      if (n > 0) {
+        const int ko = bs->slot_k[j];
+        const int q0 = bs->slot_begin[j];
+        const int chunks = bs->slot_len[j] / 4;
+        const int c0 = (chunks * part) / nparts, c1 = (chunks * (part + 1)) / nparts;
+        int x = 0;
+        for (int e = 4 * c0; e < 4 * c1; ++e) {
+          int y = 0;
+          for (int q = q0 + e * ko; q < q0 + (e + 1) * ko; ++q) y ^= img.get(pa + off[q]);
+          x += y;
+        }
+        if (part == 0 && (ko & 1)) x -= bs->slot_dummy[j] * o;
+        X[j] = x;
+      }
+    }
+  }
+}
+
+// K and F from complete cluster sums X (the same arithmetic as binary_field)
+__device__ __forceinline__ float binary_finish(cptr<BinSub> bs, int n_slots, const int (&X)[kMaxSlots],
+                                               int (&K)[kMaxSlots]) {
+  float F = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kMaxSlots; ++j) {
+    K[j] = 0;
+    if (j < n_slots) {
+      const int n = bs->slot_n[j];
+      if (n > 0) {
+        const int ko = bs->slot_k[j];
+        const int Kj = (ko & 1) ? (2 * X[j] - n) : (n - 2 * X[j]);
+        K[j] = Kj;
+        F = __builtin_fmaf(bs->slot_w[j], (float)Kj, F);
+      }
+    }
+  }
+  return F;
+}
+
+// value of the partner lane (lane ^ 1) by DPP quad permutation [1, 0, 3, 2]
+__device__ __forceinline__ int pair_swap(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
+
 // General path: G_f(s) = sum_e V_e[tau_others][f] from the flower tables.
 template <class IMG>
 __device__ __forceinline__ void general_field(const IMG &img, int pa, cptr<GenSub> gs, cptr<int> prog,
@@ -208,6 +261,15 @@ struct GenericField {
     return binary_field(img, pa, o, cst(c.sc->bin_sub + b), c.bin_off, c.n_slots, K);
   }
   template <class IMG>
+  __device__ __forceinline__ static void bin_x(const FieldCtx &c, const IMG &img, int b, int pa, int o, int part,
+                                               int (&X)[kMaxSlots]) {
+    binary_partial_x(img, pa, o, cst(c.sc->bin_sub + b), c.bin_off, c.n_slots, part, 2, X);
+  }
+  __device__ __forceinline__ static float bin_finish(const FieldCtx &c, int b, const int (&X)[kMaxSlots],
+                                                     int (&K)[kMaxSlots]) {
+    return binary_finish(cst(c.sc->bin_sub + b), c.n_slots, X, K);
+  }
+  template <class IMG>
   __device__ __forceinline__ static void gen(const FieldCtx &c, const IMG &img, int b, int pa, const float *tab,
                                              float (&G)[kMaxFunc]) {
     general_field(img, pa, cst(c.sc->gen_sub + b), c.gen_prog, tab, G);
@@ -217,7 +279,10 @@ struct GenericField {
 // ---------------------------------------------------------------------------
 // Persistent Metropolis kernel
 // ---------------------------------------------------------------------------
-template <bool BIN, int KIND, bool PACK, class FIELD>
+// SPLIT == 2 (binary path): the two lanes of a pair share one site's field in
+// the class sweeps (each sums half of every orbit's clusters, combined with a
+// DPP lane swap), which doubles the wavefronts per chain.
+template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1>
 __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int chain = args.chain_map ? args.chain_map[blockIdx.x] : (int)blockIdx.x;
@@ -325,6 +390,32 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
       if (KIND == 1) {
         // ---- semi-grand canonical: every site of the class proposes a change
+        if constexpr (BIN && SPLIT == 2) {
+          const int part = tid & 1;
+          for (int t = cb + (tid >> 1); t < ce; t += (nthr >> 1)) {
+            const int pa = sc->class_addr[t];
+            const int o = img.get(pa);
+            int X[kMaxSlots], K[kMaxSlots];
+            FIELD::bin_x(fctx, img, b, pa, o, part, X);
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) X[a] += pair_swap(X[a]);
+            const float F = FIELD::bin_finish(fctx, b, X, K);
+            if (part == 0) {
+              const float u = rng.next();
+              const float dphi = o ? -2.0f : 2.0f;
+              const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
+              const bool acc = metropolis_accept(dE, beta, u);
+              accAtt += 1;
+              const int sk = acc ? (o ? -2 : 2) : 0;
+#pragma unroll
+              for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+              accN[1] += sk / 2;
+              accN[0] -= sk / 2;
+              accAcc += acc ? 1 : 0;
+              if (acc) img.set(pa, o, 1 - o);
+            }
+          }
+        } else
         for (int t = cb + tid; t < ce; t += nthr) {
           const int pa = sc->class_addr[t];
           const int o = img.get(pa);
@@ -368,6 +459,22 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       } else {
         // ---- canonical: fields of all class sites, then random matchings
         const int n = ce - cb;
+        if constexpr (BIN && SPLIT == 2) {
+          const int part = tid & 1;
+          for (int t = tid >> 1; t < n; t += (nthr >> 1)) {
+            const int pa = sc->class_addr[cb + t];
+            int X[kMaxSlots], K[kMaxSlots];
+            FIELD::bin_x(fctx, img, b, pa, img.get(pa), part, X);
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) X[a] += pair_swap(X[a]);
+            const float F = FIELD::bin_finish(fctx, b, X, K);
+            if (part == 0) {
+              cF[t] = F;
+#pragma unroll
+              for (int a = 0; a < kMaxSlots; ++a) cK[a * max_class + t] = (int8_t)K[a];
+            }
+          }
+        } else
         for (int t = tid; t < n; t += nthr) {
           const int pa = sc->class_addr[cb + t];
           if (BIN) {

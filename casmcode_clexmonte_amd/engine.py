@@ -74,7 +74,8 @@ def lib():
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
-                     "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe"]:
+                     "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
+                     "clexmc_set_lanes_per_site", "clexmc_last_run_lanes"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -94,7 +95,8 @@ def exported_symbols() -> List[str]:
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
-            "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe"]
+            "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
+            "clexmc_set_lanes_per_site", "clexmc_last_run_lanes"]
 
 
 def _p(a, t):
@@ -476,19 +478,25 @@ class Engine:
         k = 0 if kind == "canonical" else 1
         t = C.c_int()
         p = C.c_int()
+        ln = C.c_int()
         ms = C.c_double()
         self._check(lib().clexmc_autotune(self.h, C.c_int(k), C.c_int(trial_passes), C.c_int(canonical_rounds),
-                                          C.byref(t), C.byref(p), C.byref(ms)), "clexmc_autotune")
+                                          C.byref(t), C.byref(p), C.byref(ln), C.byref(ms)), "clexmc_autotune")
         return {"threads": t.value, "packing": {1: "int8", 2: "packed2"}.get(p.value, "auto"),
-                "ms_per_pass": ms.value}
+                "lanes_per_site": ln.value, "ms_per_pass": ms.value}
 
     def set_specialization(self, on: bool):
         self._check(lib().clexmc_set_specialization(self.h, C.c_int(1 if on else 0)), "clexmc_set_specialization")
 
     def last_run_info(self):
-        sp, pk, th = C.c_int(), C.c_int(), C.c_int()
+        sp, pk, th, ln = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._check(lib().clexmc_last_run_info(self.h, C.byref(sp), C.byref(pk), C.byref(th)), "clexmc_last_run_info")
-        return {"specialized": bool(sp.value), "packed": bool(pk.value), "threads": th.value}
+        self._check(lib().clexmc_last_run_lanes(self.h, C.byref(ln)), "clexmc_last_run_lanes")
+        return {"specialized": bool(sp.value), "packed": bool(pk.value), "threads": th.value,
+                "lanes_per_site": ln.value}
+
+    def set_lanes_per_site(self, lanes: int):
+        self._check(lib().clexmc_set_lanes_per_site(self.h, C.c_int(lanes)), "clexmc_set_lanes_per_site")
 
     def specialization_log(self) -> str:
         buf = C.create_string_buffer(1 << 16)

@@ -187,13 +187,14 @@ clexmc_status clexmc_graph_replay(clexmc_handle *h, int n_replays);
 clexmc_status clexmc_graph_release(clexmc_handle *h);
 
 /* Autotuner: times trial_passes-pass launches of the loaded chains over
- * block sizes (64..512) and LDS image formats (int8 where it fits, 2-bit
- * packed), restores the chains' device state after each trial, and sets the
- * fastest as the handle's block size and packing.  Cached per (kind,
+ * block sizes (64..512), LDS image formats (int8 where it fits, 2-bit
+ * packed) and, for binary expansions, 1 or 2 lanes per site; restores the
+ * chains' device state after each trial, and sets the fastest as the
+ * handle's block size, packing and lanes per site.  Cached per (kind,
  * proposal, CE shape, chain count, largest padded supercell, largest class).
  * Outputs may be NULL; ms_per_pass is -1 when the cache answered. */
 clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int canonical_rounds, int *threads_out,
-                              int *packing_out, double *ms_per_pass_out);
+                              int *packing_out, int *lanes_out, double *ms_per_pass_out);
 
 /* Measured LDS read throughput of the device (GB/s): mode 0 byte gathers
  * (the Metropolis kernels' access type), mode 1 bank-conflict-free 16-byte
@@ -233,6 +234,14 @@ clexmc_status clexmc_specialization_log(clexmc_handle *h, char *buf, int cap);
 /* What the last clexmc_run_passes launched: specialised kernel (0/1), 2-bit
  * packed image (0/1), block size. */
 clexmc_status clexmc_last_run_info(clexmc_handle *h, int *specialized, int *packed, int *threads);
+
+/* Binary expansions: lanes per site in the class sweeps (2: the two lanes of
+ * a pair each sum half of every orbit's clusters, doubling the wavefronts per
+ * chain, at the cost of neighbour reads the two halves both need).  0
+ * automatic (1, or what clexmc_autotune chose), 1, 2.  Results do not depend
+ * on it except through the RNG stream assignment. */
+clexmc_status clexmc_set_lanes_per_site(clexmc_handle *h, int lanes);
+clexmc_status clexmc_last_run_lanes(clexmc_handle *h, int *lanes_per_site);
 
 /* LDS image format of the Metropolis kernel: 0 automatic (int8 per site when
  * the largest padded supercell fits, else 2 bits per site), 1 int8, 2 packed

@@ -94,6 +94,7 @@ def test_autotune_restores_state_and_matches_manual_setting(which):
     manual = _engine(flat, sc, occs, temps, chem, n_phases)
     manual.set_block_size(cfg["threads"])
     manual.set_image_packing(cfg["packing"])
+    manual.set_lanes_per_site(cfg["lanes_per_site"])
     tuned.run_passes(kind, 15, 1, canonical_rounds=2)
     manual.run_passes(kind, 15, 1, canonical_rounds=2)
     _same(_states(tuned, len(occs)), _states(manual, len(occs)))
@@ -101,13 +102,14 @@ def test_autotune_restores_state_and_matches_manual_setting(which):
     manual.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("kind", ["semigrand_canonical", "canonical"])
 @pytest.mark.parametrize("packing", ["int8", "packed2"])
-def test_specialized_kernel_bit_identical_to_generic(kind, packing):
+def test_specialized_kernel_bit_identical_to_generic(kind, packing, lanes):
     """The run-time specialised binary kernel (constant site offsets) leaves
     every chain bit-for-bit where the generic kernel (interpreted site
     programs) does: same occupations, exact counts, energies, samples."""
-    _print_case(f"specialised vs generic {kind} {packing}")
+    _print_case(f"specialised vs generic {kind} {packing} lanes/site {lanes}")
     ce = fcc_binary_ce()
     flat = FlatCE(ce)
     sc = Supercell(ce.prim, (12, 12, 12))
@@ -119,9 +121,11 @@ def test_specialized_kernel_bit_identical_to_generic(kind, packing):
         eng = _engine(flat, sc, occs, temps, chem, 8)
         eng.set_specialization(spec)
         eng.set_image_packing(packing)
+        eng.set_lanes_per_site(lanes)
         eng.run_passes(kind, 25, 1, canonical_rounds=2)
         info = eng.last_run_info()
-        assert info["specialized"] == spec, (info, eng.specialization_log()[-3000:])
+        assert info["specialized"] == spec and info["lanes_per_site"] == lanes, \
+            (info, eng.specialization_log()[-3000:])
         E, cnt = eng.read_samples()
         runs.append((_states(eng, len(occs)), [eng.read_binary_counts(c) for c in range(len(occs))], E, cnt))
         eng.close()

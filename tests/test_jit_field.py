@@ -5,7 +5,8 @@ results as the interpreted site program, on random images.
 * binary: the integer cluster-product counts K and the FP32 field
   F = fma(w_j, K_j, F) (the generic kernel's rule: per orbit slot, X = sum
   over clusters of the XOR of the other sites, K = 2X - n for an odd number
-  of other sites, n - 2X for an even number);
+  of other sites, n - 2X for an even number), whole and as the two lane-pair
+  partial sums followed by the finish;
 * general (table path): G_f = sum over clusters of V[toff + idx * nf + f],
   idx = sum of occupant x radix multiplier of the other sites, in entry
   order (bit-identical FP32 sums).
@@ -56,6 +57,14 @@ int main(int, char **argv) {
     const float Fj = JitField::bin(c, im, 0, pa, img[pa], Kj);
     for (int j = 0; j < 8; ++j) bad += Kj[j] != Kg[j];
     bad += Fj != Fg;
+    // lane-pair partial sums, combined, then the finish
+    int X0[8], X1[8], X[8], K2[8];
+    JitField::bin_x(c, im, 0, pa, img[pa], 0, X0);
+    JitField::bin_x(c, im, 0, pa, img[pa], 1, X1);
+    for (int j = 0; j < 8; ++j) X[j] = X0[j] + X1[j];
+    const float F2 = JitField::bin_finish(c, 0, X, K2);
+    for (int j = 0; j < 8; ++j) bad += K2[j] != Kg[j];
+    bad += F2 != Fg;
   }
   printf("%d\n", bad); return 0;
 }
