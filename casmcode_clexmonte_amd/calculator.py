@@ -55,8 +55,8 @@ class SamplingFixtureParams:
     n_equilibration_passes: int = 100
     n_passes: int = 1000
     sample_period: int = 1
-    canonical_rounds: int = 2
-    n_pair_phases: int = 8
+    canonical_rounds: int = 0       # 0: automatic, about one swap attempt per mutable site and pass
+    n_pair_phases: int = 32
 
 
 class MontePotential:

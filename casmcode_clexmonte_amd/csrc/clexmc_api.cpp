@@ -48,7 +48,7 @@ struct clexmc_handle {
   std::string err;
   // system (host copies)
   int n_sublat = 0, n_occ_max = 0, n_func_max = 0, n_orbits = 0, n_corr = 0, binary = 0;
-  std::vector<int> n_occ, orbit_k, orbit_mult, orbit_nfunc, orbit_func_begin;
+  std::vector<int> n_occ, asym, orbit_k, orbit_mult, orbit_nfunc, orbit_func_begin;
   std::vector<int> flower_begin, entry_k, entry_site_begin, entry_sites, entry_table, entry_orbit;
   std::vector<int> entry_mul;
   std::vector<double> bin_w;
@@ -200,6 +200,8 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
     h->binary = s->binary && s->n_orbits <= kMaxOrbits && n_slots <= kMaxSlots;
   }
   h->n_occ.assign(s->n_occ, s->n_occ + s->n_sublat);
+  h->asym.resize(s->n_sublat);
+  for (int b = 0; b < s->n_sublat; ++b) h->asym[b] = s->asym ? s->asym[b] : s->n_occ[b];
   h->orbit_k.assign(s->orbit_k, s->orbit_k + s->n_orbits);
   h->orbit_mult.assign(s->orbit_mult, s->orbit_mult + s->n_orbits);
   h->orbit_nfunc.assign(s->orbit_nfunc, s->orbit_nfunc + s->n_orbits);
@@ -224,6 +226,7 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   d.binary = h->binary;
   for (int b = 0; b < s->n_sublat; ++b) {
     d.n_occ[b] = s->n_occ[b];
+    d.asym[b] = h->asym[b];
     ARG_CHECK(s->n_occ[b] >= 1 && s->n_occ[b] <= kMaxOcc, "n_occ out of range");
     d.n_func[b] = s->n_occ[b] - 1;
     for (int f = 0; f < kMaxFunc; ++f)
@@ -465,7 +468,7 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
     int b = var[splitmix64(st) % var.size()];
     int b2 = var[splitmix64(st) % var.size()];
     int v[3] = {(int)(splitmix64(st) % L[0]), (int)(splitmix64(st) % L[1]), (int)(splitmix64(st) % L[2])};
-    if (h->n_occ[b2] != h->n_occ[b]) continue;
+    if (h->n_occ[b2] != h->n_occ[b] || h->asym[b2] != h->asym[b]) continue;
     if (b2 == b && v[0] == 0 && v[1] == 0 && v[2] == 0) continue;
     bool bad = false;
     for (auto const &o : offs[b])
@@ -705,7 +708,8 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       int b, i, j, k, b2, i2, j2, k2;
       decode(ls, b, i, j, k);
       decode(lt, b2, i2, j2, k2);
-      ARG_CHECK(h->n_occ[b] == h->n_occ[b2] && h->n_occ[b] >= 2, "pair sites need the same occupant count");
+      ARG_CHECK(h->n_occ[b] == h->n_occ[b2] && h->n_occ[b] >= 2 && h->asym[b] == h->asym[b2],
+                "pair sites need one asymmetric unit");
       pair_addr.push_back(make_int2(pad_addr(b, i, j, k), pad_addr(b2, i2, j2, k2)));
       pair_sub.push_back(b | (b2 << 8));
       images(ls, img);
@@ -1107,6 +1111,7 @@ static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
 static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
                               int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0,
                               const std::vector<int> *subset = nullptr, int lanes_override = 0) {
+  ARG_CHECK(canonical_rounds >= 0 && canonical_rounds <= 4096, "canonical_rounds out of range (0..4096)");
   int max_pad = 0, max_class = 1;
   const int n_sel = subset ? (int)subset->size() : h->n_chains;
   for (int i = 0; i < n_sel; ++i) {
@@ -1165,7 +1170,7 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.sample_cap = h->sample_cap;
   a.n_passes = n_passes;
   a.sample_period = sample_period;
-  a.rounds = std::max(1, canonical_rounds);
+  a.rounds = canonical_rounds <= 0 ? 0 : canonical_rounds;  // 0: automatic per independent set
   a.n_orbits = h->n_orbits;
   a.n_tab = n_tab;
   a.global_swaps = h->global_swaps;

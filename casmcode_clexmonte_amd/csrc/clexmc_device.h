@@ -27,6 +27,36 @@ constexpr int kMaxFunc = 3;
 constexpr int kMaxOrbits = 64;   // correlations kernel / exact counts storage
 constexpr int kMaxSlots = 8;     // binary path: orbits with >= 1 site (registers)
 constexpr int kMaxSitesPerProp = 4;
+constexpr int kMaxAutoRounds = 1024; // canonical matchings per independent set and pass (automatic)
+
+// Automatic canonical rounds of one independent set.  A random matching of
+// the set's n sites pairs a given site with each other site with probability
+// 1/n, so a round makes m / (2n) attempts between different occupants on
+// average, m = n^2 - sum_o n_o^2 (twice the set's unordered pairs of
+// different occupants).  Giving every set with m > 0 ceil(2 N n / M) rounds,
+// M = sum of m over the sets and N = number of mutable sites, makes about N
+// such attempts per pass in total, spread over the sets in proportion to
+// their pairs of different occupants: the reference's steps per pass, each a
+// swap of two different occupants.  The counts depend only on occupant counts
+// per set, which swaps inside the sets conserve, so the class sweep remains a
+// fixed composition of stationary moves.
+// Automatic pair-phase rounds: independent sets never exchange occupants
+// among themselves, so with automatic rounds a pass also runs enough pair
+// phases (disjoint non-interacting pairs across sets and sublattices) for
+// about n_mutable / 4 pair attempts, at most kMaxPairRounds.
+constexpr int kMaxPairRounds = 64;
+__host__ __device__ inline int auto_pair_rounds(int n_mutable, int n_pairs, int n_phases) {
+  if (n_phases <= 0 || n_pairs <= 0) return 0;
+  const long long per = (n_pairs + n_phases - 1) / n_phases;
+  const long long r = (n_mutable + 4 * per - 1) / (4 * per);
+  return r < 1 ? 1 : (r > kMaxPairRounds ? kMaxPairRounds : (int)r);
+}
+
+__host__ __device__ inline int auto_canonical_rounds(int n, long long m, long long n_mutable, long long m_total) {
+  if (n < 2 || m <= 0 || m_total <= 0) return 0;
+  const long long r = (2 * n_mutable * n + m_total - 1) / m_total;
+  return r > kMaxAutoRounds ? kMaxAutoRounds : (int)r;
+}
 // learned proposal: local-environment MLP (n_env <= kEnvMax inputs, kHidden units)
 constexpr int kEnvMax = 48;
 constexpr int kHidden = 64;
@@ -45,6 +75,7 @@ __device__ __forceinline__ cptr<T> cst(const T *p) {
 struct DevSystem {
   int n_sublat, n_occ_max, n_func_max, n_orbits, n_corr, binary;
   int n_occ[kMaxSublat];
+  int asym[kMaxSublat];         // asymmetric unit (canonical swaps stay within one)
   int n_func[kMaxSublat];
   float site_func[kMaxSublat][kMaxFunc][kMaxOcc];
   // flower (general table path, delta batch)

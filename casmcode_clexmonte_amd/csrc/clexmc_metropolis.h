@@ -375,13 +375,42 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   double accE = 0.0;
 
   const unsigned long long pass0 = cs->pass_base;
-  const int rounds = args.rounds < 1 ? 1 : args.rounds;
+  const bool auto_rounds = args.rounds <= 0;
+  const int rounds = auto_rounds ? 0 : args.rounds;
   cptr<int> bin_off = cst(sc->bin_off);
   cptr<int> gen_prog = cst(sc->gen_prog);
   const FieldCtx fctx{sc, bin_off, gen_prog, n_slots};
 
+  // n^2 - sum_o n_o^2 over the occupants of one independent set (block-uniform)
+  auto class_mixed = [&](int cb, int n) -> long long {
+    int cnt[kMaxOcc] = {0, 0, 0, 0};
+    for (int t = tid; t < n; t += nthr) {
+      const int o = img.get(sc->class_addr[cb + t]);
+#pragma unroll
+      for (int q = 0; q < kMaxOcc; ++q) cnt[q] += (o == q) ? 1 : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxOcc; ++q) {
+      const int v = wave_sum(cnt[q]);
+      if (lane == 0) redI[q * n_waves + wave] = v;
+    }
+    __syncthreads();
+    long long sq = 0;
+    for (int q = 0; q < kMaxOcc; ++q) {
+      long long m = 0;
+      for (int w = 0; w < n_waves; ++w) m += redI[q * n_waves + w];
+      sq += m * m;
+    }
+    __syncthreads();  // every lane has read redI before it is written again
+    return (long long)n * n - sq;
+  };
+
   for (int pass = 0; pass < args.n_passes; ++pass) {
     const unsigned long long gpass = pass0 + (unsigned long long)pass;
+    long long m_total = 0;
+    if (KIND == 0 && auto_rounds)
+      for (int c = 0; c < n_classes; ++c)
+        m_total += class_mixed(sc->class_begin[c], sc->class_begin[c + 1] - sc->class_begin[c]);
     for (int c = 0; c < n_classes; ++c) {
       const int b = sc->class_sublat[c];
       const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
@@ -459,6 +488,9 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       } else {
         // ---- canonical: fields of all class sites, then random matchings
         const int n = ce - cb;
+        const int n_rounds =
+            auto_rounds ? auto_canonical_rounds(n, class_mixed(cb, n), sc->class_begin[n_classes], m_total) : rounds;
+        if (n_rounds > 0) {
         if constexpr (BIN && SPLIT == 2) {
           const int part = tid & 1;
           for (int t = tid >> 1; t < n; t += (nthr >> 1)) {
@@ -491,8 +523,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           }
         }
         __syncthreads();
-        for (int rd = 0; rd < rounds; ++rd) {
-          const uint32_t r = uniform_int((uint32_t)n, gpass, (uint32_t)(c * rounds + rd), (uint32_t)chain,
+        for (int rd = 0; rd < n_rounds; ++rd) {
+          const uint32_t r = uniform_int((uint32_t)n, gpass, ((uint32_t)c << 12) | (uint32_t)rd, (uint32_t)chain,
                                          cp->key0, cp->key1);
           for (int p = tid; p < n; p += nthr) {
             int q = (int)r - p;
@@ -529,6 +561,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           }
           __syncthreads();
         }
+        }  // n_rounds > 0
       }
       // ---- halo copies of this class
       for (int t = sc->halo_begin[c] + tid; t < sc->halo_begin[c + 1]; t += nthr) {
@@ -538,10 +571,15 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       __syncthreads();
     }
 
-    // ---- canonical cross-class mixing: one randomly chosen pair phase
-    if (KIND == 0 && sc->n_pair_phases > 0) {
-      const uint32_t ph = uniform_int((uint32_t)sc->n_pair_phases, gpass, 0xFFFFFFF0u, (uint32_t)chain,
-                                      cp->key0, cp->key1);
+    // ---- canonical cross-class mixing: randomly chosen pair phases (one per
+    // pass, or with automatic rounds about n_mutable / 4 pair attempts)
+    const int n_pair_rounds =
+        (KIND == 0 && auto_rounds && sc->n_pair_phases > 0)
+            ? auto_pair_rounds(sc->class_begin[n_classes], sc->pair_begin[sc->n_pair_phases], sc->n_pair_phases)
+            : 1;
+    for (int pr = 0; KIND == 0 && sc->n_pair_phases > 0 && pr < n_pair_rounds; ++pr) {
+      const uint32_t ph = uniform_int((uint32_t)sc->n_pair_phases, gpass, 0xFFFFFFF0u - (uint32_t)pr,
+                                      (uint32_t)chain, cp->key0, cp->key1);
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
         const int2 pq = sc->pair_addr[t];
@@ -622,7 +660,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           const int sa = sc->class_addr[is], ta = sc->class_addr[it];
           if (sa == ta) continue;
           const int bsa = sa / S, bta = ta / S;
-          if (sys->n_occ[bsa] != sys->n_occ[bta]) continue;
+          if (sys->n_occ[bsa] != sys->n_occ[bta] || sys->asym[bsa] != sys->asym[bta]) continue;
           const int os = img.get(sa), ot = img.get(ta);
           if (os == ot) continue;
           float dE1, dE2;

@@ -219,6 +219,7 @@ clexmc_system_desc System::desc() const {
     d.n_bin_segs = N("bin_segs") / 3;
     d.bin_orbit_weight = D("bin_orbit_weight");
   }
+  d.asym = I("asym");  // optional; NULL: units by occupant count
   return d;
 }
 

@@ -111,8 +111,8 @@ struct SamplingFixtureParams {
   Index n_equilibration_passes = 0;
   Index n_passes = 1000;
   int sample_period = 1;
-  int canonical_rounds = 2;
-  int n_pair_phases = 8;
+  int canonical_rounds = 0;  // 0: automatic (about one swap attempt per mutable site and pass)
+  int n_pair_phases = 32;
 };
 
 struct Results {

@@ -41,7 +41,7 @@ class SystemDesc(C.Structure):
         ("entry_site_begin", i32p), ("entry_sites", i32p), ("n_entry_sites", C.c_int),
         ("entry_table", i32p), ("flower_tables", f64p), ("n_flower_tables", C.c_int),
         ("binary", C.c_int), ("bin_seg_begin", i32p), ("bin_segs", i32p), ("n_bin_segs", C.c_int),
-        ("bin_orbit_weight", f64p),
+        ("bin_orbit_weight", f64p), ("asym", i32p),
     ]
 
 
@@ -230,6 +230,8 @@ class Engine:
             d.bin_segs = _p(k(f.bin_segs, np.int32), i32p)
             d.n_bin_segs = len(f.bin_segs)
             d.bin_orbit_weight = _p(k(f.bin_orbit_weight, np.float64), f64p)
+        if getattr(f, "asym", None) is not None:
+            d.asym = _p(k(f.asym, np.int32), i32p)
         self._desc = d
         h = C.c_void_p()
         rc = L.clexmc_create(C.byref(d), C.c_int(device), C.byref(h))
@@ -324,6 +326,10 @@ class Engine:
         return out
 
     def run_passes(self, kind: str, n_passes: int, sample_period: int = 1, canonical_rounds: int = 1) -> float:
+        """Run passes on every loaded chain; returns the kernel time in ms.
+        canonical_rounds: random matchings per independent set and pass, or
+        0 for the automatic count (about one attempt between different
+        occupants per mutable site and pass, as the reference's passes)."""
         k = 0 if kind == "canonical" else 1
         self._check(lib().clexmc_run_passes(self.h, C.c_int(k), C.c_int(n_passes), C.c_int(sample_period),
                                             C.c_int(canonical_rounds)), "clexmc_run_passes")

@@ -596,6 +596,18 @@ class FlatCE:
             self.site_func[b, :p.shape[0], :p.shape[1]] = p
         self.n_corr = bs.n_functions
         self.eci = ce.eci.copy()
+        # asymmetric unit of each sublattice: sublattices mapped onto each other by
+        # the factor group share one (canonical swaps may exchange occupants
+        # between them, as the reference's OccSwap candidates per asym unit do)
+        asym = list(range(prim.n_sublat))
+        for op in bs.ops:
+            for b in range(prim.n_sublat):
+                a, c = asym[b], asym[op.perm[b]]
+                if a != c:
+                    lo, hi = min(a, c), max(a, c)
+                    asym = [lo if x == hi else x for x in asym]
+        relabel = {u: i for i, u in enumerate(sorted(set(asym)))}
+        self.asym = np.array([relabel[u] for u in asym], dtype=np.int32)
 
         # --- correlations (orbit element lists) ---
         orbit_k, orbit_mult, orbit_nfunc, orbit_func_begin = [], [], [], []
@@ -1018,7 +1030,7 @@ def save_flat(flat: FlatCE, path: str) -> None:
         ("flower_begin", flat.flower_begin), ("entry_orbit", flat.entry_orbit), ("entry_k", flat.entry_k),
         ("entry_site_begin", flat.entry_site_begin), ("entry_sites", flat.entry_sites.ravel()),
         ("entry_table", flat.entry_table), ("flower_tables", flat.flower_tables),
-        ("binary", [1 if flat.binary else 0]),
+        ("binary", [1 if flat.binary else 0]), ("asym", flat.asym),
         ("n_components", [len(axes.components)]), ("occ_component", occ_comp.ravel()),
         ("comp_origin", axes.origin), ("comp_R", axes.R.T.ravel()), ("n_independent", [axes.n_independent]),
     ]

@@ -94,6 +94,12 @@ typedef struct {
   const int *bin_segs;           /* [n_segs][3]: orbit, entry_begin, entry_end */
   int n_bin_segs;
   const double *bin_orbit_weight;/* [n_orbits] J_a / m_a */
+  /* asymmetric unit of each sublattice ([n_sublat], may be NULL): canonical
+   * swaps exchange occupants only between sites of one unit, as the
+   * reference's OccSwap candidates (monte/events/OccCandidate.hh) are keyed
+   * by (asym unit, species).  NULL: sublattices with equal occupant counts
+   * share a unit. */
+  const int *asym;
 } clexmc_system_desc;
 
 /* Supercell (diagonal, L1 x L2 x L3 primitive cells) with its independent-set
@@ -108,8 +114,8 @@ typedef struct {
   const int *class_sites;   /* linear site indices */
   /* canonical cross-class mixing: phases of disjoint, mutually
    * non-interacting site pairs (optional; n_pair_phases may be 0); the two
-   * sites of a pair may lie on different sublattices with the same number
-   * of occupants (occupant indices must denote the same species) */
+   * sites of a pair may lie on different sublattices of one asymmetric unit
+   * (occupant indices must denote the same species) */
   int n_pair_phases;
   const int *pair_phase_begin; /* [n_pair_phases+1] in pairs */
   const int *pair_sites;       /* [n_pairs][2] */
@@ -155,7 +161,12 @@ clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out)
  * 1 semi-grand canonical changes.  A sample (formation energy per
  * supercell, occupant counts) is recorded every sample_period passes.
  * canonical_rounds: random matchings per independent set and pass
- * (canonical only; >= 1). */
+ * (canonical only; <= 4096).  0 (automatic): as many matchings as make
+ * about one attempt between different occupants per mutable site and pass
+ * -- the reference's steps per pass -- shared among the sets in proportion
+ * to their pairs of different occupants (from the occupant counts per set
+ * at the start of each pass; at most 1024 per set); a set holding one
+ * occupant only is skipped. */
 clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period,
                                 int canonical_rounds);
 

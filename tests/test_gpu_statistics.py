@@ -64,8 +64,9 @@ def test_sgc_exact_enumeration():
     eng.close()
 
 
-def test_canonical_exact_enumeration():
-    print("[case] canonical exact enumeration 2x2x4", flush=True)
+@pytest.mark.parametrize("rounds", [2, 0])
+def test_canonical_exact_enumeration(rounds):
+    print(f"[case] canonical exact enumeration 2x2x4 rounds {rounds}", flush=True)
     flat = small_flat()
     sc = Supercell(flat.ce.prim, (2, 2, 4))
     T = 1200.0
@@ -76,7 +77,7 @@ def test_canonical_exact_enumeration():
     sid = eng.add_supercell(sc, n_pair_phases=64, seed=5)
     occs = [random_occupation(sc.n_sites, nb / sc.n_sites, 300 + c) for c in range(nch)]
     eng.load_chains([sid] * nch, occs, [T] * nch, 2000 + np.arange(nch))
-    eng.run_passes("canonical", 6000, 1, canonical_rounds=2)
+    eng.run_passes("canonical", 6000, 1, canonical_rounds=rounds)
     E, cnt = eng.read_samples()
     assert np.all(cnt[:, :, 1] == nb)
     _check(_chain_means(E, cnt[:, :, 1], 300), exact, ["E", "E2"])
