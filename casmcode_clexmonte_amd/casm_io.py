@@ -33,13 +33,13 @@ Reference files followed: the fixtures
 run_params_complete,run_params_sgc_complete}.json`` and
 ``tests/unit/clexmonte/data/FCC_binary_vacancy/formation_energy{,_sparse}_eci.json``;
 ECI layout as read by ``src/casm/clexmonte/system/io/json/System_json_io.cc:453-1006``;
-state generation ``src/casm/clexmonte/run/IncrementalConditionsStateGenerator``
+state generation ``include/casm/clexmonte/run/IncrementalConditionsStateGenerator.hh:54-120``
+(parsed by ``include/casm/clexmonte/run/io/json/StateGenerator_json_io_impl.hh:19-200``)
 and composition fixing ``include/casm/clexmonte/state/enforce_composition.hh:47-110``.
 """
 from __future__ import annotations
 
 import json
-import math
 from typing import Dict, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -241,6 +241,15 @@ def enforce_composition(occ: np.ndarray, axes: CompositionAxes, prim: Prim, n_ce
     raise RuntimeError("enforce_composition did not converge")
 
 
+def mol_composition_of_param(axes: CompositionAxes, param_composition) -> np.ndarray:
+    """Mean number of each component per unit cell at a parametric
+    composition: origin + R x."""
+    x = np.asarray(param_composition, float).reshape(-1)
+    if x.size != axes.n_independent:
+        raise ValueError(f"param_composition has {x.size} values, the axes {axes.n_independent}")
+    return axes.origin + (axes.R @ x if axes.n_independent else 0.0)
+
+
 def _axis_vector(values: Union[dict, Sequence[float]], names: Sequence[str]) -> np.ndarray:
     if isinstance(values, dict):
         return np.array([float(values[n]) for n in names])
@@ -252,8 +261,8 @@ def states_from_run_params(run_params, system, method: str, seed: int = 0):
 
     Conditions per state i: initial + i * increment of ``temperature`` and of
     ``mol_composition`` (canonical; a dict by component name) or
-    ``param_chem_pot`` (semi-grand canonical; a dict by axis name ``a``,
-    ``b``, ...).  ``dependent_runs`` true is refused: every state starts from
+    ``param_composition`` (canonical; a dict by axis name ``a``, ``b``, ...)
+    or ``param_chem_pot`` (semi-grand canonical; by axis name).  ``dependent_runs`` true is refused: every state starts from
     the fixed initial configuration (all sites holding their first occupant),
     composition-fixed for canonical states."""
     from .calculator import MonteCarloState
@@ -286,11 +295,17 @@ def states_from_run_params(run_params, system, method: str, seed: int = 0):
         cond = {"temperature": float(c0["temperature"]) + i * float(dc.get("temperature", 0.0))}
         occ = np.zeros(prim.n_sublat * n_cells, dtype=np.int8)
         if method == "canonical":
-            if "mol_composition" not in c0:
-                raise ValueError("canonical run parameters need mol_composition")
-            mol = _axis_vector(c0["mol_composition"], comps)
-            if "mol_composition" in dc:
-                mol = mol + i * _axis_vector(dc["mol_composition"], comps)
+            if "mol_composition" in c0:
+                mol = _axis_vector(c0["mol_composition"], comps)
+                if "mol_composition" in dc:
+                    mol = mol + i * _axis_vector(dc["mol_composition"], comps)
+            elif "param_composition" in c0:
+                x = _axis_vector(c0["param_composition"], axis_names)
+                if "param_composition" in dc:
+                    x = x + i * _axis_vector(dc["param_composition"], axis_names)
+                mol = mol_composition_of_param(axes, x)
+            else:
+                raise ValueError("canonical run parameters need mol_composition or param_composition")
             occ = enforce_composition(occ, axes, prim, n_cells, mol, rng)
             cond["mol_composition"] = axes.mol_composition(occ, n_cells)
             cond["param_composition"] = axes.param_composition(cond["mol_composition"])

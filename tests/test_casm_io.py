@@ -174,3 +174,42 @@ def test_zro_asymmetric_units_match_fixture(zro_ce):
     canonical swaps may exchange O between the two interstitial sublattices."""
     ref = [e["asym_unit"] for e in sorted(ZRO["formation_energy_eci"]["site_functions"], key=lambda e: e["sublat"])]
     assert list(FlatCE(zro_ce).asym) == ref == [0, 0, 1, 1]
+
+
+def test_enforce_composition_reference_values(zro_ce):
+    """The reference's own composition results for the ZrO system
+    (python/tests/functions/test_enforce_composition.py:49,59): a supercell of
+    1029 unit cells holding one O per three cells ([2, 5/3, 1/3]) driven to
+    the target [2, 1.8, 0.2] ends at [2, 1852/1029, 206/1029], the closest
+    reachable composition.  Here the 1029 cells form the diagonal 21x7x7
+    supercell (the composition walk depends only on counts)."""
+    from casmcode_clexmonte_amd.casm_io import enforce_composition
+
+    system = System(zro_ce)
+    axes = system.composition_axes
+    dims = (21, 7, 7)
+    n = 21 * 7 * 7
+    sc = Supercell(zro_ce.prim, dims)
+    occ = np.zeros(sc.n_sites, np.int8)
+    for k in range(7):
+        for j in range(7):
+            for i in range(0, 21, 3):
+                occ[sc.linear_index(2, i, j, k)] = 1
+    np.testing.assert_allclose(axes.mol_composition(occ, n), [2.0, 5.0 / 3.0, 1.0 / 3.0])
+    out = enforce_composition(occ, axes, zro_ce.prim, n, [2.0, 1.8, 0.2], np.random.default_rng(1))
+    np.testing.assert_allclose(axes.mol_composition(out, n), [2.0, 1852 / 1029, 206 / 1029])
+    assert np.all(out[:2 * n] == 0)
+
+
+def test_canonical_state_from_param_composition(zro_ce):
+    """param_composition a = 0.5 from the default (all Va) configuration gives
+    mol_composition [2, 1, 1] (python/tests/canonical/test_CanonicalCalculator.py:131-142)."""
+    system = System(zro_ce)
+    rp = {"state_generation": {"method": "incremental", "kwargs": {
+        "initial_configuration": {"method": "fixed", "kwargs": {
+            "transformation_matrix_to_supercell": [[10, 0, 0], [0, 10, 0], [0, 0, 10]]}},
+        "initial_conditions": {"temperature": 300.0, "param_composition": {"a": 0.5}},
+        "conditions_increment": {"temperature": 0.0}, "n_states": 1, "dependent_runs": False}}}
+    (st,) = states_from_run_params(rp, system, "canonical", seed=2)
+    np.testing.assert_allclose(st.conditions["mol_composition"], [2.0, 1.0, 1.0])
+    np.testing.assert_allclose(st.conditions["param_composition"], [0.5])
