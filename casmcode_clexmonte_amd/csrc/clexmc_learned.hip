@@ -467,6 +467,67 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
       __syncthreads();
     }
 
+    // canonical serial random-pair swaps, as in the plain kernel: two
+    // uniformly random mutable sites of one asymmetric unit anywhere in the
+    // supercell, evaluated by one lane as two sequential flips with every
+    // periodic copy updated.  Group proposals never leave a class and pair
+    // phases may be empty on small cells; these keep every cell ergodic.
+    if (KIND == 0 && args.global_swaps > 0) {
+      if (tid == 0) {
+        const int n_var = sc->class_begin[n_classes];
+        auto set_all = [&](int a, int nw) {
+          const int b = a / S;
+          const int r = a - b * S;
+          const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
+          const int P2 = P12 / P1, P3 = S / P12;
+          const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
+          for (int dk = -m3; dk <= m3; ++dk) {
+            const int qk = pk + dk * L3;
+            if (qk < 0 || qk >= P3) continue;
+            for (int dj = -m2; dj <= m2; ++dj) {
+              const int qj = pj + dj * L2;
+              if (qj < 0 || qj >= P2) continue;
+              for (int di = -m1; di <= m1; ++di) {
+                const int qi = pi + di * L1;
+                if (qi < 0 || qi >= P1) continue;
+                img[b * S + qk * P12 + qj * P1 + qi] = (uint8_t)nw;
+              }
+            }
+          }
+        };
+        for (int g = 0; g < args.global_swaps; ++g) {
+          const uint4 ug = philox4x32_10(
+              make_uint4(0xFFFFFFFEu, (uint32_t)gpass, (uint32_t)g, (uint32_t)chain ^ 0x5bd1e995u), k0, k1);
+          const int is = min((int)(u01(ug.x) * (float)n_var), n_var - 1);
+          const int it = min((int)(u01(ug.y) * (float)n_var), n_var - 1);
+          const int sa = sc->class_addr[is], ta = sc->class_addr[it];
+          if (sa == ta) continue;
+          const int bsa = sa / S, bta = ta / S;
+          if (sys->n_occ[bsa] != sys->n_occ[bta] || sys->asym[bsa] != sys->asym[bta]) continue;
+          const int os = img[sa], ot = img[ta];
+          if (os == ot) continue;
+          int Ks[kMaxSlots], Kt[kMaxSlots];
+          const float dE1 = (os ? -2.0f : 2.0f) * binary_field_l(img, sa, os, cst(sc->bin_sub + bsa), bin_off,
+                                                                  n_slots, Ks);
+          set_all(sa, ot);
+          const float dE2 = (ot ? -2.0f : 2.0f) * binary_field_l(img, ta, ot, cst(sc->bin_sub + bta), bin_off,
+                                                                  n_slots, Kt);
+          const float dE = dE1 + dE2;
+          accAtt += 1;
+          if (dE <= 0.0f || u01(ug.z) < __expf(-beta * dE)) {
+            set_all(ta, os);
+            accAcc += 1;
+            const int ss = os ? -2 : 2, st = ot ? -2 : 2;
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += ss * Ks[a] + st * Kt[a];
+          } else {
+            set_all(sa, os);
+          }
+        }
+      }
+      __syncthreads();
+    }
+
     // reduction and sampling
     const bool sample_due = ((gpass + 1) % (unsigned long long)args.sample_period) == 0ull;
     const bool flush = sample_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);

@@ -678,6 +678,36 @@ int orc_run_batch(const orc_system *s, int n_chains, int8_t *occ, const double *
 /* ------------------------------------------------------------------ */
 /* Exact enumeration on tiny supercells                                 */
 /* ------------------------------------------------------------------ */
+int orc_enumerate_energies(const orc_system *s, int n_occ_max, const int *fixed_counts, double *E_out,
+                           long long n_states, long long st0, long long st1) {
+  const long nc = n_cells(s);
+  const long N = nc * s->n_sublat;
+  long long n = 1;
+  for (long l = 0; l < N; ++l) {
+    n *= s->n_occ[l / nc];
+    if (n > (1LL << 26)) return -1;
+  }
+  if (n != n_states || st0 < 0 || st1 > n_states || st0 > st1) return -2;
+  int8_t *occ = (int8_t *)calloc(N, 1);
+  for (long long st = st0; st < st1; ++st) {
+    long long x = st;
+    int cnt[16] = {0};
+    for (long l = 0; l < N; ++l) {
+      int r = s->n_occ[l / nc];
+      occ[l] = (int8_t)(x % r);
+      x /= r;
+      if (r >= 2) cnt[occ[l]]++;
+    }
+    int ok = 1;
+    if (fixed_counts)
+      for (int o = 0; o < n_occ_max; ++o)
+        if (cnt[o] != fixed_counts[o]) ok = 0;
+    E_out[st] = ok ? orc_energy(s, occ) : NAN;
+  }
+  free(occ);
+  return 0;
+}
+
 int orc_enumerate(const orc_system *s, int kind, double temperature, double kb, const double *chem,
                   int n_occ_max, const int *fixed_counts, double *avg, double *probs, long long n_probs) {
   const long nc = n_cells(s);

@@ -114,6 +114,14 @@ int orc_enumerate(const orc_system *s, int kind, double temperature, double kb,
                   const double *chem, int n_occ_max, const int *fixed_counts, double *avg,
                   double *probs, long long n_probs);
 
+/* Energy of every state in orc_enumerate's order (site 0 fastest, radix
+ * n_occ of its sublattice); with fixed_counts != NULL, states whose
+ * variable-site occupant counts differ get NaN.  n_states must equal the
+ * number of states; only states [st0, st1) are written (E_out indexed by
+ * state).  Returns 0, or -1 (too many states) / -2 (size or range). */
+int orc_enumerate_energies(const orc_system *s, int n_occ_max, const int *fixed_counts, double *E_out,
+                           long long n_states, long long st0, long long st1);
+
 /* Philox4x32-10 */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* mt19937_64: returns the n-th output (1-based) for seed */

@@ -209,6 +209,34 @@ class OracleSystem:
             out["probs"] = probs
         return out
 
+    def state_energies(self, fixed_counts=None, n_threads: int = 8) -> np.ndarray:
+        """Energy of every state in enumerate()'s order (site 0 fastest);
+        NaN for states off fixed_counts."""
+        nom = int(self.flat.n_occ_max)
+        nst = 1
+        for b in range(self.flat.n_sublat):
+            nst *= int(self.flat.n_occ[b]) ** (self.dims[0] * self.dims[1] * self.dims[2])
+        E = np.zeros(nst)
+        fc_p = i32p()
+        if fixed_counts is not None:
+            fc = np.zeros(16, dtype=np.int32)
+            fc[:len(fixed_counts)] = fixed_counts
+            self._fc = fc
+            fc_p = _p(fc, i32p)
+        n_chunks = 16
+        bounds = [nst * i // n_chunks for i in range(n_chunks + 1)]
+
+        def run(i):
+            return lib().orc_enumerate_energies(C.byref(self.s), C.c_int(nom), fc_p, _p(E, f64p), C.c_longlong(nst),
+                                                C.c_longlong(bounds[i]), C.c_longlong(bounds[i + 1]))
+
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=n_threads) as ex:
+            rcs = list(ex.map(run, range(n_chunks)))
+        if any(rcs):
+            raise RuntimeError(f"orc_enumerate_energies failed rc={rcs}")
+        return E
+
 
 def philox4x32_10(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)

@@ -374,3 +374,24 @@ def test_engine_without_gpu_fails_loudly():
     env["ROCR_VISIBLE_DEVICES"] = ""
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120)
     assert "raised" in out.stdout, out.stdout + out.stderr
+
+
+def test_state_energies_follow_enumeration_order():
+    """orc_enumerate_energies: state index st holds the energy of the
+    occupation with site l = digit l of st (radix 2 here), NaN off the fixed
+    counts; its Boltzmann average equals orc_enumerate's."""
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    osys = O.OracleSystem(flat, (2, 2, 2))
+    E = osys.state_energies()
+    rng = np.random.default_rng(3)
+    for st in rng.integers(0, 256, 12):
+        occ = ((int(st) >> np.arange(8)) & 1).astype(np.int8)
+        assert E[st] == osys.energy(occ)
+    Ec = osys.state_energies(fixed_counts=[5, 3])
+    nb = np.array([bin(i).count("1") for i in range(256)])
+    assert np.all(np.isnan(Ec) == (nb != 3))
+    T = 800.0
+    w = np.exp(-(Ec[nb == 3] - Ec[nb == 3].min()) / (8.6173303e-05 * T))
+    ex = osys.enumerate("canonical", T, fixed_counts=[5, 3])
+    assert abs((w * Ec[nb == 3]).sum() / w.sum() - ex["E"]) < 1e-12
