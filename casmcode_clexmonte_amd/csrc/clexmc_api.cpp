@@ -86,6 +86,7 @@ struct clexmc_handle {
   bool last_packed = false;
   int last_specialized = 0, last_threads = 0, last_lanes = 1;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
+  int row_pitch = -1;               // padded image row pitch: -1 automatic, else extra bytes per row
   int n_slots = 0;                  // binary path: orbits with >= 1 site
   int lanes_per_site = 0;           // 0 automatic (1; the autotuner may pick 2), 1, 2
   int specialize = 1;               // run-time specialised binary kernels (1 when possible, 0 never)
@@ -500,6 +501,79 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
   return CLEXMC_OK;
 }
 
+// Row pitch of the padded int8 image that minimises modelled LDS bank
+// conflicts of the neighbour gathers: for the first four waves (64
+// consecutive class sites) of every class and every distinct neighbour
+// offset of the class's sublattice, each half wave (32 lanes) costs as many
+// cycles as the largest number of distinct dwords that fall into one of 32
+// four-byte banks.  (This model ranks measured pitches of the FCC 24^3 image
+// correctly: 28, 30 and 38 bytes slow, 32 fast; a 64-bank full-wave model
+// does not.)  Candidates add 0 .. max(4, P1/4) bytes per row (at most 32)
+// and keep the image within 25 % of its unpadded size; ties keep the smaller
+// pitch.  Images above 64 KiB (which may need the packed format to fit in
+// LDS) keep the unpadded pitch.
+static int choose_row_pitch(const clexmc_handle *h, int L1, int L2, int L3, int hw, int P2, int P3,
+                            const clexmc_supercell_desc *sc) {
+  const int P1_min = L1 + 2 * hw;
+  if ((long long)P1_min * P2 * P3 * h->n_sublat > 65536) return P1_min;
+  const int n_cells = L1 * L2 * L3;
+  // distinct neighbour offsets (sublattice, di, dj, dk) per home sublattice
+  std::vector<std::vector<std::array<int, 4>>> offs(h->n_sublat);
+  for (int b = 0; b < h->n_sublat; ++b) {
+    std::set<std::array<int, 4>> u;
+    for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e)
+      for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q)
+        u.insert({h->entry_sites[4 * q], h->entry_sites[4 * q + 1], h->entry_sites[4 * q + 2], h->entry_sites[4 * q + 3]});
+    offs[b].assign(u.begin(), u.end());
+  }
+  const int max_extra = std::min(32, std::max(4, P1_min / 4));
+  int best_p1 = P1_min;
+  long long best_cost = -1;
+  std::vector<long long> dw(64);
+  std::vector<int> bank_count(32);
+  for (int extra = 0; extra <= max_extra; ++extra) {
+    const int P1 = P1_min + extra;
+    if ((long long)P1 * 4 > (long long)P1_min * 5) break;
+    const long long P12 = (long long)P1 * P2, S = P12 * P3;
+    long long cost = 0;
+    for (int c = 0; c < sc->n_classes; ++c) {
+      const int t0 = sc->class_begin[c], t1 = std::min(sc->class_begin[c + 1], t0 + 256);
+      for (int w0 = t0; w0 < t1; w0 += 64) {
+        const int nl = std::min(64, t1 - w0);
+        const int l0 = sc->class_sites[w0];
+        const int b = l0 / n_cells;
+        for (const auto &o : offs[b]) {
+          for (int t = 0; t < nl; ++t) {
+            const int l = sc->class_sites[w0 + t];
+            const int cc = l % n_cells;
+            const int i = cc % L1, j = (cc / L1) % L2, k = cc / (L1 * L2);
+            const long long a =
+                o[0] * S + (long long)(k + o[3] + hw) * P12 + (long long)(j + o[2] + hw) * P1 + (i + o[1] + hw);
+            dw[t] = a >> 2;
+          }
+          // per half wave: largest number of distinct dwords in one bank
+          for (int h0 = 0; h0 < nl; h0 += 32) {
+            const int h1 = std::min(nl, h0 + 32);
+            std::sort(dw.begin() + h0, dw.begin() + h1);
+            std::fill(bank_count.begin(), bank_count.end(), 0);
+            int worst = 0;
+            for (int t = h0; t < h1; ++t) {
+              if (t > h0 && dw[t] == dw[t - 1]) continue;
+              worst = std::max(worst, ++bank_count[(int)(dw[t] & 31)]);
+            }
+            cost += worst;
+          }
+        }
+      }
+    }
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best_p1 = P1;
+    }
+  }
+  return best_p1;
+}
+
 clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc *sc, int *id_out) {
   if (!h || !sc || !id_out) return CLEXMC_ERR_ARG;
   HIP_TRY(hipSetDevice(h->device));
@@ -512,17 +586,36 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   d.L2 = L2;
   d.L3 = L3;
   d.h = hw;
-  d.P1 = L1 + 2 * hw;
   d.P2 = L2 + 2 * hw;
   d.P3 = L3 + 2 * hw;
+  d.n_cells = L1 * L2 * L3;
+  d.n_sites = d.n_cells * h->n_sublat;
+  const int N = d.n_sites;
+  // classes (given, or computed here when n_classes == 0)
+  clexmc_supercell_desc auto_desc;
+  std::vector<int> a_cb, a_cs, a_pb, a_ps;
+  if (sc->n_classes == 0) {
+    const int L[3] = {L1, L2, L3};
+    clexmc_status pst = auto_partition(h, L, sc->n_pair_phases, 0x5eedull + h->scs.size(), a_cb, a_cs, a_pb, a_ps);
+    if (pst != CLEXMC_OK) return pst;
+    auto_desc = *sc;
+    auto_desc.n_classes = (int)a_cb.size() - 1;
+    auto_desc.class_begin = a_cb.data();
+    auto_desc.class_sites = a_cs.data();
+    auto_desc.n_pair_phases = (int)a_pb.size() - 1;
+    auto_desc.pair_phase_begin = a_pb.data();
+    auto_desc.pair_sites = a_ps.data();
+    sc = &auto_desc;
+  }
+  // row pitch of the padded image: L1 + 2h, plus unused bytes per row when
+  // that spreads the class sites' neighbour gathers over more LDS banks
+  d.P1 = L1 + 2 * hw + (h->row_pitch > 0 ? h->row_pitch : 0);
+  if (h->row_pitch < 0) d.P1 = choose_row_pitch(h, L1, L2, L3, hw, d.P2, d.P3, sc);
   const long long S = (long long)d.P1 * d.P2 * d.P3;
   const long long pad = S * h->n_sublat;
   ARG_CHECK(pad < (1LL << 30), "padded image too large");
   d.S = (int)S;
   d.pad_bytes = (int)pad;
-  d.n_cells = L1 * L2 * L3;
-  d.n_sites = d.n_cells * h->n_sublat;
-  const int N = d.n_sites;
   auto pad_addr = [&](int b, int i, int j, int k) {
     return (int)(b * S + (long long)(k + hw) * d.P1 * d.P2 + (long long)(j + hw) * d.P1 + (i + hw));
   };
@@ -559,22 +652,6 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
           if (a != main) out.push_back(a);
         }
   };
-  // classes (given, or computed here when n_classes == 0)
-  clexmc_supercell_desc auto_desc;
-  std::vector<int> a_cb, a_cs, a_pb, a_ps;
-  if (sc->n_classes == 0) {
-    const int L[3] = {L1, L2, L3};
-    clexmc_status pst = auto_partition(h, L, sc->n_pair_phases, 0x5eedull + h->scs.size(), a_cb, a_cs, a_pb, a_ps);
-    if (pst != CLEXMC_OK) return pst;
-    auto_desc = *sc;
-    auto_desc.n_classes = (int)a_cb.size() - 1;
-    auto_desc.class_begin = a_cb.data();
-    auto_desc.class_sites = a_cs.data();
-    auto_desc.n_pair_phases = (int)a_pb.size() - 1;
-    auto_desc.pair_phase_begin = a_pb.data();
-    auto_desc.pair_sites = a_ps.data();
-    sc = &auto_desc;
-  }
   ARG_CHECK(sc->n_classes >= 1, "no independent-set classes");
   std::vector<int> cb(sc->class_begin, sc->class_begin + sc->n_classes + 1);
   ARG_CHECK(cb[0] == 0, "class_begin[0] must be 0");
@@ -992,6 +1069,13 @@ clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(per_pass >= 0 && per_pass <= 65536, "global swaps per pass out of range");
   h->global_swaps = per_pass;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(extra >= -1 && extra <= 64, "row pitch padding must be -1 (automatic) or 0..64");
+  h->row_pitch = extra;
   return CLEXMC_OK;
 }
 

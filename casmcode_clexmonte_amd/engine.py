@@ -72,6 +72,7 @@ def lib():
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
+            "clexmc_set_lds_row_pitch",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
@@ -93,6 +94,7 @@ def exported_symbols() -> List[str]:
             "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
             "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
+            "clexmc_set_lds_row_pitch",
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
@@ -454,6 +456,11 @@ class Engine:
     def set_image_packing(self, mode: str):
         code = {"auto": 0, "int8": 1, "packed2": 2}[mode]
         self._check(lib().clexmc_set_image_packing(self.h, C.c_int(code)), "clexmc_set_image_packing")
+
+    def set_lds_row_pitch(self, extra: int = -1):
+        """Padded-image row pitch of supercells added later: -1 automatic
+        (bank-conflict model), or a fixed number of extra bytes per row."""
+        self._check(lib().clexmc_set_lds_row_pitch(self.h, C.c_int(extra)), "clexmc_set_lds_row_pitch")
 
     def set_global_swaps(self, per_pass: int):
         self._check(lib().clexmc_set_global_swaps(self.h, C.c_int(per_pass)), "clexmc_set_global_swaps")

@@ -232,6 +232,15 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
  * supercell ergodic, including ones too small for non-interacting pairs. */
 clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
 
+/* Row pitch of the padded LDS occupation image of supercells added after
+ * this call: -1 (default) chooses, per supercell, the pitch L1 + 2h + x
+ * (x <= max(4, (L1 + 2h)/4), image growth <= 25 %, images up to 64 KiB)
+ * that minimises modelled LDS bank conflicts of the classes' neighbour
+ * gathers; 0..64 fixes x.  The
+ * extra bytes of each row are never read; results do not depend on the
+ * pitch.  No reference counterpart (memory layout of this build). */
+clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
+
 /* Run-time specialisation (hipRTC, gfx950) of the binary Metropolis kernel
  * to the loaded supercell: constant site offsets instead of interpreted site
  * programs; used when every chain shares one supercell.  1 (default): when

@@ -195,3 +195,29 @@ def test_grouped_shape_launches_bit_identical_to_one_launch():
     (s0, E0, n0), (s1, E1, n1) = runs
     _same(s0, s1)
     assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
+
+
+@pytest.mark.parametrize("which", ["binary_sgc", "ternary_canonical"])
+@pytest.mark.parametrize("specialize", [1, 0])
+def test_lds_row_pitch_does_not_change_results(which, specialize):
+    """The padded image's row pitch (automatic bank-conflict choice, a fixed
+    odd padding, none) changes only the LDS layout: every chain ends in the
+    same state with the same samples."""
+    _print_case(f"row pitch identity {which} specialize {specialize}")
+    flat, sc, occs, temps, chem, kind, pp = _setup(which)
+    out = []
+    for pitch in (0, -1, 3):
+        eng = Engine(flat)
+        eng.set_specialization(specialize)
+        eng.set_lds_row_pitch(pitch)
+        sid = eng.add_supercell(sc, n_pair_phases=pp, seed=3)
+        eng.load_chains([sid] * len(occs), occs, temps, 40 + np.arange(len(occs)), chem=chem)
+        eng.run_passes(kind, 12, 1, canonical_rounds=2)
+        E, cnt = eng.read_samples()
+        states = [eng.read_state(c) for c in range(len(occs))]
+        out.append((E, cnt, [s["occ"] for s in states], [s["E"] for s in states]))
+        eng.close()
+    for E, cnt, occ, e in out[1:]:
+        assert np.array_equal(E, out[0][0]) and np.array_equal(cnt, out[0][1])
+        assert all(np.array_equal(a, b) for a, b in zip(occ, out[0][2]))
+        assert e == out[0][3]
