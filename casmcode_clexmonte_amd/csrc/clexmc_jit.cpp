@@ -3,6 +3,7 @@
 
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -11,6 +12,45 @@
 #include "clexmc_jit_src.inc"  // kJitHeaderNames / kJitHeaderTexts: the device headers as text
 
 namespace clexmc {
+
+// Lane-pair split of one sublattice's binary site program: every orbit slot
+// goes wholly to one lane of the pair, choosing among all 2^n assignments the
+// one whose busier lane has the least work, counted as 2 per distinct LDS
+// neighbour read plus 1 per XOR / add, plus 48 on lane 0 for the proposal's
+// random number, acceptance and update.  Orbits whose clusters share
+// neighbours (pairs and the triplets built on them) can then stay on one
+// lane, so the pair reads each neighbour about once.
+static std::vector<int> split_slots(const BinSub &B, const std::vector<int> &bin_off, int n_slots) {
+  std::vector<int> part(kMaxSlots, 0);
+  std::vector<int> active;
+  for (int j = 0; j < n_slots; ++j)
+    if (B.slot_n[j] > 0 && B.slot_k[j] > 0) active.push_back(j);
+  const int na = (int)active.size();
+  if (na == 0 || na > 16) return part;
+  long best = -1;
+  int best_mask = 0;
+  for (int mask = 0; mask < (1 << na); ++mask) {
+    long cost[2] = {48, 0};
+    std::vector<int> offs[2];
+    for (int q = 0; q < na; ++q) {
+      const int j = active[q], p = (mask >> q) & 1;
+      const int n = B.slot_n[j], ko = B.slot_k[j];
+      cost[p] += (long)n * ko;
+      for (int e = 0; e < n * ko; ++e) offs[p].push_back(bin_off[B.slot_begin[j] + e]);
+    }
+    for (int p = 0; p < 2; ++p) {
+      std::sort(offs[p].begin(), offs[p].end());
+      cost[p] += 2 * (long)(std::unique(offs[p].begin(), offs[p].end()) - offs[p].begin());
+    }
+    const long c = std::max(cost[0], cost[1]);
+    if (best < 0 || c < best) {
+      best = c;
+      best_mask = mask;
+    }
+  }
+  for (int q = 0; q < na; ++q) part[active[q]] = (best_mask >> q) & 1;
+  return part;
+}
 
 static std::string hexf(float v) {
   char buf[64];
@@ -61,7 +101,7 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     o << "      return F;\n    }\n";
   }
   o << "    return F;\n  }\n";
-  // lane-pair partial sums (clusters of each orbit split in halves) and the finish
+  // lane-pair partial sums (orbit slots split between the two lanes) and the finish
   o << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static void bin_x(const FieldCtx &, const IMG &img, int b, int pa, int o, int part,\n"
     << "                                               int (&X)[kMaxSlots]) {\n"
@@ -74,13 +114,13 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     for (int j = 0; j < n_slots; ++j)
       for (int e = 0; e < B.slot_n[j] * B.slot_k[j]; ++e) minoff = std::min(minoff, bin_off[B.slot_begin[j] + e]);
     o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    const std::vector<int> slot_part = split_slots(B, bin_off, n_slots);
     for (int part = 0; part < 2; ++part) {
       o << "      if (part == " << part << ") {\n";
       for (int j = 0; j < n_slots; ++j) {
         const int n = B.slot_n[j], ko = B.slot_k[j];
-        if (n <= 0 || ko <= 0) continue;
-        const int e0 = part == 0 ? 0 : (n + 1) / 2, e1 = part == 0 ? (n + 1) / 2 : n;
-        for (int e = e0; e < e1; ++e) {
+        if (n <= 0 || ko <= 0 || slot_part[j] != part) continue;
+        for (int e = 0; e < n; ++e) {
           o << "        X[" << j << "] += ";
           for (int i = 0; i < ko; ++i) {
             if (i) o << " ^ ";

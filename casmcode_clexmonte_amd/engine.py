@@ -72,7 +72,7 @@ def lib():
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
-            "clexmc_set_lds_row_pitch",
+                     "clexmc_set_lds_row_pitch",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
