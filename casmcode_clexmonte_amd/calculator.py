@@ -20,6 +20,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import analysis as A
+from .completion import CompletionCheckParams, check as completion_check
 from .engine import Engine
 from .system import KB, ClusterExpansion, CompositionAxes, FlatCE, Supercell
 
@@ -49,6 +50,10 @@ class MonteCarloState:
 
 @dataclass
 class SamplingFixtureParams:
+    """Sampling fixture: samples every `sample_period` passes; either a fixed
+    number of passes (`n_passes`, after `n_equilibration_passes`) or, with
+    `completion`, the reference's completion checks (cutoffs, equilibration,
+    requested precision; see completion.py)."""
     label: str = "thermo"
     sampler_names: List[str] = field(default_factory=lambda: [
         "clex.formation_energy", "potential_energy", "mol_composition", "param_composition"])
@@ -57,6 +62,22 @@ class SamplingFixtureParams:
     sample_period: int = 1
     canonical_rounds: int = 0       # 0: automatic, about one swap attempt per mutable site and pass
     n_pair_phases: int = 32
+    completion: Optional[CompletionCheckParams] = None
+
+    @classmethod
+    def from_casm(cls, label: str, d: dict, **kw) -> "SamplingFixtureParams":
+        """From a CASM sampling fixture (``sampling``: ``sample_by`` "pass",
+        linear ``period``, ``quantities``; ``completion_check``), as in the
+        reference's run params (``sampling_fixtures.<label>``)."""
+        smp = d.get("sampling", {})
+        if smp.get("sample_by", "pass") != "pass" or smp.get("spacing", "linear") != "linear":
+            raise ValueError("only linear sampling by pass is supported")
+        names = []
+        for q in smp.get("quantities", []):
+            names.append("clex.formation_energy" if q == "formation_energy" else q)
+        return cls(label=label, sampler_names=names or cls().sampler_names,
+                   sample_period=int(smp.get("period", 1)), n_equilibration_passes=0,
+                   completion=CompletionCheckParams.from_casm(d.get("completion_check", {})), **kw)
 
 
 class MontePotential:
@@ -163,68 +184,122 @@ class MonteCalculator:
             self.engine.reanchor_energies()
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
         acc0 = [self.engine.read_state(c)["n_accept"] for c in range(len(states))]
-        self.engine.run_passes(kind, p.n_passes, p.sample_period, p.canonical_rounds)
-        self.engine.reanchor_energies()
-        elapsed = time.perf_counter() - t0
-        E, cnt = self.engine.read_samples()
+        if p.completion is None:
+            self.engine.run_passes(kind, p.n_passes, p.sample_period, p.canonical_rounds)
+            self.engine.reanchor_energies()
+            E, cnt = self.engine.read_samples()
+            ends = [(E.shape[1], None, self.engine.read_state(c), time.perf_counter() - t0)
+                    for c in range(len(states))]
+        else:
+            E, cnt, ends = self._run_to_completion(states, p, t0)
         out = []
         for c, s in enumerate(states):
-            st = self.engine.read_state(c)
+            n_s, res, st, elapsed = ends[c]
             s.occupation = st["occ"]
-            out.append(self._summary(s, E[c], cnt[c], st["n_attempt"] - att0[c], st["n_accept"] - acc0[c],
-                                     elapsed))
+            out.append(self._summary(s, E[c, :n_s], cnt[c, :n_s], st["n_attempt"] - att0[c],
+                                     st["n_accept"] - acc0[c], elapsed, res, p))
         return out
 
+    def _run_to_completion(self, states, p: SamplingFixtureParams, t0: float):
+        cc = p.completion
+        if cc.max_count is None and not cc.convergence:
+            raise ValueError("Error in MonteCalculator: a completion check needs a count cutoff or "
+                             "requested convergence")
+        kind = self.method
+        n = len(states)
+        ends = [None] * n
+        count = 0
+        hard_cap = int(cc.max_count) if cc.max_count is not None else 10 ** 8
+        while any(e is None for e in ends):
+            target = min(cc.next_check(count), hard_cap)
+            self.engine.run_passes(kind, target - count, p.sample_period, p.canonical_rounds)
+            count = target
+            E, cnt = self.engine.read_samples()
+            for c in range(n):
+                if ends[c] is not None:
+                    continue
+                series = self._series(states[c], E[c], cnt[c])
+                res = completion_check(series, count, cc)
+                if res.complete or count >= hard_cap:
+                    self.engine.reanchor_energies()
+                    ends[c] = (E.shape[1], res, self.engine.read_state(c), time.perf_counter() - t0)
+        E, cnt = self.engine.read_samples()
+        return E, cnt, ends
+
     # -- results -----------------------------------------------------------------
-    def _summary(self, state, E_sc, counts, n_attempt, n_accept, elapsed) -> dict:
+    def _series(self, state, E_sc, counts) -> Dict[str, np.ndarray]:
+        """Per-sample values of the samplers, per unit cell: clex.formation_energy,
+        potential_energy, mol_composition.<component>, param_composition.<axis>."""
         axes = self.system.composition_axes
         flat = self.system.flat
         n = int(np.prod(state.supercell))
-        T = float(state.conditions["temperature"])
         comps = axes.components
-        # mol composition per unit cell from occupant counts (variable
-        # sublattices share one occupant list) plus fixed sublattices
         mol = np.zeros((len(E_sc), len(comps)))
         for b in range(flat.n_sublat):
             if flat.n_occ[b] < 2:
                 mol[:, axes.occ_component[b][0]] += 1.0
         bvar = next(b for b in range(flat.n_sublat) if flat.n_occ[b] >= 2)
         for o, c in enumerate(axes.occ_component[bvar]):
-            mol[:, c] += counts[:, o] / n
+            mol[:, c] += np.asarray(counts)[:, o] / n
         x = np.array([axes.param_composition(m) for m in mol]).reshape(len(E_sc), axes.n_independent)
-        e = np.asarray(E_sc) / n
+        e = np.asarray(E_sc, dtype=float) / n
         if self.method == "semigrand_canonical":
-            mu = np.asarray(state.conditions["param_chem_pot"], float)
-            pe = e - x @ mu
+            pe = e - x @ np.asarray(state.conditions["param_chem_pot"], float)
         else:
             pe = e.copy()
+        out = {"clex.formation_energy": e, "potential_energy": pe}
+        for i, k in enumerate(comps):
+            out[f"mol_composition.{k}"] = mol[:, i]
+        for i in range(axes.n_independent):
+            out[f"param_composition.{chr(ord('a') + i)}"] = x[:, i]
+        return out
+
+    def _summary(self, state, E_sc, counts, n_attempt, n_accept, elapsed, res=None,
+                 p: Optional[SamplingFixtureParams] = None) -> dict:
+        axes = self.system.composition_axes
+        n = int(np.prod(state.supercell))
+        T = float(state.conditions["temperature"])
+        comps = axes.components
+        ser = self._series(state, E_sc, counts)
+        n_eq = res.n_equilibrate if res is not None else 0
+        z = p.completion.z() if (p is not None and p.completion is not None) else 2.0
+        for k in ser:
+            ser[k] = ser[k][n_eq:]
+        e, pe = ser["clex.formation_energy"], ser["potential_energy"]
+        mol = np.stack([ser[f"mol_composition.{k}"] for k in comps], axis=1)
         var_names = [chr(ord("a") + i) for i in range(axes.n_independent)]
-        samplers = {"clex.formation_energy": (e[:, None], None), "potential_energy": (pe[:, None], None),
-                    "mol_composition": (mol, comps), "param_composition": (x, var_names)}
-        stats = {}
-        for name, (arr, cn) in samplers.items():
-            if cn is None:
-                m, err = A.mean_and_error(arr[:, 0])
-                stats[name] = {"shape": [], "value": {"mean": [m], "calculated_precision": [2.0 * err],
-                                                      "is_converged": [True]}}
-            else:
-                d = {"component_names": list(cn), "shape": [len(cn)]}
-                for i, k in enumerate(cn):
-                    m, err = A.mean_and_error(arr[:, i])
-                    d[k] = {"mean": [m], "calculated_precision": [2.0 * err], "is_converged": [True]}
-                stats[name] = d
+        x = np.stack([ser[f"param_composition.{k}"] for k in var_names], axis=1) if var_names \
+            else np.zeros((len(e), 0))
+        requested = res.quantities if res is not None else {}
+
+        def stat(name, arr):
+            m, err = A.mean_and_error(arr)
+            d = {"mean": [m], "calculated_precision": [z * err]}
+            if name in requested:
+                d["is_converged"] = [requested[name]["is_converged"]]
+            return d
+
+        stats = {"clex.formation_energy": {"shape": [], "value": stat("clex.formation_energy", e)},
+                 "potential_energy": {"shape": [], "value": stat("potential_energy", pe)}}
+        for name, arr, cn in (("mol_composition", mol, comps), ("param_composition", x, var_names)):
+            d = {"component_names": list(cn), "shape": [len(cn)]}
+            for i, k in enumerate(cn):
+                d[k] = stat(f"{name}.{k}", arr[:, i])
+            stats[name] = d
         analysis = {"heat_capacity": {"shape": [], "value": [A.heat_capacity(pe, T, n)]}}
         if self.method == "semigrand_canonical":
-            def cov_block(a, b, an, bn, name, const):
+            mu = np.asarray(state.conditions["param_chem_pot"], float)
+
+            def cov_block(a, b, an, bn, name):
                 d = {"component_names": [f"{i},{j}" for j in bn for i in an], "shape": [len(an), len(bn)]}
                 for jj, j in enumerate(bn):
                     for ii, i in enumerate(an):
-                        d[f"{i},{j}"] = [A.covariance(a[:, ii], b[:, jj]) * n / (KB * T * const)]
+                        d[f"{i},{j}"] = [A.covariance(a[:, ii], b[:, jj]) * n / (KB * T)]
                 analysis[name] = d
-            cov_block(mol, mol, comps, comps, "mol_susc", 1.0)
-            cov_block(x, x, var_names, var_names, "param_susc", 1.0)
-            cov_block(pe[:, None], mol, ["S"], comps, "mol_thermochem_susc", 1.0)
-            cov_block(pe[:, None], x, ["S"], var_names, "param_thermochem_susc", 1.0)
+            cov_block(mol, mol, comps, comps, "mol_susc")
+            cov_block(x, x, var_names, var_names, "param_susc")
+            cov_block(pe[:, None], mol, ["S"], comps, "mol_thermochem_susc")
+            cov_block(pe[:, None], x, ["S"], var_names, "param_thermochem_susc")
             conditions = {"temperature": {"shape": [], "value": [T]},
                           "param_chem_pot": {"component_names": var_names, "shape": [len(var_names)],
                                              **{k: [float(v)] for k, v in zip(var_names, mu)}}}
@@ -236,10 +311,11 @@ class MonteCalculator:
                           "param_composition": {"component_names": var_names, "shape": [len(var_names)],
                                                 **{k: [float(v)] for k, v in
                                                    zip(var_names, axes.param_composition(mol0))}}}
-        ns = len(e)
-        completion = {"N_samples": [ns], "N_samples_for_statistics": [ns],
-                      "N_samples_for_all_to_equilibrate": [0], "all_converged": [True],
-                      "all_equilibrated": [True],
+        ns = len(E_sc)
+        completion = {"N_samples": [ns], "N_samples_for_statistics": [ns - n_eq],
+                      "N_samples_for_all_to_equilibrate": [n_eq],
+                      "all_converged": [bool(res.all_converged) if res is not None else True],
+                      "all_equilibrated": [bool(res.all_equilibrated) if res is not None else True],
                       "acceptance_rate": [float(n_accept / n_attempt) if n_attempt else float("nan")],
                       "count": [int(n_attempt)], "elapsed_clocktime": [elapsed]}
         return {"conditions": conditions, "statistics": stats, "analysis": analysis,
@@ -260,3 +336,52 @@ def merge_summaries(summaries: Sequence[dict]) -> dict:
     for s in summaries[1:]:
         out = merge(out, s)
     return out
+
+
+def run_series(calculator: MonteCalculator, run_params, output_dir: Optional[str] = None, seed: int = 0,
+               fixture: str = "thermo") -> dict:
+    """Run a CASM run-params series (``state_generation`` incremental,
+    ``sampling_fixtures.<fixture>``) on the GPU: every state not yet completed
+    is a chain of one launch (runs are independent), each chain stops
+    counting at its own completion check.  With `output_dir`, writes
+    ``<output_dir>/<fixture>/summary.json`` (one entry per run in every leaf,
+    the reference's layout) and ``<output_dir>/completed_runs.json`` (the
+    conditions of each completed run) and, when they exist, resumes after the
+    runs already completed, as the reference's state generator does with
+    ``completed_runs.json`` (python/libcasm/clexmonte/_IncrementalConditionsStateGenerator.py:158-200).
+    Returns the merged summary."""
+    import json
+    import os
+
+    from .casm_io import _read_json, states_from_run_params
+
+    rp = _read_json(run_params)
+    states = states_from_run_params(rp, calculator.system, calculator.method, seed=seed)
+    fx = rp["sampling_fixtures"][fixture]
+    params = SamplingFixtureParams.from_casm(fixture, fx)
+    done, old_summary = [], None
+    if output_dir is not None:
+        cr = os.path.join(output_dir, "completed_runs.json")
+        sp = os.path.join(output_dir, fixture, "summary.json")
+        if os.path.exists(cr):
+            with open(cr) as f:
+                done = json.load(f)
+            if done and os.path.exists(sp):
+                with open(sp) as f:
+                    old_summary = json.load(f)
+    todo = states[len(done):]
+    summaries = calculator.run_multistate(todo, params) if todo else []
+    merged = merge_summaries(([old_summary] if old_summary else []) + summaries) if (old_summary or summaries) \
+        else {}
+    if output_dir is not None:
+        os.makedirs(os.path.join(output_dir, fixture), exist_ok=True)
+        for s in todo:
+            cond = {}
+            for k, v in s.conditions.items():
+                cond[k] = [float(x) for x in np.atleast_1d(v)] if np.ndim(v) else float(v)
+            done.append({"conditions": cond})
+        with open(os.path.join(output_dir, fixture, "summary.json"), "w") as f:
+            json.dump(merged, f, indent=1)
+        with open(os.path.join(output_dir, "completed_runs.json"), "w") as f:
+            json.dump(done, f, indent=1)
+    return merged
