@@ -813,6 +813,27 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, gen_sub.data(), gen_sub.size(), (GenSub **)&d.gen_sub));
   HIP_TRY(upload(h, A, gen_prog.data(), gen_prog.size(), (int **)&d.gen_prog));
   HIP_TRY(upload(h, A, env_off.data(), env_off.size(), (int **)&d.env_off));
+  {
+    // interaction neighbours per sublattice (self first), for rate updates of the N-fold way
+    std::vector<int> nb_begin(1, 0);
+    std::vector<int4> nb_cell;
+    int nb_max = 0;
+    for (int b = 0; b < h->n_sublat; ++b) {
+      std::set<std::array<int, 4>> u;
+      for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e)
+        for (int q = h->entry_site_begin[e]; q < h->entry_site_begin[e + 1]; ++q)
+          u.insert({h->entry_sites[4 * q], h->entry_sites[4 * q + 1], h->entry_sites[4 * q + 2],
+                    h->entry_sites[4 * q + 3]});
+      u.erase({b, 0, 0, 0});
+      nb_cell.push_back(make_int4(b, 0, 0, 0));
+      for (auto const &x : u) nb_cell.push_back(make_int4(x[0], x[1], x[2], x[3]));
+      nb_begin.push_back((int)nb_cell.size());
+      nb_max = std::max(nb_max, nb_begin[b + 1] - nb_begin[b]);
+    }
+    d.nb_max = nb_max;
+    HIP_TRY(upload(h, A, nb_begin.data(), nb_begin.size(), (int **)&d.nb_begin));
+    HIP_TRY(upload(h, A, nb_cell.data(), nb_cell.size(), (int4 **)&d.nb_cell));
+  }
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
   HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
@@ -1076,6 +1097,50 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(extra >= -1 && extra <= 64, "row pitch padding must be -1 (automatic) or 0..64");
   h->row_pitch = extra;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  ARG_CHECK(h->binary, "the N-fold way is implemented for binary cluster expansions");
+  ARG_CHECK(n_events >= 0 && events_per_block >= 1 && n_events % events_per_block == 0,
+            "n_events must be a non-negative multiple of events_per_block");
+  ARG_CHECK(blocks_out || n_events == 0, "blocks_out is NULL");
+  if (n_events == 0) return CLEXMC_OK;
+  HIP_TRY(hipSetDevice(h->device));
+  int max_pad = 0, max_sites = 0, nb_max = 1;
+  for (int c = 0; c < h->n_chains; ++c) {
+    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
+    max_pad = std::max(max_pad, d.pad_bytes);
+    max_sites = std::max(max_sites, d.n_sites);
+    nb_max = std::max(nb_max, d.nb_max);
+  }
+  ARG_CHECK(max_sites <= 262144, "the N-fold kernel takes supercells of at most 262,144 sites");
+  const size_t lds = nfold_lds_bytes(max_pad, max_sites, nb_max);
+  ARG_CHECK(lds <= 160 * 1024, "supercell too large for the N-fold kernel's LDS (image + one rate per site)");
+  NfoldArgs na;
+  memset(&na, 0, sizeof(na));
+  na.a.sys = h->d_sys;
+  na.a.scs = h->d_scs;
+  na.a.params = h->d_params;
+  na.a.state = h->d_state;
+  na.a.occ = h->d_occ;
+  na.a.binC = h->d_binC;
+  na.a.n_orbits = h->n_orbits;
+  na.n_events = n_events;
+  na.events_per_block = events_per_block;
+  na.n_blocks = n_events / events_per_block;
+  const size_t nb = (size_t)h->n_chains * na.n_blocks * 5;
+  double *d_blocks = nullptr;
+  HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
+  na.blocks = d_blocks;
+  hipError_t e = hipMemsetAsync(d_blocks, 0, nb * sizeof(double), h->stream);
+  if (e == hipSuccess) e = launch_nfold(na, h->n_chains, 256, lds, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(blocks_out, d_blocks, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d_blocks);
+  HIP_TRY(e);
   return CLEXMC_OK;
 }
 

@@ -142,6 +142,11 @@ struct DevSupercell {
   const int2 *pair_halo;
   int max_class;           // largest class size
   const int *env_off;      // [n_sublat][kEnvMax] padded offsets of the environment sites
+  // interaction neighbours per sublattice (the site itself included): the
+  // sites whose change of occupation changes a flip energy of the site
+  int nb_max;              // largest list
+  const int *nb_begin;     // [n_sublat+1]
+  const int4 *nb_cell;     // (sublattice, di, dj, dk) cell offsets
 };
 
 // Learned-proposal network (device copy).  FP32 master weights are kept by

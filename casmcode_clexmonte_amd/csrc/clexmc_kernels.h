@@ -93,6 +93,15 @@ struct AdamArgs {
   float lr, b1m, b2m, b1t, b2t;  // b1t = b1m^t, b2t = b2m^t
 };
 
+// N-fold way (rejection-free) semi-grand canonical runs of the binary path.
+struct NfoldArgs {
+  RunArgs a;               // system, supercells, chain params / state, occupations, exact counts
+  int n_events;            // events per chain in this launch
+  int events_per_block;    // events per output record
+  int n_blocks;            // n_events / events_per_block
+  double *blocks;          // [n_chains][n_blocks][5]: W, sum w E, sum w N1, sum w E^2, sum w N1^2
+};
+
 #ifndef __HIPCC_RTC__
 hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
                           int threads, size_t lds, hipStream_t st, bool launch);
@@ -114,6 +123,8 @@ hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
 size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed);
+hipError_t launch_nfold(const NfoldArgs &a, int n_chains, int threads, size_t lds, hipStream_t st);
+size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max);
 
 #endif  // __HIPCC_RTC__
 

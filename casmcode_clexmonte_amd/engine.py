@@ -72,7 +72,7 @@ def lib():
                      "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
                      "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
                      "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
-                     "clexmc_set_lds_row_pitch",
+                     "clexmc_set_lds_row_pitch", "clexmc_nfold_run",
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
@@ -94,7 +94,7 @@ def exported_symbols() -> List[str]:
             "clexmc_set_proposal", "clexmc_mlp_info", "clexmc_mlp_set_params", "clexmc_mlp_get_params",
             "clexmc_mlp_forward", "clexmc_mlp_gradients", "clexmc_mlp_collect", "clexmc_mlp_train",
             "clexmc_wave_pick_device", "clexmc_set_image_packing", "clexmc_set_global_swaps",
-            "clexmc_set_lds_row_pitch",
+            "clexmc_set_lds_row_pitch", "clexmc_nfold_run",
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
@@ -456,6 +456,17 @@ class Engine:
     def set_image_packing(self, mode: str):
         code = {"auto": 0, "int8": 1, "packed2": 2}[mode]
         self._check(lib().clexmc_set_image_packing(self.h, C.c_int(code)), "clexmc_set_image_packing")
+
+    def nfold_run(self, n_events: int, events_per_block: int) -> np.ndarray:
+        """N-fold way semi-grand canonical events on every chain (binary CEs).
+        Returns [n_chains, n_events // events_per_block, 5]: per block of
+        events (sum w, sum w E, sum w N1, sum w E^2, sum w N1^2), w the
+        expected residence time of each visited state in Metropolis steps."""
+        n_blocks = n_events // events_per_block
+        out = np.zeros((self.n_chains, n_blocks, 5))
+        self._check(lib().clexmc_nfold_run(self.h, C.c_int(n_events), C.c_int(events_per_block), _p(out, f64p)),
+                    "clexmc_nfold_run")
+        return out
 
     def set_lds_row_pitch(self, extra: int = -1):
         """Padded-image row pitch of supercells added later: -1 automatic

@@ -241,6 +241,20 @@ clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
  * pitch.  No reference counterpart (memory layout of this build). */
 clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
 
+/* N-fold way (rejection-free) semi-grand canonical runs of every loaded
+ * chain (binary CEs; supercells up to 262,144 sites): the reference's nfold
+ * method (include/casm/clexmonte/nfold/nfold_impl.hh:18-98) with its event
+ * rates (src/casm/clexmonte/nfold/nfold_events.cc:28-62: 1 if the
+ * semi-grand potential change dE <= 0, else exp(-beta dE)).  Each chain makes
+ * n_events events; every state visited is weighted by its expected residence
+ * time w = n_mutable / R (Metropolis steps; R the total rate), so weighted
+ * averages are Boltzmann averages.  blocks_out[n_chains][n_events /
+ * events_per_block][5] receives, per block of events, (sum w, sum w E,
+ * sum w N1, sum w E^2, sum w N1^2) with E the formation energy per supercell
+ * and N1 the number of sites holding occupant 1.  Chain state (occupation,
+ * exact counts, energy, counters) advances as in clexmc_run_passes. */
+clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out);
+
 /* Run-time specialisation (hipRTC, gfx950) of the binary Metropolis kernel
  * to the loaded supercell: constant site offsets instead of interpreted site
  * programs; used when every chain shares one supercell.  1 (default): when

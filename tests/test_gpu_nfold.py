@@ -1,0 +1,118 @@
+"""The N-fold way (rejection-free) semi-grand canonical kernel.
+
+* Exact enumeration (FCC binary 2x2x4, 16 sites, T = 1200 K, mu = 0.03 eV):
+  residence-weighted averages of E, E^2, N_B, N_B^2 over 128 chains within
+  3 sigma of the Boltzmann averages (error: standard error over chains).
+* State bookkeeping: after the events, each chain's running energy equals
+  the oracle's energy of its final occupation (exact counts, 1e-9), and the
+  occupant counts a following Metropolis pass samples match the occupation.
+* Against the plain Metropolis kernel on FCC 12^3 at a low-acceptance point
+  (T = 400 K, mu = -0.05 eV): mean energy and composition within 3 sigma.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.system import BasisSet, ClusterExpansion, FlatCE, Supercell, fcc_prim
+from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
+
+pytestmark = pytest.mark.gpu
+
+NSIG = 3.0
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def small_flat(seed=0):
+    prim = fcc_prim()
+    bs = BasisSet(prim, {2: 1.05, 3: 0.75}, "chebychev")
+    eci = np.random.default_rng(seed).normal(0, 0.05, bs.n_functions)
+    return FlatCE(ClusterExpansion(bs, eci))
+
+
+def _weighted(blocks, burn_blocks):
+    b = blocks[:, burn_blocks:, :].sum(axis=1)
+    W = b[:, 0]
+    return {"E": b[:, 1] / W, "E2": b[:, 3] / W, "n1": b[:, 2] / W, "n12": b[:, 4] / W}
+
+
+def test_nfold_exact_enumeration():
+    _print_case("nfold sgc exact enumeration 2x2x4")
+    flat = small_flat()
+    sc = Supercell(flat.ce.prim, (2, 2, 4))
+    T, mu = 1200.0, 0.03
+    chem = [0.0, mu]
+    exact = O.OracleSystem(flat, sc.dims).enumerate("sgc", T, chem=chem)
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    occs = [random_occupation(sc.n_sites, 0.5, 100 + c, exact=False) for c in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 1000 + np.arange(nch), chem=[chem] * nch)
+    blocks = eng.nfold_run(20000, 500)
+    assert blocks.shape == (nch, 40, 5) and np.all(blocks[:, :, 0] > 0)
+    per = _weighted(blocks, 2)
+    for k in ("E", "E2", "n1", "n12"):
+        m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
+        print(f"{k}: nfold {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
+        assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
+    osys = O.OracleSystem(flat, sc.dims)
+    for c in range(0, nch, 16):
+        st = eng.read_state(c)
+        Eref = osys.energy(st["occ"])
+        assert abs(st["E"] - Eref) <= 1e-9 * max(1.0, abs(Eref)), (st["E"], Eref)
+    # occupant counts left by the N-fold run: one Metropolis pass samples them
+    eng.clear_samples()
+    eng.run_passes("semigrand_canonical", 1, 1)
+    _, cnt = eng.read_samples()
+    for c in range(nch):
+        n1 = int(eng.read_state(c)["occ"].sum())
+        assert cnt[c, -1, 1] == n1 and cnt[c, -1, 0] == sc.n_sites - n1
+    eng.close()
+
+
+def test_nfold_vs_metropolis_low_acceptance():
+    _print_case("nfold vs metropolis fcc 12^3 T=400 K")
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    T, mu = 400.0, -0.05
+    chem = [0.0, mu]
+    nch = 32
+    nc = sc.n_cells
+    occs = [random_occupation(sc.n_sites, 0.5, 700 + c, exact=False) for c in range(nch)]
+    # plain Metropolis
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    eng.load_chains([sid] * nch, occs, [T] * nch, 4000 + np.arange(nch), chem=[chem] * nch)
+    eng.run_passes("semigrand_canonical", 3000, 1)
+    E, cnt = eng.read_samples()
+    att = np.array([eng.read_state(c)["n_attempt"] for c in range(nch)])
+    acc = np.array([eng.read_state(c)["n_accept"] for c in range(nch)])
+    print(f"metropolis acceptance {acc.sum() / att.sum():.4f}")
+    mE = E[:, 1000:].mean(1) / nc
+    mX = cnt[:, 1000:, 1].mean(1) / nc
+    eng.close()
+    # N-fold way from the same starting occupations
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    eng.load_chains([sid] * nch, occs, [T] * nch, 8000 + np.arange(nch), chem=[chem] * nch)
+    blocks = eng.nfold_run(60000, 1000)
+    steps = blocks[:, :, 0].sum(1) / sc.n_sites
+    print(f"nfold: Metropolis-equivalent passes per chain {steps.mean():.0f}")
+    # discard the blocks covering the first 1000 Metropolis-equivalent passes
+    cum = np.cumsum(blocks[:, :, 0], axis=1) / sc.n_sites
+    nE, nX = [], []
+    for c in range(nch):
+        keep = cum[c] > 1000
+        b = blocks[c, keep].sum(0)
+        nE.append(b[1] / b[0] / nc)
+        nX.append(b[2] / b[0] / nc)
+    nE, nX = np.array(nE), np.array(nX)
+    for name, g, n in (("E/cell", mE, nE), ("x_B", mX, nX)):
+        tol = NSIG * np.sqrt(g.var(ddof=1) / nch + n.var(ddof=1) / nch)
+        print(f"{name}: metropolis {g.mean():.6f}  nfold {n.mean():.6f}  tol {tol:.6f}")
+        assert abs(g.mean() - n.mean()) <= tol, (name, g.mean(), n.mean(), tol)
+    eng.close()
