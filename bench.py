@@ -247,6 +247,45 @@ def workload_c5(args):
             "autotune": tune, "max_energy_drift_eV": drift}
 
 
+def workload_nfold(args):
+    """N-fold way on the FCC binary 24^3 CE, 256 semi-grand canonical chains at
+    low temperature (T = 200..400 K x mu = -0.3..0.3 eV, where Metropolis
+    acceptance is small).  Every N-fold event is an accepted move; the plain
+    kernel's accepted moves per second at the same points (its site-attempt
+    rate times its acceptance) are measured beside it."""
+    from casmcode_clexmonte_amd.engine import Engine
+    from casmcode_clexmonte_amd.system import FlatCE, Supercell
+    from casmcode_clexmonte_amd.workloads import fcc_binary_ce, fcc_sgc_grid
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (24, 24, 24))
+    batch = fcc_sgc_grid(sc, T_range=(200.0, 400.0))
+    n = len(batch.temperatures)
+    eng = Engine(flat, device=0)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    eng.load_chains([sid] * n, batch.occ, batch.temperatures, batch.seeds, chem=batch.chem)
+    eng.run_passes("semigrand_canonical", 200, 200)          # equilibrate with the plain kernel
+    a0 = [eng.read_state(c) for c in range(n)]
+    t0 = time.perf_counter()
+    eng.run_passes("semigrand_canonical", 50, 50)
+    plain_wall = time.perf_counter() - t0
+    a1 = [eng.read_state(c) for c in range(n)]
+    n_acc = sum(b["n_accept"] - a["n_accept"] for a, b in zip(a0, a1))
+    n_att = sum(b["n_attempt"] - a["n_attempt"] for a, b in zip(a0, a1))
+    n_events = args.nfold_events
+    eng.nfold_run(100, 100)                                    # warm up
+    t0 = time.perf_counter()
+    eng.nfold_run(n_events, n_events // 10)
+    wall = time.perf_counter() - t0
+    eng.close()
+    ev_rate = n * n_events / wall
+    plain_acc_rate = n_acc / plain_wall
+    return {"workload": "fcc_binary_24x24x24_sgc_256chains_T200-400_nfold", "events": n * n_events,
+            "events_per_s": ev_rate, "wall_s": wall, "plain_site_attempts_per_s": n_att / plain_wall,
+            "plain_acceptance": n_acc / n_att, "plain_accepted_moves_per_s": plain_acc_rate,
+            "nfold_over_plain_accepted_moves": ev_rate / plain_acc_rate}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +298,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal ESS comparison")
     ap.add_argument("--no-extra", action="store_true", help="skip the single-chain, HCP and BCC workloads")
     ap.add_argument("--extra-passes", type=int, default=200)
+    ap.add_argument("--nfold-events", type=int, default=20000)
     ap.add_argument("--c4-size", type=int, default=32)
     ap.add_argument("--c4-chains", type=int, default=128)
     ap.add_argument("--c4-equil", type=int, default=1000)
@@ -355,7 +395,7 @@ def main():
         out["learned_vs_plain"] = learned_comparison(args)
     if not args.no_extra:
         out["workloads"] = {"single_chain": workload_c1(args), "hcp_ternary_uneven": workload_c3(args),
-                            "bcc_quaternary_64": workload_c5(args)}
+                            "bcc_quaternary_64": workload_c5(args), "nfold_low_T": workload_nfold(args)}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_rate, r = cpu_baseline(flat, sc, batch, args.cpu_chains, args.cpu_passes, threads)

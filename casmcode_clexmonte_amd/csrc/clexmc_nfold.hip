@@ -59,7 +59,7 @@ inline __host__ __device__ size_t align16(size_t x) { return (x + 15) & ~(size_t
 size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max) {
   const int n_groups = (n_sites + 63) / 64;
   return align16((size_t)image_bytes(pad_sites, false)) + align16((size_t)n_sites * 4) +
-         align16((size_t)n_groups * 4) + align16((size_t)nb_max * 4) + 16;
+         align16((size_t)n_groups * 4) + 2 * align16((size_t)nb_max * 4) + 16;
 }
 
 __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
@@ -89,7 +89,9 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
   off += align16((size_t)n_sites * 4);
   float *G = (float *)(smem + off);
   off += align16((size_t)n_groups * 4);
-  int *aff = (int *)(smem + off);
+  int *aff = (int *)(smem + off);    // linear indices of the sites whose rates change
+  off += align16((size_t)sc->nb_max * 4);
+  int *affp = (int *)(smem + off);   // their padded addresses
   off += align16((size_t)sc->nb_max * 4);
   int *sel = (int *)(smem + off);
 
@@ -112,17 +114,19 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
     const int i = c % L1, j = (c / L1) % L2, k = c / (L1 * L2);
     return b * S + (k + h) * P12 + (j + h) * P1 + (i + h);
   };
-  // flip rate of site l (0 for fixed sites)
-  auto site_rate = [&](int l) -> float {
-    const int b = l / n_cells;
+  // flip rate of the site at padded address pa on sublattice b (0 for fixed
+  // sites), and its cluster sums K (which do not depend on the site itself)
+  auto site_rate_at = [&](int b, int pa, int (&K)[kMaxSlots]) -> float {
     if (sys->n_occ[b] < 2) return 0.0f;
-    const int pa = main_addr(l);
     const int o = img.get(pa);
-    int K[kMaxSlots];
     const float F = binary_field(img, pa, o, cst(sc->bin_sub + b), bin_off, n_slots, K);
     const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
     const float dE = (o ? -2.0f : 2.0f) * F - (o ? (chem0 - chem1) : (chem1 - chem0));
     return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
+  };
+  auto site_rate = [&](int l) -> float {
+    int K[kMaxSlots];
+    return site_rate_at(l / n_cells, main_addr(l), K);
   };
   auto refresh_groups = [&]() {
     for (int g = wave; g < n_groups; g += n_waves) {
@@ -215,16 +219,13 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
     retried = false;
     if (site < 0) break;  // no event has a positive rate (uniform across the workgroup)
 
-    // ---- apply the flip (one lane)
+    // ---- apply the flip (one lane); its cluster sums K come with its new rate below
+    int o_flip = 0;
     if (tid == 0) {
       const int b = site / n_cells;
       const int pa = main_addr(site);
       const int o = img.get(pa);
-      int K[kMaxSlots];
-      (void)binary_field(img, pa, o, cst(sc->bin_sub + b), bin_off, n_slots, K);
-      const int sk = o ? -2 : 2;
-#pragma unroll
-      for (int a = 0; a < kMaxSlots; ++a) C[a] += (long long)sk * K[a];
+      o_flip = o;
       n1 += o ? -1 : 1;
       // every periodic copy of the site
       const int r = pa - b * S;
@@ -256,6 +257,7 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
         const int4 o = sc->nb_cell[nb0 + t];
         const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
         aff[t] = o.x * n_cells + it + L1 * (jt + L2 * kt);
+        affp[t] = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
       }
       __syncthreads();
       for (int t = tid; t < nb1 - nb0; t += nthr) {
@@ -263,10 +265,17 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
         bool dup = false;
         for (int q = 0; q < t; ++q) dup = dup || (aff[q] == lt);
         if (!dup) {
-          const float nr = site_rate(lt);
+          int K[kMaxSlots];
+          const float nr = site_rate_at(sc->nb_cell[nb0 + t].x, affp[t], K);
           const float d = nr - rate[lt];
           rate[lt] = nr;
           atomicAdd(&G[lt >> 6], d);
+          if (t == 0) {
+            // the flipped site itself (first in the list, lane 0): exact cluster-count change
+            const int sk = o_flip ? -2 : 2;
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) C[a] += (long long)sk * K[a];
+          }
         }
       }
       __syncthreads();
