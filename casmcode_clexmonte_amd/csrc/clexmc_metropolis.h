@@ -414,6 +414,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     for (int c = 0; c < n_classes; ++c) {
       const int b = sc->class_sublat[c];
       const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
+      // this lane's first halo copy of the class, loaded before the sweep so
+      // that its global-memory latency overlaps the sweep
+      const int hb0 = sc->halo_begin[c] + tid, hb1 = sc->halo_begin[c + 1];
+      const int2 hc0 = hb0 < hb1 ? sc->halo[hb0] : make_int2(0, 0);
       cptr<BinSub> bs = cst(sc->bin_sub + b);
       cptr<GenSub> gs = cst(sc->gen_sub + b);
       const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
@@ -564,7 +568,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         }  // n_rounds > 0
       }
       // ---- halo copies of this class
-      for (int t = sc->halo_begin[c] + tid; t < sc->halo_begin[c + 1]; t += nthr) {
+      if (hb0 < hb1) img.copy(hc0.y, hc0.x);
+      for (int t = hb0 + nthr; t < hb1; t += nthr) {
         const int2 hc = sc->halo[t];
         img.copy(hc.y, hc.x);
       }
