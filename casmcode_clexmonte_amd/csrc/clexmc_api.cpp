@@ -33,11 +33,12 @@ struct HostSupercell {
   std::vector<int> bin_off;
   std::vector<GenSub> gen_sub;
   std::vector<int> gen_prog;
-  // specialised kernels per (kind, packed, lanes): 0 not tried, 1 loaded, -1 unavailable
-  int jit_state[2][2][2] = {};
+  // specialised kernels per (kind, packed, lanes, launch bound 512 / 1024):
+  // 0 not tried, 1 loaded, -1 unavailable
+  int jit_state[2][2][2][2] = {};
   size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
   std::vector<hipModule_t> jit_mods;
-  hipFunction_t jit_fn[2][2][2] = {};
+  hipFunction_t jit_fn[2][2][2][2] = {};
 };
 
 }  // namespace
@@ -1080,8 +1081,8 @@ clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int3
 
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads) {
   if (!h) return CLEXMC_ERR_ARG;
-  ARG_CHECK(threads == 0 || (threads >= 64 && threads <= 512 && threads % 64 == 0),
-            "block size must be 0 or a multiple of 64 in [64, 512]");
+  ARG_CHECK(threads == 0 || (threads >= 64 && threads <= kMaxJitThreads && threads % 64 == 0),
+            "block size must be 0 or a multiple of 64 in [64, 1024]");
   h->block_size = threads;
   return CLEXMC_OK;
 }
@@ -1201,31 +1202,34 @@ struct RunPlan {
   LearnArgs la;
 };
 
-// The specialised kernel for (supercell, kind, packing), compiled on first
-// use; nullptr when specialisation is off or unavailable.
-static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes) {
+// The specialised kernel for (supercell, kind, packing, lanes per site, block
+// size above 512 or not), compiled on first use; nullptr when specialisation
+// is off or unavailable.
+static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes, int threads) {
   HostSupercell &hs = h->scs[sc_id];
-  const int k = kind ? 1 : 0, p = packed ? 1 : 0, l = lanes == 2 ? 1 : 0;
-  if (hs.jit_state[k][p][l] == 0) {
-    hs.jit_state[k][p][l] = -1;
+  const int k = kind ? 1 : 0, p = packed ? 1 : 0, l = lanes == 2 ? 1 : 0, w = lanes * threads > kAotMaxThreads;
+  int &state = hs.jit_state[k][p][l][w];
+  hipFunction_t &fn = hs.jit_fn[k][p][l][w];
+  if (state == 0) {
+    state = -1;
     std::vector<int> variable(h->n_sublat);
     for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
     const std::string src = jit_source(h->n_sublat, variable, h->binary != 0, hs.bin_sub, hs.bin_off, h->n_slots,
-                                       hs.gen_sub, hs.gen_prog, kind, packed, lanes);
+                                       hs.gen_sub, hs.gen_prog, kind, packed, lanes, w ? kMaxJitThreads : kAotMaxThreads);
     hipModule_t mod = nullptr;
     const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
     if (je != hipSuccess) h->jit_log += std::string("\n[compile/load: ") + hipGetErrorString(je) + "]";
     if (je == hipSuccess) {
-      if (hipModuleGetFunction(&hs.jit_fn[k][p][l], mod, jit_kernel_name(kind, packed, lanes)) == hipSuccess) {
+      if (hipModuleGetFunction(&fn, mod, jit_kernel_name(kind, packed, lanes)) == hipSuccess) {
         hs.jit_mods.push_back(mod);
-        hs.jit_state[k][p][l] = 1;
+        state = 1;
       } else {
         h->jit_log += "\n[kernel lookup failed]";
         hipModuleUnload(mod);
       }
     }
   }
-  return hs.jit_state[k][p][l] == 1 ? hs.jit_fn[k][p][l] : nullptr;
+  return state == 1 ? fn : nullptr;
 }
 
 static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
@@ -1269,8 +1273,10 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     max_pad = std::max(max_pad, d.pad_bytes);
     max_class = std::max(max_class, d.max_class);
   }
+  // one lane per class site up to 1024 threads; blocks above 512 threads need
+  // the specialised kernel (the ahead-of-time kernels are bounded at 512)
   int threads = threads_override ? threads_override : h->block_size;
-  if (threads == 0) threads = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+  if (threads == 0) threads = std::min(kMaxJitThreads, std::max(64, ((max_class + 63) / 64) * 64));
   const int packing = packing_override ? packing_override : h->packing;
   const int n_tab = h->binary ? 0 : h->n_flower_tables;
   int max_lds = 0;
@@ -1335,9 +1341,20 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     for (int i = 1; i < n_sel; ++i) one_shape = one_shape && h->chain_sc[subset ? (*subset)[i] : i] == h->chain_sc[c0];
     const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
     P.sc_id = h->chain_sc[c0];
-    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes);
+    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes, P.threads);
   }
+  if (!P.jit) P.threads = std::min(P.threads, kAotMaxThreads);
   return CLEXMC_OK;
+}
+
+// The runtime refused a specialised launch (dynamic LDS of a module kernel
+// above its limit): remember the LDS size and fall back to the ahead-of-time
+// kernel, whose block size is bounded at 512 threads.
+static void drop_jit(clexmc_handle *h, RunPlan &P, int sc_id) {
+  (void)hipGetLastError();
+  h->scs[sc_id].jit_lds_fail = P.lds;
+  P.jit = nullptr;
+  P.threads = std::min(P.threads, kAotMaxThreads);
 }
 
 // attribute setting (not a stream operation) and the launch, separately, so
@@ -1401,9 +1418,7 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
     hipStream_t st = h->side[g % ns];
     hipError_t le = plan_launch(P, h, st);
     if (le != hipSuccess && P.jit) {
-      (void)hipGetLastError();
-      h->scs[P.sc_id].jit_lds_fail = P.lds;
-      P.jit = nullptr;
+      drop_jit(h, P, P.sc_id);
       HIP_TRY(plan_attributes(P, h));
       le = plan_launch(P, h, st);
     }
@@ -1447,12 +1462,9 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   hipError_t le = plan_launch(P, h);
   if (le != hipSuccess && P.jit) {
-    // the runtime refused the specialised launch (dynamic LDS of a module
-    // kernel above its limit): remember the size, run the generic kernel
-    (void)hipGetLastError();
-    h->scs[h->chain_sc[0]].jit_lds_fail = P.lds;
-    P.jit = nullptr;
+    drop_jit(h, P, h->chain_sc[0]);
     h->last_specialized = 0;
+    h->last_threads = P.threads;
     HIP_TRY(plan_attributes(P, h));
     le = plan_launch(P, h);
   }
@@ -1504,11 +1516,9 @@ clexmc_status clexmc_graph_capture(clexmc_handle *h, int kind, int passes_per_re
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess || !P.jit) break;
     // specialised launch refused: capture the generic kernel instead
-    (void)hipGetLastError();
     if (g) hipGraphDestroy(g);
     g = nullptr;
-    h->scs[h->chain_sc[0]].jit_lds_fail = P.lds;
-    P.jit = nullptr;
+    drop_jit(h, P, h->chain_sc[0]);
     HIP_TRY(plan_attributes(P, h));
   }
   if (e != hipSuccess) {
@@ -1599,16 +1609,21 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
     HIP_TRY(hipMemcpy(s_state, h->d_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToDevice));
     HIP_TRY(hipMemcpy(s_binC, h->d_binC, sizeof(long long) * h->n_chains * h->n_orbits, hipMemcpyDeviceToDevice));
     HIP_TRY(hipMemcpy(s_cnt, h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToDevice));
-    const int t_max = std::min(512, std::max(64, ((max_class + 63) / 64) * 64));
+    // block sizes: powers of two from 64, and one lane per site of the
+    // largest class (up to 1024 threads, specialised kernels only)
+    const int t_class = std::min(kMaxJitThreads, std::max(64, ((max_class + 63) / 64) * 64));
+    std::vector<int> t_list;
+    for (int t = 64; t <= t_class; t *= 2) t_list.push_back(t);
+    if (t_list.back() != t_class) t_list.push_back(t_class);
     for (int lanes = 1; lanes <= (h->binary ? 2 : 1); ++lanes)
     for (int packing = 1; packing <= 2; ++packing) {
-      for (int t = 64; t <= t_max; t *= 2) {
+      for (int t : t_list) {
         RunPlan P;
         if (plan_run(h, kind, trial_passes, trial_passes + 1, canonical_rounds, trial_passes, P, t, packing, nullptr,
                      lanes) != CLEXMC_OK)
           continue;
         if (packing == 2 && (P.learned || !P.packed)) continue;
-        if (P.lanes != lanes) continue;
+        if (P.lanes != lanes || P.threads != t) continue;
         HIP_TRY(plan_attributes(P, h));
         HIP_TRY(hipEventRecord(h->ev0, h->stream));
         HIP_TRY(plan_launch(P, h));

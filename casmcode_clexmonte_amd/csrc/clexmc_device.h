@@ -28,6 +28,8 @@ constexpr int kMaxOrbits = 64;   // correlations kernel / exact counts storage
 constexpr int kMaxSlots = 8;     // binary path: orbits with >= 1 site (registers)
 constexpr int kMaxSitesPerProp = 4;
 constexpr int kMaxAutoRounds = 1024; // canonical matchings per independent set and pass (automatic)
+constexpr int kAotMaxThreads = 512;  // launch bound of the ahead-of-time Metropolis kernels
+constexpr int kMaxJitThreads = 1024; // launch bound of run-time specialised kernels for large classes
 
 // Automatic canonical rounds of one independent set.  A random matching of
 // the set's n sites pairs a given site with each other site with probability

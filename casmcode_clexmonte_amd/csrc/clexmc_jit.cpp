@@ -60,7 +60,7 @@ static std::string hexf(float v) {
 
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes) {
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
   std::ostringstream o;
   o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\nstruct JitField {\n"
     << "  template <class IMG>\n"
@@ -188,7 +188,7 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     o << "      return;\n    }\n";
   }
   o << "  }\n};\n}  // namespace clexmc\n";
-  o << "extern \"C\" __global__ __launch_bounds__(" << (lanes == 2 ? 1024 : 512) << ") void "
+  o << "extern \"C\" __global__ __launch_bounds__(" << (lanes == 2 || max_threads > 512 ? 1024 : 512) << ") void "
     << jit_kernel_name(kind, packed, lanes) << "(clexmc::RunArgs a) { clexmc::metropolis_body<"
     << (binary ? "true" : "false") << ", " << kind << ", " << (packed ? "true" : "false") << ", clexmc::JitField, "
     << lanes << ">(a); }\n";

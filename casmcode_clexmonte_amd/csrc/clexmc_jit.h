@@ -17,10 +17,11 @@ namespace clexmc {
 // the binary programs (subs[n_sublat], offsets as laid out by the library)
 // when `binary`, else the general table programs (gsubs[n_sublat], gen_prog:
 // per cluster k, table offset, then k (offset, radix multiplier) pairs).
-// One kernel per (kind, packed, lanes per site) variant.
+// One kernel per (kind, packed, lanes per site) variant; `max_threads` (512 or
+// 1024) is its launch bound.
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes);
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads);
 
 // Kernel name for (kind, packed, lanes per site) in a module built from
 // jit_source (two lanes per site: binary expansions only).

@@ -222,7 +222,10 @@ clexmc_status clexmc_reanchor_energies(clexmc_handle *h, double *max_drift);
  * stream, milliseconds). */
 clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
 
-/* Tuning knobs (block size of the Metropolis kernel; 0 = automatic). */
+/* Tuning knobs (block size of the Metropolis kernel, a multiple of 64 up to
+ * 1024; 0 = automatic: one lane per site of the largest class.  Blocks above
+ * 512 threads run only on run-time specialised kernels; without one the
+ * library caps the block at 512). */
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
 
 /* Canonical runs: serial swaps of uniformly random site pairs per pass

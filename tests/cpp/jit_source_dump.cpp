@@ -50,6 +50,6 @@ int main(int argc, char **argv) {
       }
     }
   }
-  std::cout << clexmc::jit_source(1, {1}, mode == "bin", subs, off, ns, gsubs, prog, 1, false, mode == "bin" ? 2 : 1);
+  std::cout << clexmc::jit_source(1, {1}, mode == "bin", subs, off, ns, gsubs, prog, 1, false, mode == "bin" ? 2 : 1, 512);
   return 0;
 }
