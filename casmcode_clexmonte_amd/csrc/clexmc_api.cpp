@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <numeric>
 #include <random>
@@ -85,7 +86,10 @@ struct clexmc_handle {
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
-  int last_specialized = 0, last_threads = 0, last_lanes = 1;
+  int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1;
+  int slabs = 0;  // slabs per chain: 0 automatic, 1 off, >= 2 requested
+  SlabRecord *d_slab_rec = nullptr;  // [n_chains][kSlabMax][kSlabRecords] (slab launches)
+  int slab_rec_chains = 0;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
   int row_pitch = -1;               // padded image row pitch: -1 automatic, else extra bytes per row
   int n_slots = 0;                  // binary path: orbits with >= 1 site
@@ -834,6 +838,33 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
     d.nb_max = nb_max;
     HIP_TRY(upload(h, A, nb_begin.data(), nb_begin.size(), (int **)&d.nb_begin));
     HIP_TRY(upload(h, A, nb_cell.data(), nb_cell.size(), (int4 **)&d.nb_cell));
+    // slabs: reach along k of the clusters (largest |dk| of a flower site) and
+    // per class the first list position with cell plane >= K; a class whose
+    // sites are not in plane order leaves the supercell without slabs
+    int kreach = 0;
+    for (int q = 0; q < (int)nb_cell.size(); ++q) kreach = std::max(kreach, std::abs(nb_cell[q].w));
+    std::vector<int> kidx((size_t)sc->n_classes * (L3 + 1));
+    bool ordered = true;
+    for (int c = 0; c < sc->n_classes && ordered; ++c) {
+      int *kx = kidx.data() + (size_t)c * (L3 + 1);
+      int t = cb[c];
+      int prev = -1;
+      for (int K = 0; K <= L3; ++K) {
+        while (t < cb[c + 1]) {
+          const int l = sc->class_sites[t];
+          const int kt = (l % d.n_cells) / (L1 * L2);
+          if (kt < prev) ordered = false;
+          if (kt >= K) break;
+          prev = kt;
+          ++t;
+        }
+        kx[K] = t - cb[c];
+      }
+      if (t != cb[c + 1]) ordered = false;
+    }
+    d.kreach = ordered ? kreach : -1;
+    d.class_kidx = nullptr;
+    if (ordered) HIP_TRY(upload(h, A, kidx.data(), kidx.size(), (int **)&d.class_kidx));
   }
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
@@ -876,6 +907,9 @@ static void free_chains(clexmc_handle *h) {
   hipFree(h->d_sampE);
   hipFree(h->d_sampN);
   hipFree(h->d_sample_count);
+  hipFree(h->d_slab_rec);
+  h->d_slab_rec = nullptr;
+  h->slab_rec_chains = 0;
   h->d_params = nullptr;
   h->d_state = nullptr;
   h->d_occ = nullptr;
@@ -1087,6 +1121,19 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads) {
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_set_slabs(clexmc_handle *h, int slabs) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(slabs >= 0 && slabs <= 64, "slabs must be 0 (automatic), 1 (off) or 2..64");
+  h->slabs = slabs;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs) {
+  if (!h || !slabs) return CLEXMC_ERR_ARG;
+  *slabs = h->last_slabs;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(per_pass >= 0 && per_pass <= 65536, "global swaps per pass out of range");
@@ -1196,8 +1243,11 @@ struct RunPlan {
   size_t lds = 0;
   hipFunction_t jit = nullptr;  // specialised kernel (one supercell shape)
   int lanes = 1;                // lanes per site in the class sweeps (binary path: 1 or 2)
-  int n_blocks = 0;             // workgroups (chains) of this launch
+  int n_blocks = 0;             // chains of this launch
   int sc_id = 0;                // supercell of a single-shape launch
+  bool one_shape = false;       // every chain of the launch on supercell sc_id
+  int slabs = 1;                // workgroups per chain (slab decomposition along k)
+  unsigned long long pass_start = 0;  // the chains' pass counter before the run
   RunArgs a;
   LearnArgs la;
 };
@@ -1330,18 +1380,22 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.n_tab = n_tab;
   a.global_swaps = h->global_swaps;
   a.chain_map = nullptr;
+  a.slabs = 1;
+  a.slab_rec = h->d_slab_rec;
   P.n_blocks = n_sel;
   P.la.mlp = h->d_mlp;
   P.la.rounds_per_group = h->rounds_per_group;
   // specialised kernel when every chain uses the same supercell
   P.jit = nullptr;
-  if (h->specialize && !P.learned) {
+  {
     const int c0 = subset ? (*subset)[0] : 0;
     bool one_shape = true;
     for (int i = 1; i < n_sel; ++i) one_shape = one_shape && h->chain_sc[subset ? (*subset)[i] : i] == h->chain_sc[c0];
-    const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
     P.sc_id = h->chain_sc[c0];
-    if (one_shape && (fail == 0 || lds < fail)) P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes, P.threads);
+    P.one_shape = one_shape;
+    const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
+    if (h->specialize && !P.learned && one_shape && (fail == 0 || lds < fail))
+      P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes, P.threads);
   }
   if (!P.jit) P.threads = std::min(P.threads, kAotMaxThreads);
   return CLEXMC_OK;
@@ -1367,18 +1421,106 @@ static hipError_t plan_attributes(const RunPlan &P, clexmc_handle *h) {
   return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, h->n_chains, P.threads, P.lds, h->stream, false);
 }
 
-static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st = nullptr) {
+static hipError_t plan_launch_one(const RunPlan &P, const RunArgs &args_in, clexmc_handle *h, hipStream_t st) {
+  const int grid = P.n_blocks * std::max(1, args_in.slabs);
   if (P.jit) {
-    RunArgs args = P.a;
+    RunArgs args = args_in;
     void *params[] = {(void *)&args};
-    return hipModuleLaunchKernel(P.jit, (unsigned)P.n_blocks, 1, 1, (unsigned)(P.threads * P.lanes), 1, 1,
-                                 (unsigned)P.lds, st ? st : h->stream, params, nullptr);
+    return hipModuleLaunchKernel(P.jit, (unsigned)grid, 1, 1, (unsigned)(P.threads * P.lanes), 1, 1, (unsigned)P.lds, st,
+                                 params, nullptr);
   }
-  if (P.lanes == 2)
-    return launch_metropolis_split(P.a, P.kind, P.packed, P.n_blocks, P.threads * 2, P.lds, st ? st : h->stream, true);
-  if (P.learned) return launch_learned(P.a, P.la, P.kind, P.weights, P.n_blocks, P.threads, P.lds, h->stream, true);
-  return launch_metropolis(P.a, h->binary != 0, P.kind, P.packed, P.n_blocks, P.threads, P.lds, st ? st : h->stream,
-                           true);
+  if (P.lanes == 2) return launch_metropolis_split(args_in, P.kind, P.packed, grid, P.threads * 2, P.lds, st, true);
+  if (P.learned) return launch_learned(args_in, P.la, P.kind, P.weights, P.n_blocks, P.threads, P.lds, h->stream, true);
+  return launch_metropolis(args_in, h->binary != 0, P.kind, P.packed, grid, P.threads, P.lds, st, true);
+}
+
+// Slab plans run as a sequence of launches of at most kSlabEpochPasses passes
+// (a launch may hold several sample points); each is followed by
+// slab_post_kernel (samples, chain totals, pass and draw counters).  The slab
+// boundaries move between launches.
+
+static std::vector<int> slab_epochs(unsigned long long pass_start, int n_passes, int sample_period) {
+  std::vector<int> lens;
+  int done = 0;
+  while (done < n_passes) {
+    (void)pass_start;
+    (void)sample_period;
+    const int len = std::min(n_passes - done, kSlabEpochPasses);
+    lens.push_back(len);
+    done += len;
+  }
+  return lens;
+}
+
+// `len` passes of the plan (its next launch of an epoch sequence)
+static hipError_t plan_launch_window(const RunPlan &P, clexmc_handle *h, hipStream_t s, int len) {
+  RunArgs a = P.a;
+  a.n_passes = len;
+  a.slabs = P.slabs;
+  hipError_t e = plan_launch_one(P, a, h, s);
+  if (e == hipSuccess && P.slabs > 1) e = launch_slab_post(a, P.n_blocks, s);
+  return e;
+}
+
+static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st = nullptr) {
+  hipStream_t s = st ? st : h->stream;
+  if (P.slabs <= 1) return plan_launch_one(P, P.a, h, s);
+  for (int len : slab_epochs(P.pass_start, P.a.n_passes, P.a.sample_period)) {
+    const hipError_t e = plan_launch_window(P, h, s, len);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Workgroups per chain.  Automatic: when a batch of large chains does not
+// fill the device, chains are cut along k into slabs so that each workgroup
+// holds about total_sites / compute_units sites (at least kMinSlabSites),
+// with slabs at least three cluster reaches thick (two thirds or more of the
+// planes updated per launch), at most kMaxSlabs.  Requested (>= 2): that many
+// slabs where every slab keeps an updated plane.  Single-shape, plain
+// Metropolis launches with one lane per site only.
+static constexpr int kMinSlabSites = 8192;
+static constexpr int kMaxSlabs = 16;
+
+static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &plans) {
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) n_cu = 1;
+  long long total = 0;
+  for (int c = 0; c < h->n_chains; ++c) total += h->scs[h->chain_sc[c]].dev.n_sites;
+  const long long per_wg = std::max<long long>(kMinSlabSites, total / std::max(1, n_cu));
+  for (RunPlan *P : plans) {
+    P->slabs = 1;
+    P->a.slabs = 1;
+    if (h->slabs == 1 || P->learned || P->lanes != 1 || !P->one_shape) continue;
+    const DevSupercell &d = h->scs[P->sc_id].dev;
+    if (d.kreach < 0 || !d.class_kidx) continue;
+    int S;
+    if (h->slabs >= 2) {
+      S = std::min(h->slabs, d.L3 / (d.kreach + 1));
+    } else {
+      const int cap = std::min(kMaxSlabs, d.kreach > 0 ? d.L3 / (3 * d.kreach) : d.L3);
+      S = (int)std::min<long long>(cap, (d.n_sites + per_wg / 2) / per_wg);
+    }
+    if (S < 2) continue;
+    if (!h->d_slab_rec || h->slab_rec_chains < h->n_chains) {
+      if (h->d_slab_rec) (void)hipFree(h->d_slab_rec);
+      h->d_slab_rec = nullptr;
+      h->slab_rec_chains = 0;
+      const hipError_t e =
+          hipMalloc(&h->d_slab_rec, sizeof(SlabRecord) * (size_t)h->n_chains * kSlabMax * kSlabRecords);
+      if (e != hipSuccess) return e;
+      h->slab_rec_chains = h->n_chains;
+    }
+    P->a.slab_rec = h->d_slab_rec;
+    P->slabs = S;
+    // a slab's class ranges are a fraction of the class: 512-thread blocks
+    // (measured faster than 1024 for slabs of the HCP and BCC workloads)
+    if (h->block_size == 0 && P->threads > kAotMaxThreads) {
+      P->threads = kAotMaxThreads;
+      if (P->jit) P->jit = jit_kernel(h, P->sc_id, P->kind, P->packed, P->lanes, P->threads);
+    }
+  }
+  return hipSuccess;
 }
 
 // Chains on different supercells: one launch per supercell group, each with
@@ -1387,6 +1529,32 @@ static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st
 // waits for all of them.
 static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
   const int ng = (int)h->groups.size();
+  std::vector<RunPlan> plans(ng);
+  for (int g = 0; g < ng; ++g) {
+    clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, plans[g], 0, 0,
+                                &h->groups[g]);
+    if (st != CLEXMC_OK) return st;
+    plans[g].a.chain_map = h->d_group_map + h->group_off[g];
+  }
+  // sample buffers may have grown while planning: every plan uses the final ones
+  std::vector<RunPlan *> pp;
+  for (auto &P : plans) {
+    P.a.sampE = h->d_sampE;
+    P.a.sampN = h->d_sampN;
+    P.a.sample_cap = h->sample_cap;
+    pp.push_back(&P);
+  }
+  HIP_TRY(choose_slabs(h, pp));
+  for (auto &P : plans) HIP_TRY(plan_attributes(P, h));
+  unsigned long long pass_start = 0;
+  HIP_TRY(hipMemcpy(&pass_start, &h->d_state[0].pass_base, sizeof(pass_start), hipMemcpyDeviceToHost));
+  int max_slabs = 1;
+  for (auto &P : plans) {
+    P.pass_start = pass_start;
+    max_slabs = std::max(max_slabs, P.slabs);
+  }
+  // one stream per group (up to 8): groups of whole chains run one persistent
+  // launch, slab groups their sequence of launches
   const int ns = std::min(ng, 8);
   while ((int)h->side.size() < ns) {
     hipStream_t s;
@@ -1395,20 +1563,6 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
     HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     h->side.push_back(s);
     h->side_ev.push_back(e);
-  }
-  std::vector<RunPlan> plans(ng);
-  for (int g = 0; g < ng; ++g) {
-    clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, plans[g], 0, 0,
-                                &h->groups[g]);
-    if (st != CLEXMC_OK) return st;
-    plans[g].a.chain_map = h->d_group_map + h->group_off[g];
-    HIP_TRY(plan_attributes(plans[g], h));
-  }
-  // sample buffers may have grown while planning: every plan uses the final ones
-  for (auto &P : plans) {
-    P.a.sampE = h->d_sampE;
-    P.a.sampN = h->d_sampN;
-    P.a.sample_cap = h->sample_cap;
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   for (int s = 0; s < ns; ++s) HIP_TRY(hipStreamWaitEvent(h->side[s], h->ev0, 0));
@@ -1439,6 +1593,7 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
   h->last_packed = any_packed;
   h->last_threads = plans[0].threads;
   h->last_lanes = plans[0].lanes;
+  h->last_slabs = max_slabs;
   return CLEXMC_OK;
 }
 
@@ -1454,10 +1609,13 @@ clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sa
   RunPlan P;
   clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, P);
   if (st != CLEXMC_OK) return st;
+  HIP_TRY(choose_slabs(h, {&P}));
+  if (P.slabs > 1) HIP_TRY(hipMemcpy(&P.pass_start, &h->d_state[0].pass_base, sizeof(P.pass_start), hipMemcpyDeviceToHost));
   h->last_packed = P.packed;
   h->last_specialized = P.jit ? 1 : 0;
   h->last_threads = P.threads;
   h->last_lanes = P.lanes;
+  h->last_slabs = P.slabs;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   hipError_t le = plan_launch(P, h);

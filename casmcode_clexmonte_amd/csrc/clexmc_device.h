@@ -149,6 +149,11 @@ struct DevSupercell {
   int nb_max;              // largest list
   const int *nb_begin;     // [n_sublat+1]
   const int4 *nb_cell;     // (sublattice, di, dj, dk) cell offsets
+  // slabs: largest |dk| between two sites that share a cluster, and per class
+  // the index (relative to class_begin) of its first site with cell k >= K,
+  // K = 0..L3 (class sites are in linear-index order, so k is sorted)
+  int kreach;
+  const int *class_kidx;   // [n_classes][L3+1]
 };
 
 // Learned-proposal network (device copy).  FP32 master weights are kept by
@@ -161,6 +166,17 @@ struct MlpDev {
   float b1[kHidden];
   float W2[kHidden];
   uint16_t W1b[kHidden * kEnvMax]; // bf16 bits, [hidden][input]
+};
+
+// Slab launches: what one workgroup has added to its chain since the launch
+// started, recorded at each sampled pass and at the launch's last pass
+// (dI: binary exact-count slots, then occupant counts, attempts, accepts).
+constexpr int kSlabMax = 64;       // slabs per chain
+constexpr int kSlabEpochPasses = 4; // passes per slab launch
+constexpr int kSlabRecords = kSlabEpochPasses + 1;
+struct SlabRecord {
+  double dE;
+  long long dI[kMaxSlots + kMaxOcc + 2];
 };
 
 // Read-only per-chain parameters (constant during a launch).
@@ -184,6 +200,7 @@ struct ChainState {
   // FP64 observable accumulators over recorded samples
   double sumE, sumE2, sumN1, sumN12, sumEN1;
   long long n_samples_acc;
+  unsigned long long draw_hi;  // slab launches: largest Philox draw counter used
 };
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,13 @@ struct RunArgs {
   int n_tab;           // general path: flower table floats staged in LDS
   int global_swaps;    // canonical: serial swaps of uniformly random site pairs per pass
   const int *chain_map; // workgroup -> chain (nullptr: workgroup i runs chain i)
+  // slabs > 1: workgroup w runs slab w % slabs of chain w / slabs; it updates
+  // only the sites of its slab's planes (minus a frozen band of kreach
+  // planes), records what it added to the chain at the sampled passes and at
+  // the end (slab_rec), and leaves samples, chain totals and the pass / draw
+  // counters to slab_post_kernel
+  int slabs;
+  SlabRecord *slab_rec;  // [n_chains][kSlabMax][kSlabRecords]
 };
 
 struct DeltaArgs {
@@ -119,6 +126,7 @@ hipError_t launch_mlp_publish(const float *p, MlpDev *m, int n_env, float eps, h
 
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
                              size_t lds, hipStream_t st, bool launch);
+hipError_t launch_slab_post(const RunArgs &a, int n_chains, hipStream_t st);
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);

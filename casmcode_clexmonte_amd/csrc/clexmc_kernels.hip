@@ -195,6 +195,91 @@ hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool pa
 }
 
 // ---------------------------------------------------------------------------
+// After a slab launch (RunArgs::slabs > 1), one thread per chain: sums the
+// slabs' records in pass order (what each slab had added since the launch
+// started, at every sampled pass and at the last pass), writes the samples,
+// then the chain's energy, counts and exact binary counts, and advances the
+// pass and draw counters, which the slabs read but do not change.
+// ---------------------------------------------------------------------------
+__global__ void slab_post_kernel(RunArgs a, int n_chains) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_chains) return;
+  const int chain = a.chain_map ? a.chain_map[i] : i;
+  const DevSystem *sys = a.sys;
+  ChainState *cs = a.state + chain;
+  const bool bin = sys->binary != 0;
+  long long *C = a.binC + (long long)chain * a.n_orbits;
+  const double E0 = cs->E;
+  const unsigned long long pass0 = cs->pass_base;
+  constexpr int nI = kMaxSlots + kMaxOcc + 2;
+  long long tot[nI];
+  double totE = 0.0;
+  int j = 0;
+  for (int p = 0; p < a.n_passes; ++p) {
+    const bool due = ((pass0 + (unsigned long long)p + 1ull) % (unsigned long long)a.sample_period) == 0ull;
+    if (!due && p != a.n_passes - 1) continue;
+    for (int q = 0; q < nI; ++q) tot[q] = 0;
+    totE = 0.0;
+    for (int s = 0; s < a.slabs; ++s) {
+      const SlabRecord &r = a.slab_rec[((long long)chain * kSlabMax + s) * kSlabRecords + j];
+      totE += r.dE;
+      for (int q = 0; q < nI; ++q) tot[q] += r.dI[q];
+    }
+    ++j;
+    if (!due) continue;
+    double E;
+    if (bin) {
+      E = 0.0;
+      for (int o = 0; o < a.n_orbits; ++o) {
+        long long c = C[o];
+        for (int q = 0; q < sys->n_slots; ++q)
+          if (sys->slot_orbit[q] == o) c += tot[q];
+        E += sys->bin_w64[o] * (double)c;
+      }
+    } else {
+      E = E0 + totE;
+    }
+    long long cnt[kMaxOcc];
+    for (int q = 0; q < kMaxOcc; ++q) cnt[q] = cs->counts[q] + tot[kMaxSlots + q];
+    const int idx = a.sample_count[chain];
+    if (idx < a.sample_cap) {
+      const long long o = (long long)chain * a.sample_cap + idx;
+      a.sampE[o] = E;
+      for (int q = 0; q < kMaxOcc; ++q) a.sampN[o * kMaxOcc + q] = (int)cnt[q];
+      a.sample_count[chain] = idx + 1;
+    }
+    const double n1 = (double)cnt[1];
+    cs->sumE += E;
+    cs->sumE2 += E * E;
+    cs->sumN1 += n1;
+    cs->sumN12 += n1 * n1;
+    cs->sumEN1 += E * n1;
+    cs->n_samples_acc += 1;
+  }
+  // the last record holds the launch's totals
+  if (bin) {
+    for (int q = 0; q < sys->n_slots; ++q) C[sys->slot_orbit[q]] += tot[q];
+    double E = 0.0;
+    for (int o = 0; o < a.n_orbits; ++o) E += sys->bin_w64[o] * (double)C[o];
+    cs->E = E;
+  } else {
+    cs->E = E0 + totE;
+  }
+  for (int q = 0; q < kMaxOcc; ++q) cs->counts[q] += tot[kMaxSlots + q];
+  cs->n_attempt += tot[kMaxSlots + kMaxOcc];
+  cs->n_accept += tot[kMaxSlots + kMaxOcc + 1];
+  cs->pass_base += (unsigned long long)a.n_passes;
+  const unsigned long long next = cs->draw_hi + 1ull;
+  if (next > cs->draw_base) cs->draw_base = next;
+}
+
+hipError_t launch_slab_post(const RunArgs &a, int n_chains, hipStream_t st) {
+  if (n_chains <= 0) return hipSuccess;
+  hipLaunchKernelGGL(slab_post_kernel, dim3((n_chains + 63) / 64), dim3(64), 0, st, a, n_chains);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // LDS read-throughput probe: every lane reads LDS in a loop and folds the
 // values into a checksum (written out so that nothing is optimised away).
 // mode 0: byte gathers, lanes of a wavefront 3 bytes apart (the stride of
@@ -256,7 +341,7 @@ size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_ma
   size_t img = (size_t)image_bytes(pad_sites, packed);
   size_t tab = binary ? 0 : (size_t)((n_tab * 4 + 15) & ~15);
   const int n_islot = (binary ? kMaxSlots : 0) + kMaxOcc + 2;
-  size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16;
+  size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16 + 128;
   const int nfs = binary ? 1 : n_func_max;
   size_t scratch = (size_t)nfs * max_class * sizeof(float) + (binary ? (size_t)kMaxSlots * max_class : 0);
   return img + tab + red + scratch + 64;

@@ -285,7 +285,10 @@ struct GenericField {
 template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1>
 __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int chain = args.chain_map ? args.chain_map[blockIdx.x] : (int)blockIdx.x;
+  // slabs > 1: this workgroup runs slab `slab` of its chain
+  const int n_slabs = args.slabs > 1 ? args.slabs : 1;
+  const int wg = (int)blockIdx.x / n_slabs, slab = (int)blockIdx.x - wg * n_slabs;
+  const int chain = args.chain_map ? args.chain_map[wg] : wg;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int lane = tid & 63;
@@ -318,6 +321,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   off_b += 16 * sizeof(double);
   unsigned long long *maxDraw = (unsigned long long *)(smem + off_b);
   off_b += 16;
+  // slab launches: this workgroup's totals since the launch started
+  long long *slabI = (long long *)(smem + off_b);        // [kMaxSlots + kMaxOcc + 2]
+  double *slabE = (double *)(slabI + kMaxSlots + kMaxOcc + 2);
+  off_b += 128;
   const int nfs = BIN ? 1 : sys->n_func_max;              // field values kept per class site
   float *cF = (float *)(smem + off_b);                   // [nfs][max_class]
   int8_t *cK = (int8_t *)(cF + nfs * max_class);         // [kMaxSlots][max_class] (binary)
@@ -354,15 +361,20 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     const float *__restrict__ tg = sys->flower_tables;
     for (int p = tid; p < n_tab; p += nthr) tabL[p] = tg[p];
   }
-  if (tid == 0) *maxDraw = 0ull;
+  if (tid == 0) {
+    *maxDraw = 0ull;
+    for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) slabI[q] = 0;
+    *slabE = 0.0;
+  }
   __syncthreads();
 
   const float beta = cp->beta;
   LaneRng rng;
   rng.left = 0;
   rng.draw = cs->draw_base;
-  rng.lane = (uint32_t)tid;
+  rng.lane = (uint32_t)tid | ((uint32_t)slab << 16);
   rng.chain = (uint32_t)chain;
+  const uint32_t rchain = (uint32_t)chain | ((uint32_t)slab << 24);  // block-uniform draws of this workgroup
   rng.k0 = cp->key0;
   rng.k1 = cp->key1;
 
@@ -373,8 +385,46 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   int accN[kMaxOcc] = {0, 0, 0, 0};
   int accAtt = 0, accAcc = 0;
   double accE = 0.0;
+  int n_records = 0;  // slab launches (thread 0): records written
 
   const unsigned long long pass0 = cs->pass_base;
+  // ---- slab of this workgroup: cell planes [ka, ka + U) (mod L3) are
+  // updated; the slab's top kreach planes stay frozen, so the updated sites of
+  // two slabs are more than kreach planes apart and share no cluster, and
+  // every site this workgroup reads is its own or frozen for the launch.  The
+  // slab boundaries move by a random plane offset per launch (drawn from the
+  // chain's key and pass counter, not from the state).
+  int ka = 0, U = L3;
+  if (n_slabs > 1) {
+    // phase word 0xFFFF0000: above the class-matching words (class << 12 | round,
+    // below 2^24) and below the pair-phase words (0xFFFFFFF0 - round, round < 64)
+    const int k0 = (int)uniform_int((uint32_t)L3, pass0, 0xFFFF0000u, (uint32_t)chain, cp->key0, cp->key1);
+    const int s0 = (int)(((long long)slab * L3) / n_slabs), s1 = (int)(((long long)(slab + 1) * L3) / n_slabs);
+    ka = (k0 + s0) % L3;
+    U = s1 - s0 - sc->kreach;
+  }
+  // class sites of this workgroup: list positions [r0, r0 + n0) and [r1, r1 + n1)
+  struct Ranges {
+    int r0, n0, r1, n1;
+    __device__ __forceinline__ int at(int v) const { return v < n0 ? r0 + v : r1 + (v - n0); }
+    __device__ __forceinline__ int n() const { return n0 + n1; }
+  };
+  auto ranges = [&](int c) -> Ranges {
+    const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
+    if (n_slabs == 1) return Ranges{cb, ce - cb, ce, 0};
+    cptr<int> kx = cst(sc->class_kidx + c * (L3 + 1));
+    const int kb = ka + U;
+    if (kb <= L3) return Ranges{cb + kx[ka], kx[kb] - kx[ka], ce, 0};
+    return Ranges{cb + kx[ka], kx[L3] - kx[ka], cb, kx[kb - L3]};
+  };
+  // whether the site at padded main address a lies on an updated plane
+  auto updated = [&](int a) -> bool {
+    if (n_slabs == 1) return true;
+    const int k = (a % S) / P12 - h;
+    int d = k - ka;
+    if (d < 0) d += L3;
+    return d < U;
+  };
   const bool auto_rounds = args.rounds <= 0;
   const int rounds = auto_rounds ? 0 : args.rounds;
   cptr<int> bin_off = cst(sc->bin_off);
@@ -382,10 +432,11 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   const FieldCtx fctx{sc, bin_off, gen_prog, n_slots};
 
   // n^2 - sum_o n_o^2 over the occupants of one independent set (block-uniform)
-  auto class_mixed = [&](int cb, int n) -> long long {
+  auto class_mixed = [&](const Ranges &R) -> long long {
+    const int n = R.n();
     int cnt[kMaxOcc] = {0, 0, 0, 0};
     for (int t = tid; t < n; t += nthr) {
-      const int o = img.get(sc->class_addr[cb + t]);
+      const int o = img.get(sc->class_addr[R.at(t)]);
 #pragma unroll
       for (int q = 0; q < kMaxOcc; ++q) cnt[q] += (o == q) ? 1 : 0;
     }
@@ -408,12 +459,17 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   for (int pass = 0; pass < args.n_passes; ++pass) {
     const unsigned long long gpass = pass0 + (unsigned long long)pass;
     long long m_total = 0;
+    int n_mut = sc->class_begin[n_classes];  // mutable sites this workgroup updates
+    if (n_slabs > 1) {
+      n_mut = 0;
+      for (int c = 0; c < n_classes; ++c) n_mut += ranges(c).n();
+    }
     if (KIND == 0 && auto_rounds)
-      for (int c = 0; c < n_classes; ++c)
-        m_total += class_mixed(sc->class_begin[c], sc->class_begin[c + 1] - sc->class_begin[c]);
+      for (int c = 0; c < n_classes; ++c) m_total += class_mixed(ranges(c));
     for (int c = 0; c < n_classes; ++c) {
       const int b = sc->class_sublat[c];
-      const int cb = sc->class_begin[c], ce = sc->class_begin[c + 1];
+      const Ranges R = ranges(c);
+      const int nR = R.n();
       // this lane's first halo copy of the class, loaded before the sweep so
       // that its global-memory latency overlaps the sweep
       const int hb0 = sc->halo_begin[c] + tid, hb1 = sc->halo_begin[c + 1];
@@ -425,8 +481,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         // ---- semi-grand canonical: every site of the class proposes a change
         if constexpr (BIN && SPLIT == 2) {
           const int part = tid & 1;
-          for (int t = cb + (tid >> 1); t < ce; t += (nthr >> 1)) {
-            const int pa = sc->class_addr[t];
+          for (int v = tid >> 1; v < nR; v += (nthr >> 1)) {
+            const int pa = sc->class_addr[R.at(v)];
             const int o = img.get(pa);
             int X[kMaxSlots], K[kMaxSlots];
             FIELD::bin_x(fctx, img, b, pa, o, part, X);
@@ -449,8 +505,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             }
           }
         } else
-        for (int t = cb + tid; t < ce; t += nthr) {
-          const int pa = sc->class_addr[t];
+        for (int v = tid; v < nR; v += nthr) {
+          const int pa = sc->class_addr[R.at(v)];
           const int o = img.get(pa);
           const float u = rng.next();
           if (BIN) {
@@ -491,14 +547,13 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         __syncthreads();
       } else {
         // ---- canonical: fields of all class sites, then random matchings
-        const int n = ce - cb;
-        const int n_rounds =
-            auto_rounds ? auto_canonical_rounds(n, class_mixed(cb, n), sc->class_begin[n_classes], m_total) : rounds;
+        const int n = nR;
+        const int n_rounds = auto_rounds ? auto_canonical_rounds(n, class_mixed(R), n_mut, m_total) : rounds;
         if (n_rounds > 0) {
         if constexpr (BIN && SPLIT == 2) {
           const int part = tid & 1;
           for (int t = tid >> 1; t < n; t += (nthr >> 1)) {
-            const int pa = sc->class_addr[cb + t];
+            const int pa = sc->class_addr[R.at(t)];
             int X[kMaxSlots], K[kMaxSlots];
             FIELD::bin_x(fctx, img, b, pa, img.get(pa), part, X);
 #pragma unroll
@@ -512,7 +567,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           }
         } else
         for (int t = tid; t < n; t += nthr) {
-          const int pa = sc->class_addr[cb + t];
+          const int pa = sc->class_addr[R.at(t)];
           if (BIN) {
             int K[kMaxSlots];
             cF[t] = FIELD::bin(fctx, img, b, pa, img.get(pa), K);
@@ -528,13 +583,13 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         }
         __syncthreads();
         for (int rd = 0; rd < n_rounds; ++rd) {
-          const uint32_t r = uniform_int((uint32_t)n, gpass, ((uint32_t)c << 12) | (uint32_t)rd, (uint32_t)chain,
+          const uint32_t r = uniform_int((uint32_t)n, gpass, ((uint32_t)c << 12) | (uint32_t)rd, rchain,
                                          cp->key0, cp->key1);
           for (int p = tid; p < n; p += nthr) {
             int q = (int)r - p;
             if (q < 0) q += n;
             if (p >= q) continue;  // q == p: unpaired; p > q: the partner leads
-            const int pa = sc->class_addr[cb + p], qa = sc->class_addr[cb + q];
+            const int pa = sc->class_addr[R.at(p)], qa = sc->class_addr[R.at(q)];
             const int op = img.get(pa), oq = img.get(qa);
             if (op == oq) continue;
             const float u = rng.next();
@@ -588,6 +643,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
         const int2 pq = sc->pair_addr[t];
+        if (!updated(pq.x) || !updated(pq.y)) continue;
         const int bpq = sc->pair_sublat[t];
         const int bp = bpq & 0xFF, bq = bpq >> 8;
         const int op = img.get(pq.x), oq = img.get(pq.y);
@@ -635,7 +691,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     // first site is updated before the second site's field is taken).  These
     // are the reference's unrestricted pair moves; they make every supercell
     // ergodic, however small, for the canonical ensemble.
-    if (KIND == 0 && args.global_swaps > 0) {
+    if (KIND == 0 && args.global_swaps > 0 && n_slabs == 1) {
       if (tid == 0) {
         const int n_var = sc->class_begin[n_classes];
         auto set_all = [&](int a, int old, int nw) {
@@ -706,8 +762,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     }
 
     // ---- reduction of lane accumulators and sampling
-    const bool sample_due = ((gpass + 1) % (unsigned long long)args.sample_period) == 0ull;
-    const bool flush = sample_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);
+    const bool due = ((gpass + 1) % (unsigned long long)args.sample_period) == 0ull;
+    const bool sample_due = n_slabs == 1 && due;
+    const bool record_due = n_slabs > 1 && (due || pass == args.n_passes - 1);
+    const bool flush = sample_due || record_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);
     if (flush) {
       int s = 0;
       if (BIN) {
@@ -750,6 +808,21 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         }
         int s2 = 0;
         double E;
+        if (n_slabs > 1) {
+          // running totals of this workgroup since the launch started; the
+          // chain's totals and samples are formed by slab_post_kernel
+          for (int q = 0; q < n_islot; ++q) slabI[q] += sumI[q];
+          if (!BIN)
+            for (int w = 0; w < n_waves; ++w) *slabE += redD[w];
+          if (record_due) {
+            SlabRecord *r = args.slab_rec + ((long long)chain * kSlabMax + slab) * kSlabRecords + n_records;
+            r->dE = *slabE;
+            const int base = BIN ? 0 : kMaxSlots;  // dI layout: slots, counts, attempts, accepts
+            for (int q = 0; q < base; ++q) r->dI[q] = 0;
+            for (int q = 0; q < n_islot; ++q) r->dI[base + q] = slabI[q];
+            ++n_records;
+          }
+        } else {
         if (BIN) {
           long long *C = args.binC + (long long)chain * args.n_orbits;
           for (int j = 0; j < n_slots; ++j) C[sys->slot_orbit[j]] += sumI[j];
@@ -781,6 +854,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           cs->sumEN1 += E * n1;
           cs->n_samples_acc += 1;
         }
+        }  // n_slabs == 1
       }
       __syncthreads();
     }
@@ -789,21 +863,42 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   // ---- write back occupation and RNG state
   {
     int8_t *occW = args.occ + cp->occ_offset;
-    for (int l = tid; l < n_sites; l += nthr) {
-      int b = l / n_cells;
-      int cc = l - b * n_cells;
-      int i = cc % L1;
-      int j = (cc / L1) % L2;
-      int k = cc / (L1 * L2);
-      occW[l] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
+    if (n_slabs == 1) {
+      for (int l = tid; l < n_sites; l += nthr) {
+        int b = l / n_cells;
+        int cc = l - b * n_cells;
+        int i = cc % L1;
+        int j = (cc / L1) % L2;
+        int k = cc / (L1 * L2);
+        occW[l] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
+      }
+    } else {
+      // the sites of the updated planes only (one or two plane ranges per sublattice)
+      const int plane = L1 * L2, n_sub = n_cells > 0 ? n_sites / n_cells : 0;
+      const int kb = ka + U;
+      const int lo0 = ka, hi0 = kb <= L3 ? kb : L3, hi1 = kb <= L3 ? 0 : kb - L3;
+      const int n0 = (hi0 - lo0) * plane, n1 = hi1 * plane, per = n0 + n1;
+      for (int x = tid; x < n_sub * per; x += nthr) {
+        const int b = x / per;
+        int r = x - b * per;
+        const int cc = r < n0 ? lo0 * plane + r : r - n0;
+        const int i = cc % L1, j = (cc / L1) % L2, k = cc / plane;
+        occW[b * n_cells + cc] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
+      }
     }
   }
   // all lanes advance by the same bound so the next launch starts fresh
   atomicMax(maxDraw, rng.draw);
   __syncthreads();
   if (tid == 0) {
-    cs->draw_base = *maxDraw + 1ull;
-    cs->pass_base = pass0 + (unsigned long long)args.n_passes;
+    if (n_slabs > 1) {
+      // the chain's counters advance in slab_post_kernel, after every slab
+      // of the launch has read them
+      atomicMax(&cs->draw_hi, *maxDraw);
+    } else {
+      cs->draw_base = *maxDraw + 1ull;
+      cs->pass_base = pass0 + (unsigned long long)args.n_passes;
+    }
   }
 }
 

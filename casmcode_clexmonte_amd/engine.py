@@ -76,7 +76,7 @@ def lib():
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
-                     "clexmc_set_lanes_per_site", "clexmc_last_run_lanes"]:
+                     "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -98,7 +98,7 @@ def exported_symbols() -> List[str]:
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
-            "clexmc_set_lanes_per_site", "clexmc_last_run_lanes"]
+            "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs"]
 
 
 def _p(a, t):
@@ -472,6 +472,15 @@ class Engine:
         """Padded-image row pitch of supercells added later: -1 automatic
         (bank-conflict model), or a fixed number of extra bytes per row."""
         self._check(lib().clexmc_set_lds_row_pitch(self.h, C.c_int(extra)), "clexmc_set_lds_row_pitch")
+
+    def set_slabs(self, slabs: int):
+        """Workgroups per chain: 0 automatic, 1 off, >= 2 requested."""
+        self._check(lib().clexmc_set_slabs(self.h, C.c_int(slabs)), "clexmc_set_slabs")
+
+    def last_run_slabs(self) -> int:
+        v = C.c_int()
+        self._check(lib().clexmc_last_run_slabs(self.h, C.byref(v)), "clexmc_last_run_slabs")
+        return v.value
 
     def set_global_swaps(self, per_pass: int):
         self._check(lib().clexmc_set_global_swaps(self.h, C.c_int(per_pass)), "clexmc_set_global_swaps")

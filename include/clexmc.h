@@ -228,6 +228,26 @@ clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
  * library caps the block at 512). */
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
 
+/* Slab decomposition of large chains (no reference counterpart: the
+ * reference runs one chain on one thread, occupation_metropolis.hh:72-123).
+ * A chain split into S slabs along the third cell axis runs on S workgroups
+ * at once; workgroup s updates only the sites of its slab's planes except a
+ * frozen band, as thick as the clusters reach along that axis, at the slab's
+ * top, so updated sites of different slabs share no cluster and every site a
+ * workgroup reads is its own or frozen.  The slab boundaries move by a random
+ * plane offset (from the chain's key and pass counter) every launch of at
+ * most 4 passes; each slab records its additions at the sampled passes and a
+ * follow-up kernel forms the samples and chain totals.  slabs: 0 = automatic
+ * (large chains in batches that do not fill the device, about
+ * total_sites / compute_units sites per workgroup), 1 = off, 2..64 = that
+ * many slabs where each keeps an updated plane.  Plain Metropolis runs
+ * (clexmc_run_passes) only; graph captures and the autotuner run whole chains.
+ * Canonical runs under slabs skip the serial random-pair swaps. */
+clexmc_status clexmc_set_slabs(clexmc_handle *h, int slabs);
+/* Slabs per chain of the last clexmc_run_passes (the largest over its
+ * launches; 1 = whole chains). */
+clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs);
+
 /* Canonical runs: serial swaps of uniformly random site pairs per pass
  * (default 1), beside the parallel independent-set matchings and pair
  * phases.  The pair may share clusters (the reference's unrestricted

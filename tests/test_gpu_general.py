@@ -171,6 +171,7 @@ def test_largest_shapes_packed_automatically():
             for i, d in enumerate(shapes)]
     eng.load_chains(sids, occs, [900.0, 1500.0], [1, 2])
     eng.run_passes("canonical", 3, 1, canonical_rounds=2)
+    assert eng.last_run_slabs() > 1  # the large chain runs as slabs (automatic)
     _energy_tracking(eng, flat, shapes, occs, 2)
     eng.close()
 
@@ -183,5 +184,6 @@ def test_largest_shapes_packed_automatically():
     occs = [random_occupation_counts(counts, 11 + i) for i in range(2)]
     eng.load_chains([sid, sid], occs, [600.0, 2000.0], [3, 4])
     eng.run_passes("canonical", 2, 1, canonical_rounds=2)
+    assert eng.last_run_slabs() > 1
     _energy_tracking(eng, flat5, [sc.dims] * 2, occs, 2)
     eng.close()

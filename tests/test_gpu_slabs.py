@@ -1,0 +1,151 @@
+"""Slab decomposition of large chains (several workgroups per chain).
+
+* Bookkeeping under slabs, general table path (HCP ternary 16x16x12, 4 slabs)
+  and binary integer path (FCC 24^3, 6 slabs), canonical and semi-grand
+  canonical: each chain's running energy equals the oracle's energy of its
+  final occupation (general path: within the FP32 accumulation bound
+  4e-6 eV x sqrt(accepted moves); binary path: 1e-9 relative, exact counts),
+  the last recorded sample equals the final state, canonical runs keep their
+  composition, sample counts follow the sample period across launches.
+* Statistics: slab runs against whole-chain runs of the same system (FCC
+  binary 16^3 semi-grand canonical; HCP ternary 12x12x8 canonical, both
+  sides from the same pre-equilibrated states): mean energy and composition
+  per chain within 3 sigma (chain-to-chain standard errors of both sides).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.system import FlatCE, Supercell
+from casmcode_clexmonte_amd.workloads import (composition_counts, fcc_binary_ce, hcp_ternary_ce, random_occupation,
+                                              random_occupation_counts)
+
+pytestmark = pytest.mark.gpu
+
+NSIG = 3.0
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def _check_states(eng, flat, dims, occ0, kind, exact):
+    osys = O.OracleSystem(flat, dims)
+    E_s, cnt_s = eng.read_samples()
+    for c in range(len(occ0)):
+        st = eng.read_state(c)
+        Eref = osys.energy(st["occ"])
+        n_acc = max(1, st["n_accept"])
+        tol = 1e-9 * max(1.0, abs(Eref)) if exact else 4e-6 * np.sqrt(n_acc) + 1e-9 * abs(Eref)
+        assert abs(st["E"] - Eref) <= tol, (c, st["E"], Eref, tol)
+        assert st["n_accept"] > 0
+        nb = np.bincount(st["occ"], minlength=4)
+        if kind == "canonical":
+            assert np.array_equal(nb, np.bincount(occ0[c], minlength=4))
+        assert E_s[c, -1] == st["E"]
+        assert np.array_equal(cnt_s[c, -1, :4], nb[:4])
+
+
+@pytest.mark.parametrize("kind", ["canonical", "semigrand_canonical"])
+def test_slabs_general_path_bookkeeping(kind):
+    _print_case(f"slabs general path hcp 16x16x12 {kind}")
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    dims = (16, 16, 12)
+    sc = Supercell(ce.prim, dims)
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=4, seed=1)
+    nch = 4
+    occs = [random_occupation_counts(composition_counts(sc.n_sites, (0.4, 0.3, 0.3)), 20 + c) for c in range(nch)]
+    chem = [[0.0, 0.0, 0.0, 0.0, 0.02, -0.03]] * nch if kind != "canonical" else None
+    eng.load_chains([sid] * nch, occs, np.linspace(900.0, 1500.0, nch), 50 + np.arange(nch), chem=chem)
+    eng.set_slabs(4)
+    eng.run_passes(kind, 12, 3)
+    assert eng.last_run_slabs() == 4
+    eng.run_passes(kind, 6, 3)
+    E_s, _ = eng.read_samples()
+    assert E_s.shape[1] == 12 // 3 + 6 // 3
+    _check_states(eng, flat, dims, occs, kind, exact=False)
+    eng.close()
+
+
+@pytest.mark.parametrize("kind", ["canonical", "semigrand_canonical"])
+def test_slabs_binary_path_exact(kind):
+    _print_case(f"slabs binary path fcc 24^3 {kind}")
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    dims = (24, 24, 24)
+    sc = Supercell(ce.prim, dims)
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=4, seed=2)
+    nch = 3
+    occs = [random_occupation(sc.n_sites, 0.5, 70 + c) for c in range(nch)]
+    chem = [[0.0, -0.05]] * nch if kind != "canonical" else None
+    eng.load_chains([sid] * nch, occs, [600.0, 1000.0, 1400.0], 90 + np.arange(nch), chem=chem)
+    eng.set_slabs(6)
+    eng.run_passes(kind, 9, 1)
+    assert eng.last_run_slabs() == 6
+    E_s, _ = eng.read_samples()
+    assert E_s.shape[1] == 9
+    _check_states(eng, flat, dims, occs, kind, exact=True)
+    eng.close()
+
+
+def _chain_means(eng, kind, n_passes, burn, nc):
+    eng.run_passes(kind, n_passes, 1)
+    E, cnt = eng.read_samples()
+    return E[:, burn:].mean(1) / nc, cnt[:, burn:, 1].mean(1) / nc
+
+
+def _agree(name, a, b):
+    tol = NSIG * np.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
+    print(f"{name}: slabs {a.mean():.6f} whole {b.mean():.6f} tol {tol:.6f}")
+    assert abs(a.mean() - b.mean()) <= tol, (name, a.mean(), b.mean(), tol)
+
+
+def test_slabs_statistics_match_whole_chains():
+    _print_case("slabs statistics fcc 16^3 sgc and hcp 12x12x8 canonical")
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (16, 16, 16))
+    nch = 32
+    occs = [random_occupation(sc.n_sites, 0.5, 300 + c) for c in range(nch)]
+    chem = [[0.0, 0.02]] * nch
+    res = {}
+    for slabs in (4, 1):
+        eng = Engine(flat)
+        sid = eng.add_supercell(sc, n_pair_phases=4, seed=3)
+        eng.load_chains([sid] * nch, occs, [1000.0] * nch, (slabs << 20) + np.arange(nch), chem=chem)
+        eng.set_slabs(slabs)
+        res[slabs] = _chain_means(eng, "semigrand_canonical", 400, 100, sc.n_cells)
+        assert eng.last_run_slabs() == slabs
+        eng.close()
+    _agree("fcc sgc E/cell", res[4][0], res[1][0])
+    _agree("fcc sgc x_B", res[4][1], res[1][1])
+
+    # canonical: both sides start from the same states, equilibrated by 600
+    # whole-chain passes (slab launches update fewer sites per pass and skip
+    # the serial random-pair swaps, so they equilibrate more slowly per pass)
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 8))
+    nch = 16
+    occs = [random_occupation_counts(composition_counts(sc.n_sites, (0.4, 0.3, 0.3)), 400 + c) for c in range(nch)]
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=8, seed=4)
+    eng.load_chains([sid] * nch, occs, [1200.0] * nch, 7000 + np.arange(nch))
+    eng.set_slabs(1)
+    eng.run_passes("canonical", 600, 600)
+    occs = [eng.read_state(c)["occ"].copy() for c in range(nch)]
+    eng.close()
+    res = {}
+    for slabs in (3, 1):
+        eng = Engine(flat)
+        sid = eng.add_supercell(sc, n_pair_phases=8, seed=4)
+        eng.load_chains([sid] * nch, occs, [1200.0] * nch, (slabs << 20) + np.arange(nch))
+        eng.set_slabs(slabs)
+        res[slabs] = _chain_means(eng, "canonical", 400, 50, sc.n_cells)
+        assert eng.last_run_slabs() == slabs
+        eng.close()
+    _agree("hcp canonical E/cell", res[3][0], res[1][0])
