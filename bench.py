@@ -216,6 +216,7 @@ def workload_c3(args):
     eng.run_passes("canonical", 2, 1, canonical_rounds=2)
     info = eng.last_run_info()
     rate, wall, kms = _run_timed(eng, "canonical", args.extra_passes, n)
+    info["slabs_per_chain_max"] = eng.last_run_slabs()
     drift = eng.reanchor_energies()
     sites = sum(len(o) for o in b.occ)
     eng.close()
@@ -240,6 +241,7 @@ def workload_c5(args):
     eng.load_chains([sid] * n, b.occ, b.temperatures, b.seeds)
     tune = eng.autotune("canonical", trial_passes=3)
     rate, wall, kms = _run_timed(eng, "canonical", args.extra_passes, n)
+    tune["slabs_per_chain"] = eng.last_run_slabs()
     drift = eng.reanchor_energies()
     eng.close()
     return {"workload": "bcc_quaternary_64x64x64_canonical_equiatomic_32chains_T300-2000",

@@ -1247,6 +1247,7 @@ struct RunPlan {
   int sc_id = 0;                // supercell of a single-shape launch
   bool one_shape = false;       // every chain of the launch on supercell sc_id
   int slabs = 1;                // workgroups per chain (slab decomposition along k)
+  bool threads_auto = false;    // block size chosen by the planner (not requested)
   unsigned long long pass_start = 0;  // the chains' pass counter before the run
   RunArgs a;
   LearnArgs la;
@@ -1326,6 +1327,7 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   // one lane per class site up to 1024 threads; blocks above 512 threads need
   // the specialised kernel (the ahead-of-time kernels are bounded at 512)
   int threads = threads_override ? threads_override : h->block_size;
+  P.threads_auto = threads == 0;
   if (threads == 0) threads = std::min(kMaxJitThreads, std::max(64, ((max_class + 63) / 64) * 64));
   const int packing = packing_override ? packing_override : h->packing;
   const int n_tab = h->binary ? 0 : h->n_flower_tables;
@@ -1515,7 +1517,7 @@ static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &p
     P->slabs = S;
     // a slab's class ranges are a fraction of the class: 512-thread blocks
     // (measured faster than 1024 for slabs of the HCP and BCC workloads)
-    if (h->block_size == 0 && P->threads > kAotMaxThreads) {
+    if (P->threads_auto && P->threads > kAotMaxThreads) {
       P->threads = kAotMaxThreads;
       if (P->jit) P->jit = jit_kernel(h, P->sc_id, P->kind, P->packed, P->lanes, P->threads);
     }
@@ -1782,6 +1784,9 @@ clexmc_status clexmc_autotune(clexmc_handle *h, int kind, int trial_passes, int 
           continue;
         if (packing == 2 && (P.learned || !P.packed)) continue;
         if (P.lanes != lanes || P.threads != t) continue;
+        HIP_TRY(choose_slabs(h, {&P}));  // trials run as the real runs will (slabs included)
+        if (P.slabs > 1)
+          HIP_TRY(hipMemcpy(&P.pass_start, &h->d_state[0].pass_base, sizeof(P.pass_start), hipMemcpyDeviceToHost));
         HIP_TRY(plan_attributes(P, h));
         HIP_TRY(hipEventRecord(h->ev0, h->stream));
         HIP_TRY(plan_launch(P, h));
