@@ -1441,12 +1441,10 @@ static hipError_t plan_launch_one(const RunPlan &P, const RunArgs &args_in, clex
 // slab_post_kernel (samples, chain totals, pass and draw counters).  The slab
 // boundaries move between launches.
 
-static std::vector<int> slab_epochs(unsigned long long pass_start, int n_passes, int sample_period) {
+static std::vector<int> slab_epochs(int n_passes) {
   std::vector<int> lens;
   int done = 0;
   while (done < n_passes) {
-    (void)pass_start;
-    (void)sample_period;
     const int len = std::min(n_passes - done, kSlabEpochPasses);
     lens.push_back(len);
     done += len;
@@ -1467,7 +1465,7 @@ static hipError_t plan_launch_window(const RunPlan &P, clexmc_handle *h, hipStre
 static hipError_t plan_launch(const RunPlan &P, clexmc_handle *h, hipStream_t st = nullptr) {
   hipStream_t s = st ? st : h->stream;
   if (P.slabs <= 1) return plan_launch_one(P, P.a, h, s);
-  for (int len : slab_epochs(P.pass_start, P.a.n_passes, P.a.sample_period)) {
+  for (int len : slab_epochs(P.a.n_passes)) {
     const hipError_t e = plan_launch_window(P, h, s, len);
     if (e != hipSuccess) return e;
   }
