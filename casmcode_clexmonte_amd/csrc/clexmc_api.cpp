@@ -1151,7 +1151,6 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra) {
 clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
-  ARG_CHECK(h->binary, "the N-fold way is implemented for binary cluster expansions");
   ARG_CHECK(n_events >= 0 && events_per_block >= 1 && n_events % events_per_block == 0,
             "n_events must be a non-negative multiple of events_per_block");
   ARG_CHECK(blocks_out || n_events == 0, "blocks_out is NULL");
@@ -1184,7 +1183,7 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
   na.blocks = d_blocks;
   hipError_t e = hipMemsetAsync(d_blocks, 0, nb * sizeof(double), h->stream);
-  if (e == hipSuccess) e = launch_nfold(na, h->n_chains, 256, lds, h->stream);
+  if (e == hipSuccess) e = launch_nfold(na, h->binary != 0, h->n_chains, 256, lds, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(blocks_out, d_blocks, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_blocks);

@@ -131,7 +131,7 @@ hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st);
 hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t st);
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
 size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed);
-hipError_t launch_nfold(const NfoldArgs &a, int n_chains, int threads, size_t lds, hipStream_t st);
+hipError_t launch_nfold(const NfoldArgs &a, bool binary, int n_chains, int threads, size_t lds, hipStream_t st);
 size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max);
 
 #endif  // __HIPCC_RTC__

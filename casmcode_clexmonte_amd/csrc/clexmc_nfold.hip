@@ -1,20 +1,26 @@
 // N-fold way (rejection-free, Bortz-Kalos-Lebowitz) semi-grand canonical
-// sampling of the binary path: the reference's nfold method
+// sampling: the reference's nfold method
 // (include/casm/clexmonte/nfold/nfold_impl.hh:18-98: events are the
 // semi-grand canonical changes of single sites; rates
 // src/casm/clexmonte/nfold/nfold_events.cc:28-62: 1 if dE <= 0, else
 // exp(-beta dE), dE the semi-grand potential change).
 //
+// The events are those of the plain semi-grand canonical proposal (a site
+// and a uniformly chosen other occupant): a site's rate is the mean
+// Metropolis acceptance of its changes, (1/(n_occ-1)) sum_o' min(1,
+// exp(-beta dOmega)); binary expansions have one change per site.
+//
 // One workgroup per chain.  LDS holds the padded int8 image, the rate of
-// every site's flip and its sums over groups of 64 sites.  Per event:
+// every site and its sums over groups of 64 sites.  Per event:
 //  * wavefront 0 forms the total rate R, draws u, finds the group by a scan
 //    over per-lane chunks of group sums, then the site by a scan of the
 //    group's 64 rates (all prefix sums are wavefront scans);
 //  * the current state is weighted by its expected residence time
 //    w = n_mutable / R in Metropolis steps (the jump chain visits states in
 //    proportion to pi(x) R(x); weights 1/R recover pi);
-//  * one lane applies the flip (exact integer cluster counts, every periodic
-//    copy of the site), then the lanes recompute the rates of the site and
+//  * one lane applies the change (the new occupant drawn in proportion to
+//    its rate; binary: exact integer cluster counts, general: FP64 energy;
+//    every periodic copy of the site), then the lanes recompute the rates of the site and
 //    of its interaction neighbours (each neighbour once) and add the changes
 //    to the group sums with LDS float atomics;
 //  * every 128 events the group sums are summed again from the rates, so
@@ -62,6 +68,7 @@ size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max) {
          align16((size_t)n_groups * 4) + 2 * align16((size_t)nb_max * 4) + 16;
 }
 
+template <bool BIN>
 __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
   extern __shared__ __align__(16) unsigned char smem[];
   const RunArgs &args = na.a;
@@ -114,15 +121,31 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
     const int i = c % L1, j = (c / L1) % L2, k = c / (L1 * L2);
     return b * S + (k + h) * P12 + (j + h) * P1 + (i + h);
   };
-  // flip rate of the site at padded address pa on sublattice b (0 for fixed
-  // sites), and its cluster sums K (which do not depend on the site itself)
-  auto site_rate_at = [&](int b, int pa, int (&K)[kMaxSlots]) -> float {
-    if (sys->n_occ[b] < 2) return 0.0f;
-    const int o = img.get(pa);
-    const float F = binary_field(img, pa, o, cst(sc->bin_sub + b), bin_off, n_slots, K);
-    const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
-    const float dE = (o ? -2.0f : 2.0f) * F - (o ? (chem0 - chem1) : (chem1 - chem0));
+  const float *tab = sys->flower_tables;
+  cptr<int> gen_prog = cst(sc->gen_prog);
+  // rate of one change o -> nw of a general site with cluster sums G
+  auto change_rate = [&](int b, int o, int nw, const float (&G)[kMaxFunc]) -> float {
+    const float dE = general_dE(sys, b, o, nw, G) - (cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o]);
     return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
+  };
+  // rate of the site at padded address pa on sublattice b (0 for fixed
+  // sites); binary: its cluster sums K (which do not depend on the site itself)
+  auto site_rate_at = [&](int b, int pa, int (&K)[kMaxSlots]) -> float {
+    const int no = sys->n_occ[b];
+    if (no < 2) return 0.0f;
+    const int o = img.get(pa);
+    if (BIN) {
+      const float F = binary_field(img, pa, o, cst(sc->bin_sub + b), bin_off, n_slots, K);
+      const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
+      const float dE = (o ? -2.0f : 2.0f) * F - (o ? (chem0 - chem1) : (chem1 - chem0));
+      return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
+    }
+    float G[kMaxFunc];
+    general_field(img, pa, cst(sc->gen_sub + b), gen_prog, tab, G);
+    float r = 0.0f;
+    for (int q = 0; q < no; ++q)
+      if (q != o) r += change_rate(b, o, q, G);
+    return r / (float)(no - 1);
   };
   auto site_rate = [&](int l) -> float {
     int K[kMaxSlots];
@@ -142,15 +165,17 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
   __syncthreads();
 
   // per-chain running quantities (lane 0 of the workgroup)
-  long long C[kMaxSlots] = {0, 0, 0, 0, 0, 0, 0, 0};  // cluster-count changes per slot
-  long long n1 = cs->counts[1];
+  long long C[kMaxSlots] = {0, 0, 0, 0, 0, 0, 0, 0};  // binary: cluster-count changes per slot
+  long long cnt[kMaxOcc];
+  for (int q = 0; q < kMaxOcc; ++q) cnt[q] = cs->counts[q];
+  double E_run = cs->E;  // general: running energy
   double W = 0.0, WE = 0.0, WN = 0.0, WE2 = 0.0, WN2 = 0.0;
   long long n_done = 0;
   double n_mutable = 0.0;
   for (int b = 0; b < sys->n_sublat; ++b)
     if (sys->n_occ[b] >= 2) n_mutable += (double)n_cells;
-  double E0 = 0.0;  // energy of the exact counts at launch
-  {
+  double E0 = 0.0;  // binary: energy of the exact counts at launch
+  if (BIN) {
     const long long *C0 = args.binC + (long long)chain * args.n_orbits;
     for (int a = 0; a < args.n_orbits; ++a) E0 += sys->bin_w64[a] * (double)C0[a];
   }
@@ -193,12 +218,16 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
         // a group sum left positive by rounding with no positive rate in the
         // group: sum the groups again and draw once more
         sel[0] = (site < 0 && R > 0.0f && !retried) ? -2 : site;
+        sel[1] = __float_as_int(u01(r.y));  // draws the new occupant (general sites)
         if (site >= 0) {
           // weight of the current state: expected Metropolis steps spent in it
           double E = E0;
-          for (int j = 0; j < n_slots; ++j) E += sys->bin_w64[sys->slot_orbit[j]] * (double)C[j];
+          if (BIN)
+            for (int j = 0; j < n_slots; ++j) E += sys->bin_w64[sys->slot_orbit[j]] * (double)C[j];
+          else
+            E = E_run;
           const double w = n_mutable / (double)R;
-          const double x = (double)n1;
+          const double x = (double)cnt[1];
           W += w;
           WE += w * E;
           WN += w * x;
@@ -219,14 +248,38 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
     retried = false;
     if (site < 0) break;  // no event has a positive rate (uniform across the workgroup)
 
-    // ---- apply the flip (one lane); its cluster sums K come with its new rate below
+    // ---- apply the change (one lane); binary: its cluster sums K come with
+    // its new rate below; general: the new occupant in proportion to its rate
     int o_flip = 0;
     if (tid == 0) {
       const int b = site / n_cells;
       const int pa = main_addr(site);
       const int o = img.get(pa);
       o_flip = o;
-      n1 += o ? -1 : 1;
+      int nw = 1 - o;
+      if (!BIN) {
+        const int no = sys->n_occ[b];
+        float G[kMaxFunc];
+        general_field(img, pa, cst(sc->gen_sub + b), gen_prog, tab, G);
+        float rq[kMaxOcc] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
+        for (int q = 0; q < no; ++q)
+          if (q != o) {
+            rq[q] = change_rate(b, o, q, G);
+            tot += rq[q];
+          }
+        const float t = __int_as_float(sel[1]) * tot;
+        float acc = 0.0f;
+        nw = -1;
+        for (int q = 0; q < no; ++q) {
+          if (q == o) continue;
+          acc += rq[q];
+          if (nw < 0 && t < acc) nw = q;
+        }
+        if (nw < 0) nw = (o + 1 < no) ? no - 1 : no - 2;  // rounding: the last other occupant
+        E_run += (double)general_dE(sys, b, o, nw, G);
+      }
+      cnt[o] -= 1;
+      cnt[nw] += 1;
       // every periodic copy of the site
       const int r = pa - b * S;
       const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
@@ -241,7 +294,7 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
           for (int di = -m1; di <= m1; ++di) {
             const int qi = pi + di * L1;
             if (qi < 0 || qi >= P1) continue;
-            img.p[b * S + qk * P12 + qj * P1 + qi] = (uint8_t)(1 - o);
+            img.p[b * S + qk * P12 + qj * P1 + qi] = (uint8_t)nw;
           }
         }
       }
@@ -270,7 +323,7 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
           const float d = nr - rate[lt];
           rate[lt] = nr;
           atomicAdd(&G[lt >> 6], d);
-          if (t == 0) {
+          if (BIN && t == 0) {
             // the flipped site itself (first in the list, lane 0): exact cluster-count change
             const int sk = o_flip ? -2 : 2;
 #pragma unroll
@@ -302,25 +355,30 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
   int8_t *occW = args.occ + cp->occ_offset;
   for (int l = tid; l < n_sites; l += nthr) occW[l] = (int8_t)img.get(main_addr(l));
   if (tid == 0) {
-    long long *Cg = args.binC + (long long)chain * args.n_orbits;
-    for (int j = 0; j < n_slots; ++j) Cg[sys->slot_orbit[j]] += C[j];
-    double E = 0.0;
-    for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)Cg[a];
-    cs->E = E;
-    const long long d1 = n1 - cs->counts[1];
-    cs->counts[1] = n1;
-    cs->counts[0] -= d1;
+    if (BIN) {
+      long long *Cg = args.binC + (long long)chain * args.n_orbits;
+      for (int j = 0; j < n_slots; ++j) Cg[sys->slot_orbit[j]] += C[j];
+      double E = 0.0;
+      for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)Cg[a];
+      cs->E = E;
+    } else {
+      cs->E = E_run;
+    }
+    for (int q = 0; q < kMaxOcc; ++q) cs->counts[q] = cnt[q];
     cs->n_attempt += n_done;
     cs->n_accept += n_done;
     cs->pass_base = ev0 + (unsigned long long)na.n_events;
   }
 }
 
-hipError_t launch_nfold(const NfoldArgs &a, int n_chains, int threads, size_t lds, hipStream_t st) {
-  hipError_t e = hipFuncSetAttribute((const void *)nfold_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
+hipError_t launch_nfold(const NfoldArgs &a, bool binary, int n_chains, int threads, size_t lds, hipStream_t st) {
+  const void *fn = binary ? (const void *)nfold_kernel<true> : (const void *)nfold_kernel<false>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(nfold_kernel, dim3(n_chains), dim3(threads), lds, st, a);
+  if (binary)
+    hipLaunchKernelGGL(nfold_kernel<true>, dim3(n_chains), dim3(threads), lds, st, a);
+  else
+    hipLaunchKernelGGL(nfold_kernel<false>, dim3(n_chains), dim3(threads), lds, st, a);
   return hipGetLastError();
 }
 

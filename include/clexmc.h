@@ -265,10 +265,15 @@ clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
 clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
 
 /* N-fold way (rejection-free) semi-grand canonical runs of every loaded
- * chain (binary CEs; supercells up to 262,144 sites): the reference's nfold
- * method (include/casm/clexmonte/nfold/nfold_impl.hh:18-98) with its event
- * rates (src/casm/clexmonte/nfold/nfold_events.cc:28-62: 1 if the
- * semi-grand potential change dE <= 0, else exp(-beta dE)).  Each chain makes
+ * chain (supercells up to 262,144 sites): the reference's nfold method
+ * (include/casm/clexmonte/nfold/nfold_impl.hh:18-98) with its event rates
+ * (src/casm/clexmonte/nfold/nfold_events.cc:28-62: 1 if the semi-grand
+ * potential change dE <= 0, else exp(-beta dE)).  The events are the plain
+ * semi-grand canonical proposal's (a site and a uniformly chosen other
+ * occupant), so a site's rate is (1/(n_occ-1)) times the sum of its
+ * changes' rates, and a chosen site takes a new occupant in proportion to
+ * the change's rate.  Binary expansions keep exact cluster counts, others
+ * an FP64 running energy of FP32 changes.  Each chain makes
  * n_events events; every state visited is weighted by its expected residence
  * time w = n_mutable / R (Metropolis steps; R the total rate), so weighted
  * averages are Boltzmann averages.  blocks_out[n_chains][n_events /

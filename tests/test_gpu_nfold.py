@@ -8,13 +8,20 @@
   occupant counts a following Metropolis pass samples match the occupation.
 * Against the plain Metropolis kernel on FCC 12^3 at a low-acceptance point
   (T = 400 K, mu = -0.05 eV): mean energy and composition within 3 sigma.
+* General (table) expansions: ternary HCP 2x2x1 (2 sublattices, 3^8
+  states) and quaternary BCC 2x2x2 (4^8 states), T = 1000 K: weighted
+  averages of E, E^2, N_1, N_1^2 over 128 chains within 3 sigma of exact
+  enumeration; final energies equal the oracle's within the FP32
+  accumulation bound (4e-6 eV x sqrt(events)); occupant counts follow the
+  occupations.
 """
 import numpy as np
 import pytest
 
 import oracle as O
 from casmcode_clexmonte_amd.engine import Engine
-from casmcode_clexmonte_amd.system import BasisSet, ClusterExpansion, FlatCE, Supercell, fcc_prim
+from casmcode_clexmonte_amd.system import (BasisSet, ClusterExpansion, FlatCE, Supercell, bcc_prim, fcc_prim,
+                                           hcp_prim)
 from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
 
 pytestmark = pytest.mark.gpu
@@ -115,4 +122,54 @@ def test_nfold_vs_metropolis_low_acceptance():
         tol = NSIG * np.sqrt(g.var(ddof=1) / nch + n.var(ddof=1) / nch)
         print(f"{name}: metropolis {g.mean():.6f}  nfold {n.mean():.6f}  tol {tol:.6f}")
         assert abs(g.mean() - n.mean()) <= tol, (name, g.mean(), n.mean(), tol)
+    eng.close()
+
+
+def _hcp_ternary_small(seed=0):
+    prim = hcp_prim(1.0, occupants=("A", "B", "C"))
+    bs = BasisSet(prim, {2: 1.01, 3: 1.01}, "chebychev")
+    return FlatCE(ClusterExpansion(bs, np.random.default_rng(seed).normal(0, 0.04, bs.n_functions)))
+
+
+def _bcc_quaternary_small(seed=1):
+    prim = bcc_prim(1.0, ("A", "B", "C", "D"))
+    bs = BasisSet(prim, {2: 0.87, 3: 1.01}, "chebychev")
+    return FlatCE(ClusterExpansion(bs, np.random.default_rng(seed).normal(0, 0.04, bs.n_functions)))
+
+
+@pytest.mark.parametrize("system", ["hcp_ternary", "bcc_quaternary"])
+def test_nfold_general_exact_enumeration(system):
+    _print_case(f"nfold general path exact enumeration {system}")
+    make, dims, nocc, chem_b = {"hcp_ternary": (_hcp_ternary_small, (2, 2, 1), 3, [0.0, 0.02, -0.03]),
+                                "bcc_quaternary": (_bcc_quaternary_small, (2, 2, 2), 4, [0.0, 0.03, -0.02, 0.01])}[system]
+    flat = make()
+    sc = Supercell(flat.ce.prim, dims)
+    T = 1000.0
+    chem = list(chem_b) * flat.n_sublat
+    osys = O.OracleSystem(flat, sc.dims)
+    exact = osys.enumerate("sgc", T, chem=chem)
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    rng = np.random.default_rng(9)
+    occs = [rng.integers(0, nocc, sc.n_sites).astype(np.int8) for _ in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 2000 + np.arange(nch), chem=[chem] * nch)
+    n_events = 20000
+    blocks = eng.nfold_run(n_events, 500)
+    assert blocks.shape == (nch, 40, 5) and np.all(blocks[:, :, 0] > 0)
+    per = _weighted(blocks, 2)
+    for k in ("E", "E2", "n1", "n12"):
+        m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
+        print(f"{k}: nfold {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
+        assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
+    for c in range(0, nch, 16):
+        st = eng.read_state(c)
+        Eref = osys.energy(st["occ"])
+        assert abs(st["E"] - Eref) <= 4e-6 * np.sqrt(n_events) + 1e-9 * abs(Eref), (st["E"], Eref)
+    eng.clear_samples()
+    eng.run_passes("semigrand_canonical", 1, 1)
+    _, cnt = eng.read_samples()
+    for c in range(nch):
+        nb = np.bincount(eng.read_state(c)["occ"], minlength=4)
+        assert np.array_equal(cnt[c, -1, :4], nb[:4])
     eng.close()
