@@ -664,6 +664,7 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       class_sub(sc->n_classes);
   std::vector<int> halo_begin(1, 0);
   std::vector<int2> halo;
+  std::vector<int> halo_site(cb.back() + 1, 0);  // first halo entry of each class site
   std::vector<int> img;
   std::vector<char> seen(N, 0);
   int max_class = 1;
@@ -684,11 +685,13 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       class_addr[t] = pad_addr(b, i, j, k);
       class_lin[t] = l;
       images(l, img);
+      halo_site[t] = (int)halo.size();
       for (int a : img) halo.push_back(make_int2(class_addr[t], a));
     }
     class_sub[c] = b0 < 0 ? 0 : b0;
     halo_begin.push_back((int)halo.size());
   }
+  halo_site[cb.back()] = (int)halo.size();
   for (int l = 0; l < N; ++l) {
     int b = l / d.n_cells;
     if (h->n_occ[b] >= 2) ARG_CHECK(seen[l], "a variable site is in no class");
@@ -867,6 +870,7 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
     if (ordered) HIP_TRY(upload(h, A, kidx.data(), kidx.size(), (int **)&d.class_kidx));
   }
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
+  HIP_TRY(upload(h, A, halo_site.data(), halo_site.size(), (int **)&d.halo_site));
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
   HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
   HIP_TRY(upload(h, A, pair_begin.data(), pair_begin.size(), (int **)&d.pair_begin));

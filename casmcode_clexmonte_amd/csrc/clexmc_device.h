@@ -131,6 +131,7 @@ struct DevSupercell {
   const int *class_addr;   // padded LDS address of each class site
   const int *halo_begin;   // [n_classes+1]
   const int2 *halo;        // (src, dst) padded addresses
+  const int *halo_site;    // [n_class_sites+1] first halo entry of each class site
   const int *entry_off;    // [n_entry_sites] padded relative offset (general path)
   const BinSub *bin_sub;   // [n_sublat]
   const int *bin_off;      // binary site programs

@@ -472,7 +472,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       const int nR = R.n();
       // this lane's first halo copy of the class, loaded before the sweep so
       // that its global-memory latency overlaps the sweep
-      const int hb0 = sc->halo_begin[c] + tid, hb1 = sc->halo_begin[c + 1];
+      const int hb0 = sc->halo_begin[c] + tid, hb1 = n_slabs == 1 ? sc->halo_begin[c + 1] : 0;
       const int2 hc0 = hb0 < hb1 ? sc->halo[hb0] : make_int2(0, 0);
       cptr<BinSub> bs = cst(sc->bin_sub + b);
       cptr<GenSub> gs = cst(sc->gen_sub + b);
@@ -622,11 +622,22 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         }
         }  // n_rounds > 0
       }
-      // ---- halo copies of this class
-      if (hb0 < hb1) img.copy(hc0.y, hc0.x);
-      for (int t = hb0 + nthr; t < hb1; t += nthr) {
-        const int2 hc = sc->halo[t];
-        img.copy(hc.y, hc.x);
+      // ---- halo copies of this class (slabs: of this workgroup's sites only;
+      // the other sites are unchanged in this image)
+      if (n_slabs == 1) {
+        if (hb0 < hb1) img.copy(hc0.y, hc0.x);
+        for (int t = hb0 + nthr; t < hb1; t += nthr) {
+          const int2 hc = sc->halo[t];
+          img.copy(hc.y, hc.x);
+        }
+      } else {
+        const int a0 = sc->halo_site[R.r0], a1 = sc->halo_site[R.r0 + R.n0];
+        const int b0 = sc->halo_site[R.r1], b1 = sc->halo_site[R.r1 + R.n1];
+        const int na = a1 - a0, nh = na + (b1 - b0);
+        for (int t = tid; t < nh; t += nthr) {
+          const int2 hc = sc->halo[t < na ? a0 + t : b0 + (t - na)];
+          img.copy(hc.y, hc.x);
+        }
       }
       __syncthreads();
     }
