@@ -839,6 +839,18 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       nb_max = std::max(nb_max, nb_begin[b + 1] - nb_begin[b]);
     }
     d.nb_max = nb_max;
+    // whether the periodic supercell keeps every list's sites distinct
+    d.nb_distinct = 1;
+    auto md = [](int x, int m) { return ((x % m) + m) % m; };
+    for (int b = 0; b < h->n_sublat && d.nb_distinct; ++b)
+      for (int p = nb_begin[b]; p < nb_begin[b + 1] && d.nb_distinct; ++p)
+        for (int q = p + 1; q < nb_begin[b + 1]; ++q) {
+          const int4 x = nb_cell[p], y = nb_cell[q];
+          if (x.x == y.x && md(x.y - y.y, L1) == 0 && md(x.z - y.z, L2) == 0 && md(x.w - y.w, L3) == 0) {
+            d.nb_distinct = 0;
+            break;
+          }
+        }
     HIP_TRY(upload(h, A, nb_begin.data(), nb_begin.size(), (int **)&d.nb_begin));
     HIP_TRY(upload(h, A, nb_cell.data(), nb_cell.size(), (int4 **)&d.nb_cell));
     // slabs: reach along k of the clusters (largest |dk| of a flower site) and
@@ -871,6 +883,11 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   }
   HIP_TRY(upload(h, A, halo_begin.data(), halo_begin.size(), (int **)&d.halo_begin));
   HIP_TRY(upload(h, A, halo_site.data(), halo_site.size(), (int **)&d.halo_site));
+  {
+    std::vector<int> tpos(N, -1);
+    for (int t = 0; t < cb.back(); ++t) tpos[class_lin[t]] = t;
+    HIP_TRY(upload(h, A, tpos.data(), tpos.size(), (int **)&d.site_tpos));
+  }
   HIP_TRY(upload(h, A, halo.data(), halo.size(), (int2 **)&d.halo));
   HIP_TRY(upload(h, A, entry_off.data(), entry_off.size(), (int **)&d.entry_off));
   HIP_TRY(upload(h, A, pair_begin.data(), pair_begin.size(), (int **)&d.pair_begin));

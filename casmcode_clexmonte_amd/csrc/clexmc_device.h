@@ -132,6 +132,7 @@ struct DevSupercell {
   const int *halo_begin;   // [n_classes+1]
   const int2 *halo;        // (src, dst) padded addresses
   const int *halo_site;    // [n_class_sites+1] first halo entry of each class site
+  const int *site_tpos;    // [n_sites] class-list position of each site (-1: fixed site)
   const int *entry_off;    // [n_entry_sites] padded relative offset (general path)
   const BinSub *bin_sub;   // [n_sublat]
   const int *bin_off;      // binary site programs
@@ -150,6 +151,7 @@ struct DevSupercell {
   int nb_max;              // largest list
   const int *nb_begin;     // [n_sublat+1]
   const int4 *nb_cell;     // (sublattice, di, dj, dk) cell offsets
+  int nb_distinct;         // 1: no two offsets of a list reach the same site of this supercell
   // slabs: largest |dk| between two sites that share a cluster, and per class
   // the index (relative to class_begin) of its first site with cell k >= K,
   // K = 0..L3 (class sites are in linear-index order, so k is sorted)

@@ -280,24 +280,10 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
       }
       cnt[o] -= 1;
       cnt[nw] += 1;
-      // every periodic copy of the site
-      const int r = pa - b * S;
-      const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
-      const int P2 = P12 / P1, P3 = S / P12;
-      const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
-      for (int dk = -m3; dk <= m3; ++dk) {
-        const int qk = pk + dk * L3;
-        if (qk < 0 || qk >= P3) continue;
-        for (int dj = -m2; dj <= m2; ++dj) {
-          const int qj = pj + dj * L2;
-          if (qj < 0 || qj >= P2) continue;
-          for (int di = -m1; di <= m1; ++di) {
-            const int qi = pi + di * L1;
-            if (qi < 0 || qi >= P1) continue;
-            img.p[b * S + qk * P12 + qj * P1 + qi] = (uint8_t)nw;
-          }
-        }
-      }
+      // the site and every periodic copy of it (the halo list of its class position)
+      img.p[pa] = (uint8_t)nw;
+      const int tp = sc->site_tpos[site];
+      for (int q = sc->halo_site[tp]; q < sc->halo_site[tp + 1]; ++q) img.p[sc->halo[q].y] = (uint8_t)nw;
       ++n_done;
     }
     // ---- rates of the site and its interaction neighbours
@@ -306,20 +292,32 @@ __global__ __launch_bounds__(256) void nfold_kernel(NfoldArgs na) {
       const int c = site - b * n_cells;
       const int i = c % L1, j = (c / L1) % L2, k = c / (L1 * L2);
       const int nb0 = sc->nb_begin[b], nb1 = sc->nb_begin[b + 1];
+      const bool distinct = sc->nb_distinct != 0;  // no duplicate sites: no list in LDS, no search
+      if (!distinct)
+        for (int t = tid; t < nb1 - nb0; t += nthr) {
+          const int4 o = sc->nb_cell[nb0 + t];
+          const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
+          aff[t] = o.x * n_cells + it + L1 * (jt + L2 * kt);
+          affp[t] = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
+        }
+      __syncthreads();  // the flip (thread 0) and the list before any rate is read
       for (int t = tid; t < nb1 - nb0; t += nthr) {
-        const int4 o = sc->nb_cell[nb0 + t];
-        const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
-        aff[t] = o.x * n_cells + it + L1 * (jt + L2 * kt);
-        affp[t] = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
-      }
-      __syncthreads();
-      for (int t = tid; t < nb1 - nb0; t += nthr) {
-        const int lt = aff[t];
+        int lt, pt;
+        if (distinct) {
+          const int4 o = sc->nb_cell[nb0 + t];
+          const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
+          lt = o.x * n_cells + it + L1 * (jt + L2 * kt);
+          pt = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
+        } else {
+          lt = aff[t];
+          pt = affp[t];
+        }
         bool dup = false;
-        for (int q = 0; q < t; ++q) dup = dup || (aff[q] == lt);
+        if (!distinct)
+          for (int q = 0; q < t; ++q) dup = dup || (aff[q] == lt);
         if (!dup) {
           int K[kMaxSlots];
-          const float nr = site_rate_at(sc->nb_cell[nb0 + t].x, affp[t], K);
+          const float nr = site_rate_at(sc->nb_cell[nb0 + t].x, pt, K);
           const float d = nr - rate[lt];
           rate[lt] = nr;
           atomicAdd(&G[lt >> 6], d);
