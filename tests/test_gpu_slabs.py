@@ -7,6 +7,10 @@
   4e-6 eV x sqrt(accepted moves); binary path: 1e-9 relative, exact counts),
   the last recorded sample equals the final state, canonical runs keep their
   composition, sample counts follow the sample period across launches.
+* Through the C++ calculator plugin (GpuCanonicalCalculator via the pybind11
+  face): one HCP ternary 24x24x16 chain runs as automatic slabs; the
+  composition is kept and the last sampled formation energy equals the
+  oracle's energy of the final configuration (FP32 accumulation bound).
 * Statistics: slab runs against whole-chain runs of the same system (FCC
   binary 16^3 semi-grand canonical; HCP ternary 12x12x8 canonical, both
   sides from the same pre-equilibrated states): mean energy and composition
@@ -149,3 +153,34 @@ def test_slabs_statistics_match_whole_chains():
         assert eng.last_run_slabs() == slabs
         eng.close()
     _agree("hcp canonical E/cell", res[3][0], res[1][0])
+
+
+def test_slabs_through_cpp_calculator(tmp_path):
+    _print_case("slabs through the C++ calculator hcp 24x24x16")
+    from casmcode_clexmonte_amd import plugin as P
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    system = P.system_from_flat(flat, str(tmp_path / "system.flat"))
+    calc = P.MonteCalculator("canonical", system)
+    dims = (24, 24, 16)
+    sc = Supercell(ce.prim, dims)
+    occ = random_occupation_counts(composition_counts(sc.n_sites, (0.4, 0.3, 0.3)), 5)
+    state = P.make_state(dims, occ, {"temperature": 1200.0})
+    loc = P.OccLocation()
+    calc.set_state_and_potential(state, loc)
+    rm = P.RunManager()
+    rm.seed = 3
+    params = P.SamplingFixtureParams()
+    params.n_equilibration_passes = 0
+    params.n_passes = 12
+    rm.params = params
+    calc.run(state, loc, rm)
+    res = rm.results
+    final = np.array(state.configuration.occupation, dtype=np.int8)
+    assert np.array_equal(np.bincount(final, minlength=3), np.bincount(occ, minlength=3))
+    assert res.n_accept > 0 and len(res.samplers["clex.formation_energy"]) == 12
+    Eref = O.OracleSystem(flat, dims).energy(final)
+    e_last = res.samplers["clex.formation_energy"][-1][0] * sc.n_cells
+    tol = 4e-6 * np.sqrt(res.n_accept) + 1e-9 * abs(Eref)
+    print(f"last sample {e_last:.9f} oracle {Eref:.9f} accepted {res.n_accept}")
+    assert abs(e_last - Eref) <= tol, (e_last, Eref, tol)
