@@ -275,15 +275,19 @@ def workload_nfold(args):
     n_acc = sum(b["n_accept"] - a["n_accept"] for a, b in zip(a0, a1))
     n_att = sum(b["n_attempt"] - a["n_attempt"] for a, b in zip(a0, a1))
     n_events = args.nfold_events
+    if args.nfold_generic:
+        eng.set_specialization(False)
     eng.nfold_run(100, 100)                                    # warm up
     t0 = time.perf_counter()
     eng.nfold_run(n_events, n_events // 10)
     wall = time.perf_counter() - t0
+    specialized = eng.last_run_info()["specialized"]
     eng.close()
     ev_rate = n * n_events / wall
     plain_acc_rate = n_acc / plain_wall
     return {"workload": "fcc_binary_24x24x24_sgc_256chains_T200-400_nfold", "events": n * n_events,
-            "events_per_s": ev_rate, "wall_s": wall, "plain_site_attempts_per_s": n_att / plain_wall,
+            "events_per_s": ev_rate, "wall_s": wall, "specialized": specialized,
+            "plain_site_attempts_per_s": n_att / plain_wall,
             "plain_acceptance": n_acc / n_att, "plain_accepted_moves_per_s": plain_acc_rate,
             "nfold_over_plain_accepted_moves": ev_rate / plain_acc_rate}
 
@@ -301,6 +305,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the single-chain, HCP and BCC workloads")
     ap.add_argument("--extra-passes", type=int, default=200)
     ap.add_argument("--nfold-events", type=int, default=20000)
+    ap.add_argument("--nfold-generic", action="store_true",
+                    help="run the N-fold workload on the generic (interpreted) kernel")
     ap.add_argument("--c4-size", type=int, default=32)
     ap.add_argument("--c4-chains", type=int, default=128)
     ap.add_argument("--c4-equil", type=int, default=1000)

@@ -40,6 +40,7 @@ struct HostSupercell {
   size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
   std::vector<hipModule_t> jit_mods;
   hipFunction_t jit_fn[2][2][2][2] = {};
+  hipFunction_t jit_nfold = nullptr;  // N-fold kernel of the binary SGC variant's module
 };
 
 }  // namespace
@@ -1169,6 +1170,8 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra) {
   return CLEXMC_OK;
 }
 
+static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes, int threads);
+
 clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
@@ -1203,8 +1206,32 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   double *d_blocks = nullptr;
   HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
   na.blocks = d_blocks;
+  // binary expansions on one supercell: the run-time specialised N-fold
+  // kernel (constant site-program offsets), else the generic one
+  hipFunction_t jf = nullptr;
+  if (h->specialize && h->binary) {
+    bool one_shape = true;
+    for (int c = 1; c < h->n_chains; ++c) one_shape = one_shape && h->chain_sc[c] == h->chain_sc[0];
+    if (one_shape) {
+      (void)jit_kernel(h, h->chain_sc[0], 1, false, 1, kAotMaxThreads);
+      jf = h->scs[h->chain_sc[0]].jit_nfold;
+    }
+  }
   hipError_t e = hipMemsetAsync(d_blocks, 0, nb * sizeof(double), h->stream);
-  if (e == hipSuccess) e = launch_nfold(na, h->binary != 0, h->n_chains, 256, lds, h->stream);
+  h->last_specialized = 0;
+  if (e == hipSuccess && jf) {
+    NfoldArgs args = na;
+    void *params[] = {(void *)&args};
+    const hipError_t le =
+        hipModuleLaunchKernel(jf, (unsigned)h->n_chains, 1, 1, 256, 1, 1, (unsigned)lds, h->stream, params, nullptr);
+    if (le == hipSuccess) {
+      h->last_specialized = 1;
+    } else {
+      (void)hipGetLastError();
+      jf = nullptr;
+    }
+  }
+  if (e == hipSuccess && !jf) e = launch_nfold(na, h->binary != 0, h->n_chains, 256, lds, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(blocks_out, d_blocks, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_blocks);
@@ -1294,6 +1321,8 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
       if (hipModuleGetFunction(&fn, mod, jit_kernel_name(kind, packed, lanes)) == hipSuccess) {
         hs.jit_mods.push_back(mod);
         state = 1;
+        if (h->binary && k == 1 && p == 0 && l == 0 && hipModuleGetFunction(&hs.jit_nfold, mod, kJitNfoldName) != hipSuccess)
+          hs.jit_nfold = nullptr;
       } else {
         h->jit_log += "\n[kernel lookup failed]";
         hipModuleUnload(mod);

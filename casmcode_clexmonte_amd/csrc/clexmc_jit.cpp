@@ -62,7 +62,10 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
                        const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
   std::ostringstream o;
-  o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\nstruct JitField {\n"
+  // the binary semi-grand canonical variant also carries the N-fold kernel
+  const bool with_nfold = binary && kind == 1 && !packed && lanes == 1;
+  o << "#include \"clexmc_metropolis.h\"\n" << (with_nfold ? "#include \"clexmc_nfold.h\"\n" : "")
+    << "namespace clexmc {\nstruct JitField {\n"
     << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static float bin(const FieldCtx &, const IMG &img, int b, int pa, int o,\n"
     << "                                              int (&K)[kMaxSlots]) {\n"
@@ -192,6 +195,9 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     << jit_kernel_name(kind, packed, lanes) << "(clexmc::RunArgs a) { clexmc::metropolis_body<"
     << (binary ? "true" : "false") << ", " << kind << ", " << (packed ? "true" : "false") << ", clexmc::JitField, "
     << lanes << ">(a); }\n";
+  if (with_nfold)
+    o << "extern \"C\" __global__ __launch_bounds__(256) void " << kJitNfoldName
+      << "(clexmc::NfoldArgs na) { clexmc::nfold_body<true, clexmc::JitField>(na); }\n";
   return o.str();
 }
 

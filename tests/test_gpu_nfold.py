@@ -8,6 +8,9 @@
   occupant counts a following Metropolis pass samples match the occupation.
 * Against the plain Metropolis kernel on FCC 12^3 at a low-acceptance point
   (T = 400 K, mu = -0.05 eV): mean energy and composition within 3 sigma.
+* The run-time specialised N-fold kernel (binary, one supercell shape)
+  against the generic one, same states and seeds: identical block sums and
+  final states, bit for bit.
 * General (table) expansions: ternary HCP 2x2x1 (2 sublattices, 3^8
   states) and quaternary BCC 2x2x2 (4^8 states), T = 1000 K: weighted
   averages of E, E^2, N_1, N_1^2 over 128 chains within 3 sigma of exact
@@ -173,3 +176,27 @@ def test_nfold_general_exact_enumeration(system):
         nb = np.bincount(eng.read_state(c)["occ"], minlength=4)
         assert np.array_equal(cnt[c, -1, :4], nb[:4])
     eng.close()
+
+
+def test_nfold_specialized_matches_generic():
+    _print_case("nfold specialised vs generic fcc 12^3")
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (12, 12, 12))
+    nch = 16
+    occs = [random_occupation(sc.n_sites, 0.5, 900 + c, exact=False) for c in range(nch)]
+    T = np.linspace(300.0, 900.0, nch)
+    chem = [[0.0, m] for m in np.linspace(-0.1, 0.1, nch)]
+    out = {}
+    for spec in (1, 0):
+        eng = Engine(flat)
+        eng.set_specialization(spec)
+        sid = eng.add_supercell(sc, n_pair_phases=0)
+        eng.load_chains([sid] * nch, occs, T, 4242 + np.arange(nch), chem=chem)
+        blocks = eng.nfold_run(6000, 500)
+        assert eng.last_run_info()["specialized"] == bool(spec)
+        out[spec] = (blocks, [eng.read_state(c) for c in range(nch)])
+        eng.close()
+    assert np.array_equal(out[1][0], out[0][0])
+    for a, b in zip(out[1][1], out[0][1]):
+        assert np.array_equal(a["occ"], b["occ"]) and a["E"] == b["E"]
