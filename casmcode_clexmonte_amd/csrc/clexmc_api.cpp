@@ -40,7 +40,7 @@ struct HostSupercell {
   size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
   std::vector<hipModule_t> jit_mods;
   hipFunction_t jit_fn[2][2][2][2] = {};
-  hipFunction_t jit_nfold = nullptr;  // N-fold kernel of the binary SGC variant's module
+  hipFunction_t jit_nfold = nullptr;  // N-fold kernel of the SGC (unpacked, one-lane) module
 };
 
 }  // namespace
@@ -1206,10 +1206,10 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   double *d_blocks = nullptr;
   HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
   na.blocks = d_blocks;
-  // binary expansions on one supercell: the run-time specialised N-fold
-  // kernel (constant site-program offsets), else the generic one
+  // every chain on one supercell: the run-time specialised N-fold kernel
+  // (constant site-program offsets), else the generic one
   hipFunction_t jf = nullptr;
-  if (h->specialize && h->binary) {
+  if (h->specialize) {
     bool one_shape = true;
     for (int c = 1; c < h->n_chains; ++c) one_shape = one_shape && h->chain_sc[c] == h->chain_sc[0];
     if (one_shape) {
@@ -1321,7 +1321,7 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
       if (hipModuleGetFunction(&fn, mod, jit_kernel_name(kind, packed, lanes)) == hipSuccess) {
         hs.jit_mods.push_back(mod);
         state = 1;
-        if (h->binary && k == 1 && p == 0 && l == 0 && hipModuleGetFunction(&hs.jit_nfold, mod, kJitNfoldName) != hipSuccess)
+        if (k == 1 && p == 0 && l == 0 && hipModuleGetFunction(&hs.jit_nfold, mod, kJitNfoldName) != hipSuccess)
           hs.jit_nfold = nullptr;
       } else {
         h->jit_log += "\n[kernel lookup failed]";

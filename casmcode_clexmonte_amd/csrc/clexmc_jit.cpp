@@ -62,8 +62,8 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
                        const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
   std::ostringstream o;
-  // the binary semi-grand canonical variant also carries the N-fold kernel
-  const bool with_nfold = binary && kind == 1 && !packed && lanes == 1;
+  // the semi-grand canonical, unpacked, one-lane variant also carries the N-fold kernel
+  const bool with_nfold = kind == 1 && !packed && lanes == 1;
   o << "#include \"clexmc_metropolis.h\"\n" << (with_nfold ? "#include \"clexmc_nfold.h\"\n" : "")
     << "namespace clexmc {\nstruct JitField {\n"
     << "  template <class IMG>\n"
@@ -197,7 +197,8 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
     << lanes << ">(a); }\n";
   if (with_nfold)
     o << "extern \"C\" __global__ __launch_bounds__(256) void " << kJitNfoldName
-      << "(clexmc::NfoldArgs na) { clexmc::nfold_body<true, clexmc::JitField>(na); }\n";
+      << "(clexmc::NfoldArgs na) { clexmc::nfold_body<" << (binary ? "true" : "false")
+      << ", clexmc::JitField>(na); }\n";
   return o.str();
 }
 

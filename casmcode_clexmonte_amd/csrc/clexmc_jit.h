@@ -27,8 +27,8 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
 // jit_source (two lanes per site: binary expansions only).
 const char *jit_kernel_name(int kind, bool packed, int lanes_per_site);
 
-// The N-fold kernel in the module of the binary, semi-grand canonical,
-// unpacked, one-lane variant.
+// The N-fold kernel in the module of the semi-grand canonical, unpacked,
+// one-lane variant.
 constexpr const char *kJitNfoldName = "clexmc_jit_nfold";
 
 // Compiles `src` (hipRTC, gfx950) and loads it as a module; code objects are

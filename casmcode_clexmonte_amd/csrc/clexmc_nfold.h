@@ -134,7 +134,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
     }
     float G[kMaxFunc];
-    general_field(img, pa, cst(sc->gen_sub + b), gen_prog, tab, G);
+    FIELD::gen(fctx, img, b, pa, tab, G);
     float r = 0.0f;
     for (int q = 0; q < no; ++q)
       if (q != o) r += change_rate(b, o, q, G);
@@ -253,7 +253,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       if (!BIN) {
         const int no = sys->n_occ[b];
         float G[kMaxFunc];
-        general_field(img, pa, cst(sc->gen_sub + b), gen_prog, tab, G);
+        FIELD::gen(fctx, img, b, pa, tab, G);
         float rq[kMaxOcc] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
         for (int q = 0; q < no; ++q)
           if (q != o) {

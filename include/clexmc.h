@@ -273,9 +273,9 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
  * occupant), so a site's rate is (1/(n_occ-1)) times the sum of its
  * changes' rates, and a chosen site takes a new occupant in proportion to
  * the change's rate.  Binary expansions keep exact cluster counts, others
- * an FP64 running energy of FP32 changes; binary expansions with every chain
- * on one supercell run the run-time specialised kernel when specialisation
- * is on (clexmc_set_specialization; identical results).  Each chain makes
+ * an FP64 running energy of FP32 changes; with every chain on one
+ * supercell the run-time specialised kernel runs when specialisation is on
+ * (clexmc_set_specialization; identical results).  Each chain makes
  * n_events events; every state visited is weighted by its expected residence
  * time w = n_mutable / R (Metropolis steps; R the total rate), so weighted
  * averages are Boltzmann averages.  blocks_out[n_chains][n_events /

@@ -8,9 +8,9 @@
   occupant counts a following Metropolis pass samples match the occupation.
 * Against the plain Metropolis kernel on FCC 12^3 at a low-acceptance point
   (T = 400 K, mu = -0.05 eV): mean energy and composition within 3 sigma.
-* The run-time specialised N-fold kernel (binary, one supercell shape)
-  against the generic one, same states and seeds: identical block sums and
-  final states, bit for bit.
+* The run-time specialised N-fold kernel (one supercell shape; binary FCC
+  12^3 and ternary HCP 4x4x3) against the generic one, same states and
+  seeds: identical block sums and final states, bit for bit.
 * General (table) expansions: ternary HCP 2x2x1 (2 sublattices, 3^8
   states) and quaternary BCC 2x2x2 (4^8 states), T = 1000 K: weighted
   averages of E, E^2, N_1, N_1^2 over 128 chains within 3 sigma of exact
@@ -178,15 +178,22 @@ def test_nfold_general_exact_enumeration(system):
     eng.close()
 
 
-def test_nfold_specialized_matches_generic():
-    _print_case("nfold specialised vs generic fcc 12^3")
-    ce = fcc_binary_ce()
-    flat = FlatCE(ce)
-    sc = Supercell(ce.prim, (12, 12, 12))
+@pytest.mark.parametrize("system", ["fcc_binary", "hcp_ternary"])
+def test_nfold_specialized_matches_generic(system):
+    _print_case(f"nfold specialised vs generic {system}")
     nch = 16
-    occs = [random_occupation(sc.n_sites, 0.5, 900 + c, exact=False) for c in range(nch)]
     T = np.linspace(300.0, 900.0, nch)
-    chem = [[0.0, m] for m in np.linspace(-0.1, 0.1, nch)]
+    if system == "fcc_binary":
+        flat = FlatCE(fcc_binary_ce())
+        sc = Supercell(flat.ce.prim, (12, 12, 12))
+        occs = [random_occupation(sc.n_sites, 0.5, 900 + c, exact=False) for c in range(nch)]
+        chem = [[0.0, m] for m in np.linspace(-0.1, 0.1, nch)]
+    else:
+        flat = _hcp_ternary_small()
+        sc = Supercell(flat.ce.prim, (4, 4, 3))
+        rng = np.random.default_rng(17)
+        occs = [rng.integers(0, 3, sc.n_sites).astype(np.int8) for _ in range(nch)]
+        chem = [[0.0, 0.02, -0.03] * flat.n_sublat] * nch
     out = {}
     for spec in (1, 0):
         eng = Engine(flat)
