@@ -25,6 +25,10 @@
 //    to the group sums with LDS float atomics;
 //  * every 128 events the group sums are summed again from the rates, so
 //    rounding from the atomic updates does not accumulate.
+//
+// nfold_body<BIN, FIELD>: FIELD is GenericField (interpreted site programs,
+// clexmc_nfold.hip) or the run-time specialised JitField (clexmc_jit.cpp);
+// both sum the clusters in the same order, so the results are identical.
 #pragma once
 #include "clexmc_metropolis.h"
 
