@@ -42,6 +42,11 @@ sys.path.insert(0, ROOT)
 # bandwidth, 256 B/clk): 256 CU x 256 B/clk x 2.4 GHz = 157.3 TB/s.  bench.py
 # also measures the device's own LDS throughput (16-byte reads and byte gathers).
 LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
+# VALU issue peak for unpacked 32-bit ops: 256 CUs x 64 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# VALU lane-ops per attempt of the specialised kernel on this workload
+# (SQ_INSTS_VALU x 64 / attempts, profiles/metropolis_specialized_fcc24_sgc256.md)
+VALU_OPS_PER_ATTEMPT = 267
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
@@ -392,7 +397,12 @@ def main():
                      "frac": achieved / LDS_PEAK_GBS, "traffic": None,
                      "bytes_per_attempt": bytes_per_attempt, "kernel_ms": kernel_ms,
                      "measured_lds_byte_gather_GBs": lds_u8, "frac_of_measured_byte_gather": achieved / lds_u8,
-                     "measured_lds_16B_read_GBs": lds_b128},
+                     "measured_lds_16B_read_GBs": lds_b128,
+                     # the roof that comes closest: VALU issue (profiled lane-ops per attempt x live rate)
+                     "valu": {"lane_ops_per_attempt": VALU_OPS_PER_ATTEMPT,
+                              "achieved_lane_ops_per_s": attempts * VALU_OPS_PER_ATTEMPT / (kernel_ms * 1e-3),
+                              "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                              "frac": attempts * VALU_OPS_PER_ATTEMPT / (kernel_ms * 1e-3) / VALU_PEAK_LANE_OPS}},
         "launch": launch,
         "ess_per_s": ess_total / wall,
         "tau_int_E_median_passes": float(np.median(taus)),
