@@ -378,7 +378,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   rng.k0 = cp->key0;
   rng.k1 = cp->key1;
 
-  // per-lane accumulators
+  // per-lane accumulators (cluster-count changes are +-2 times sums of at
+  // most a few hundred cluster products: 24-bit multiplies are exact)
   int accC[kMaxSlots];
 #pragma unroll
   for (int a = 0; a < kMaxSlots; ++a) accC[a] = 0;
@@ -497,7 +498,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
               accAtt += 1;
               const int sk = acc ? (o ? -2 : 2) : 0;
 #pragma unroll
-              for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+              for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sk, K[a]);
               accN[1] += sk / 2;
               accN[0] -= sk / 2;
               accAcc += acc ? 1 : 0;
@@ -518,7 +519,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             accAtt += 1;
             const int sk = acc ? (o ? -2 : 2) : 0;
 #pragma unroll
-            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sk, K[a]);
             accN[1] += sk / 2;
             accN[0] -= sk / 2;
             accAcc += acc ? 1 : 0;
@@ -612,7 +613,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
                 const int sp = op ? -2 : 2;
 #pragma unroll
                 for (int a = 0; a < kMaxSlots; ++a)
-                  accC[a] += sp * ((int)cK[a * max_class + p] - (int)cK[a * max_class + q]);
+                  accC[a] += __mul24(sp, (int)cK[a * max_class + p] - (int)cK[a * max_class + q]);
               } else {
                 accE += (double)dE;
               }
@@ -682,7 +683,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           if (BIN) {
             const int sp = op ? -2 : 2;
 #pragma unroll
-            for (int a = 0; a < kMaxSlots; ++a) accC[a] += sp * (Kp[a] - Kq[a]);
+            for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sp, Kp[a] - Kq[a]);
           } else {
             accE += (double)dE;
           }
