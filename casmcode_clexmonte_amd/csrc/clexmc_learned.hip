@@ -417,7 +417,7 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
           img[pa] = (uint8_t)o;
           const int sk = o0 ? -2 : 2;
 #pragma unroll
-          for (int a = 0; a < kMaxSlots; ++a) accC[a] += sk * K[a];
+          for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sk, K[a]);  // small operands: exact
           if (KIND == 1) accN1 += o0 ? -1 : 1;
         }
       }
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
           accAcc += 1;
           const int sp = op ? -2 : 2;
 #pragma unroll
-          for (int a = 0; a < kMaxSlots; ++a) accC[a] += sp * (Kp[a] - Kq[a]);
+          for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sp, Kp[a] - Kq[a]);
         }
       }
       ++draw;
