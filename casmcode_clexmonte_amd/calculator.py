@@ -24,7 +24,8 @@ from .completion import CompletionCheckParams, check as completion_check
 from .engine import Engine
 from .system import KB, ClusterExpansion, CompositionAxes, FlatCE, Supercell
 
-_METHODS = ("canonical", "semigrand_canonical")
+_METHODS = ("canonical", "semigrand_canonical", "nfold")
+_SGC = ("semigrand_canonical", "nfold")  # methods with semi-grand canonical conditions
 
 
 class System:
@@ -94,7 +95,7 @@ class MontePotential:
         c = self.calculator
         corr = c.engine.correlations(0)
         E = float(np.dot(c.system.flat.eci, corr))
-        if c.method == "semigrand_canonical":
+        if c.method in _SGC:
             x = c.system.composition_axes.param_composition(
                 c.system.composition_axes.mol_composition(c.engine.read_state(0)["occ"], self._n))
             E -= self._n * float(np.dot(np.asarray(self.state.conditions["param_chem_pot"], float), x))
@@ -106,7 +107,7 @@ class MontePotential:
     def occ_delta_per_supercell(self, linear_site_index: Sequence[int], new_occ: Sequence[int]) -> float:
         d = self.calculator.engine.delta_energy_batch(
             0, np.array([list(linear_site_index)]), np.array([list(new_occ)]),
-            include_chem=self.calculator.method == "semigrand_canonical")
+            include_chem=self.calculator.method in _SGC)
         return float(d[0])
 
 
@@ -129,7 +130,7 @@ class MonteCalculator:
         if "temperature" not in cond:
             raise ValueError("Error in MonteCalculator: missing required condition 'temperature'")
         axes = self.system.composition_axes
-        if self.method == "semigrand_canonical":
+        if self.method in _SGC:
             mu = cond.get("param_chem_pot")
             if mu is None or len(mu) != axes.n_independent:
                 raise ValueError("Error in MonteCalculator: 'param_chem_pot' missing or of the wrong size")
@@ -142,7 +143,7 @@ class MonteCalculator:
                                  "not consistent with conditions composition.")
 
     def _chem(self, state: MonteCarloState) -> np.ndarray:
-        if self.method == "semigrand_canonical":
+        if self.method in _SGC:
             return self.system.composition_axes.chem_weights(state.conditions["param_chem_pot"]).ravel()
         return np.zeros(self.system.flat.n_sublat * self.system.flat.n_occ_max)
 
@@ -174,6 +175,8 @@ class MonteCalculator:
     def run_multistate(self, states: Sequence[MonteCarloState], p: SamplingFixtureParams) -> List[dict]:
         """Run many states as independent chains of one persistent launch;
         final occupations are written back into the states."""
+        if self.method == "nfold":
+            return self._run_nfold(states, p)
         self._load(states, p.n_pair_phases)
         kind = self.method
         t0 = time.perf_counter()
@@ -226,6 +229,121 @@ class MonteCalculator:
         E, cnt = self.engine.read_samples()
         return E, cnt, ends
 
+    # -- N-fold way -----------------------------------------------------------------
+    def _run_nfold(self, states: Sequence[MonteCarloState], p: SamplingFixtureParams) -> List[dict]:
+        """N-fold way runs (the reference's nfold method,
+        include/casm/clexmonte/nfold/nfold_impl.hh:18-98, with its rates
+        src/casm/clexmonte/nfold/nfold_events.cc:28-62): equilibration by
+        p.n_equilibration_passes plain semi-grand canonical passes, then
+        p.n_passes passes of mol_size rejection-free events each.  Every visited
+        state is weighted by its expected residence time, so the statistics are
+        residence-weighted averages; one block of events per pass gives the
+        series the precisions are estimated from.  Binary expansions (the
+        events' record holds the count of one occupant); all states on one
+        supercell shape; fixed pass counts (no completion checks)."""
+        flat = self.system.flat
+        if not flat.binary:
+            raise ValueError("Error in MonteCalculator: the nfold method reports binary cluster expansions only")
+        if p.completion is not None:
+            raise ValueError("Error in MonteCalculator: nfold runs take a fixed number of passes")
+        shapes = {tuple(int(x) for x in s.supercell) for s in states}
+        if len(shapes) != 1:
+            raise ValueError("Error in MonteCalculator: nfold states must share one supercell")
+        self._load(states, 0)
+        t0 = time.perf_counter()
+        if p.n_equilibration_passes > 0:
+            self.engine.run_passes("semigrand_canonical", p.n_equilibration_passes, p.n_equilibration_passes)
+        n_cells = int(np.prod(next(iter(shapes))))
+        n_var = n_cells * sum(1 for b in range(flat.n_sublat) if flat.n_occ[b] >= 2)
+        att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
+        blocks = self.engine.nfold_run(p.n_passes * n_var, n_var)
+        elapsed = time.perf_counter() - t0
+        out = []
+        for c, s in enumerate(states):
+            st = self.engine.read_state(c)
+            s.occupation = st["occ"]
+            out.append(self._nfold_summary(s, blocks[c], n_var, st["n_attempt"] - att0[c], elapsed, p))
+        return out
+
+    def _nfold_summary(self, state, blk, n_var, n_events, elapsed, p) -> dict:
+        """Summary in the reference layout from residence-weighted block sums
+        (W, WE, WN, WE2, WN2, WEN per block).  Every sampled quantity q is
+        affine in (E, N1) per supercell, q = a + b E + c N1, so its weighted
+        mean and the covariances of the analysis functions follow exactly from
+        the weighted moments; precisions use the per-block means."""
+        axes = self.system.composition_axes
+        n = int(np.prod(state.supercell))
+        T = float(state.conditions["temperature"])
+        mu = np.asarray(state.conditions["param_chem_pot"], float)
+        comps = axes.components
+        var_names = [chr(ord("a") + i) for i in range(axes.n_independent)]
+        Wt = blk[:, 0].sum()
+        E1, N1 = blk[:, 1].sum() / Wt, blk[:, 2].sum() / Wt
+        E2, N2, EN = blk[:, 3].sum() / Wt, blk[:, 4].sum() / Wt, blk[:, 5].sum() / Wt
+        vEE, vNN, vEN = E2 - E1 * E1, N2 - N1 * N1, EN - E1 * N1
+        # composition as affine functions of N1 (occupant-1 count on the variable sites)
+        ser0 = self._series(state, np.zeros(2), np.array([[n_var, 0, 0, 0], [n_var - 1, 1, 0, 0]]))
+        aff = {}  # name -> (a, b, c): q = a + b E + c N1
+        for i, k in enumerate(comps):
+            m = ser0[f"mol_composition.{k}"]
+            aff[f"mol_composition.{k}"] = (m[0], 0.0, m[1] - m[0])
+        xa = np.array([ser0[f"param_composition.{k}"][0] for k in var_names])
+        xb = np.array([ser0[f"param_composition.{k}"][1] for k in var_names]) - xa
+        for j, k in enumerate(var_names):
+            aff[f"param_composition.{k}"] = (xa[j], 0.0, xb[j])
+        aff["clex.formation_energy"] = (0.0, 1.0 / n, 0.0)
+        aff["potential_energy"] = (-float(mu @ xa), 1.0 / n, -float(mu @ xb))
+
+        def mean(q):
+            a, b, c = aff[q]
+            return a + b * E1 + c * N1
+
+        def cov(q, r):
+            _, bq, cq = aff[q]
+            _, br, cr = aff[r]
+            return bq * br * vEE + (bq * cr + cq * br) * vEN + cq * cr * vNN
+
+        z = 2.0
+        Wb = blk[:, 0]
+        Eb, Nb = blk[:, 1] / Wb, blk[:, 2] / Wb
+
+        def stat(q):
+            a, b, c = aff[q]
+            _, err = A.mean_and_error(a + b * Eb + c * Nb)
+            return {"mean": [float(mean(q))], "calculated_precision": [z * err]}
+
+        stats = {"clex.formation_energy": {"shape": [], "value": stat("clex.formation_energy")},
+                 "potential_energy": {"shape": [], "value": stat("potential_energy")}}
+        for name, cn in (("mol_composition", comps), ("param_composition", var_names)):
+            d = {"component_names": list(cn), "shape": [len(cn)]}
+            for k in cn:
+                d[k] = stat(f"{name}.{k}")
+            stats[name] = d
+        kT = KB * T
+        analysis = {"heat_capacity": {"shape": [], "value": [cov("potential_energy", "potential_energy") * n / (kT * T)]}}
+
+        def cov_block(an, bn, pa, pb, name):
+            d = {"component_names": [f"{i},{j}" for j in bn for i in an], "shape": [len(an), len(bn)]}
+            for j in bn:
+                for i in an:
+                    d[f"{i},{j}"] = [cov(f"{pa}{i}", f"{pb}{j}") * n / kT]
+            analysis[name] = d
+        cov_block(comps, comps, "mol_composition.", "mol_composition.", "mol_susc")
+        cov_block(var_names, var_names, "param_composition.", "param_composition.", "param_susc")
+        aff["S"] = aff["potential_energy"]
+        cov_block(["S"], comps, "", "mol_composition.", "mol_thermochem_susc")
+        cov_block(["S"], var_names, "", "param_composition.", "param_thermochem_susc")
+        conditions = {"temperature": {"shape": [], "value": [T]},
+                      "param_chem_pot": {"component_names": var_names, "shape": [len(var_names)],
+                                         **{k: [float(v)] for k, v in zip(var_names, mu)}}}
+        nb = len(blk)
+        completion = {"N_samples": [nb], "N_samples_for_statistics": [nb],
+                      "N_samples_for_all_to_equilibrate": [0], "all_converged": [True],
+                      "all_equilibrated": [True], "acceptance_rate": [1.0],
+                      "count": [int(n_events)], "elapsed_clocktime": [elapsed]}
+        return {"conditions": conditions, "statistics": stats, "analysis": analysis,
+                "completion_check_results": completion}
+
     # -- results -----------------------------------------------------------------
     def _series(self, state, E_sc, counts) -> Dict[str, np.ndarray]:
         """Per-sample values of the samplers, per unit cell: clex.formation_energy,
@@ -243,7 +361,7 @@ class MonteCalculator:
             mol[:, c] += np.asarray(counts)[:, o] / n
         x = np.array([axes.param_composition(m) for m in mol]).reshape(len(E_sc), axes.n_independent)
         e = np.asarray(E_sc, dtype=float) / n
-        if self.method == "semigrand_canonical":
+        if self.method in _SGC:
             pe = e - x @ np.asarray(state.conditions["param_chem_pot"], float)
         else:
             pe = e.copy()
@@ -287,7 +405,7 @@ class MonteCalculator:
                 d[k] = stat(f"{name}.{k}", arr[:, i])
             stats[name] = d
         analysis = {"heat_capacity": {"shape": [], "value": [A.heat_capacity(pe, T, n)]}}
-        if self.method == "semigrand_canonical":
+        if self.method in _SGC:
             mu = np.asarray(state.conditions["param_chem_pot"], float)
 
             def cov_block(a, b, an, bn, name):

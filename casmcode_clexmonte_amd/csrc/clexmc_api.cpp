@@ -1202,7 +1202,7 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   na.n_events = n_events;
   na.events_per_block = events_per_block;
   na.n_blocks = n_events / events_per_block;
-  const size_t nb = (size_t)h->n_chains * na.n_blocks * 5;
+  const size_t nb = (size_t)h->n_chains * na.n_blocks * kNfoldBlockSums;
   double *d_blocks = nullptr;
   HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
   na.blocks = d_blocks;

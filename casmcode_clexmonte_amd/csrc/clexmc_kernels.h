@@ -100,13 +100,15 @@ struct AdamArgs {
   float lr, b1m, b2m, b1t, b2t;  // b1t = b1m^t, b2t = b2m^t
 };
 
-// N-fold way (rejection-free) semi-grand canonical runs of the binary path.
+// N-fold way (rejection-free) semi-grand canonical runs.
+constexpr int kNfoldBlockSums = 6;
 struct NfoldArgs {
   RunArgs a;               // system, supercells, chain params / state, occupations, exact counts
   int n_events;            // events per chain in this launch
   int events_per_block;    // events per output record
   int n_blocks;            // n_events / events_per_block
-  double *blocks;          // [n_chains][n_blocks][5]: W, sum w E, sum w N1, sum w E^2, sum w N1^2
+  double *blocks;          // [n_chains][n_blocks][kNfoldBlockSums]: W, sum w E, sum w N1,
+                           // sum w E^2, sum w N1^2, sum w E N1
 };
 
 #ifndef __HIPCC_RTC__

@@ -166,7 +166,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   long long cnt[kMaxOcc];
   for (int q = 0; q < kMaxOcc; ++q) cnt[q] = cs->counts[q];
   double E_run = cs->E;  // general: running energy
-  double W = 0.0, WE = 0.0, WN = 0.0, WE2 = 0.0, WN2 = 0.0;
+  double W = 0.0, WE = 0.0, WN = 0.0, WE2 = 0.0, WN2 = 0.0, WEN = 0.0;
   long long n_done = 0;
   double n_mutable = 0.0;
   for (int b = 0; b < sys->n_sublat; ++b)
@@ -230,6 +230,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
           WN += w * x;
           WE2 += w * E * E;
           WN2 += w * x * x;
+          WEN += w * E * x;
         }
       }
     }
@@ -335,14 +336,15 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     if (tid == 0 && ((ev + 1) % na.events_per_block) == 0) {
       const int blk = (ev + 1) / na.events_per_block - 1;
       if (blk < na.n_blocks) {
-        double *out = na.blocks + ((long long)chain * na.n_blocks + blk) * 5;
+        double *out = na.blocks + ((long long)chain * na.n_blocks + blk) * kNfoldBlockSums;
         out[0] = W;
         out[1] = WE;
         out[2] = WN;
         out[3] = WE2;
         out[4] = WN2;
+        out[5] = WEN;
       }
-      W = WE = WN = WE2 = WN2 = 0.0;
+      W = WE = WN = WE2 = WN2 = WEN = 0.0;
     }
   }
 

@@ -458,12 +458,14 @@ class Engine:
         self._check(lib().clexmc_set_image_packing(self.h, C.c_int(code)), "clexmc_set_image_packing")
 
     def nfold_run(self, n_events: int, events_per_block: int) -> np.ndarray:
-        """N-fold way semi-grand canonical events on every chain (binary CEs).
-        Returns [n_chains, n_events // events_per_block, 5]: per block of
-        events (sum w, sum w E, sum w N1, sum w E^2, sum w N1^2), w the
-        expected residence time of each visited state in Metropolis steps."""
+        """N-fold way semi-grand canonical events on every chain.
+        Returns [n_chains, n_events // events_per_block, 6]: per block of
+        events (sum w, sum w E, sum w N1, sum w E^2, sum w N1^2, sum w E N1),
+        w the expected residence time of each visited state in Metropolis
+        steps, E the formation energy per supercell, N1 the sites holding
+        occupant 1."""
         n_blocks = n_events // events_per_block
-        out = np.zeros((self.n_chains, n_blocks, 5))
+        out = np.zeros((self.n_chains, n_blocks, 6))
         self._check(lib().clexmc_nfold_run(self.h, C.c_int(n_events), C.c_int(events_per_block), _p(out, f64p)),
                     "clexmc_nfold_run")
         return out

@@ -279,9 +279,9 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
  * n_events events; every state visited is weighted by its expected residence
  * time w = n_mutable / R (Metropolis steps; R the total rate), so weighted
  * averages are Boltzmann averages.  blocks_out[n_chains][n_events /
- * events_per_block][5] receives, per block of events, (sum w, sum w E,
- * sum w N1, sum w E^2, sum w N1^2) with E the formation energy per supercell
- * and N1 the number of sites holding occupant 1.  Chain state (occupation,
+ * events_per_block][6] receives, per block of events, (sum w, sum w E,
+ * sum w N1, sum w E^2, sum w N1^2, sum w E N1) with E the formation energy
+ * per supercell and N1 the number of sites holding occupant 1.  Chain state (occupation,
  * exact counts, energy, counters) advances as in clexmc_run_passes. */
 clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out);
 

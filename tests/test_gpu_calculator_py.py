@@ -98,3 +98,73 @@ def test_calculator_rejects_bad_conditions(system):
     """Condition validation happens before any device call."""
     with pytest.raises(Exception):
         MonteCalculator("grand_canonical_typo", system)
+
+
+def test_nfold_summary_equals_replicated_samples(system):
+    """The N-fold summary is formed from residence-weighted moments (sums of
+    w, wE, wN1, wE^2, wN1^2, wEN1 per block).  With integer weights, the same
+    states repeated w times are ordinary samples: their plain summary (the
+    semi-grand canonical one) must give the same means, heat capacity and
+    susceptibilities.  No device work: the calculators are built without an
+    engine."""
+    rng = np.random.default_rng(5)
+    n = 512
+    E = rng.normal(-20.0, 3.0, 300)
+    N1 = rng.integers(150, 360, 300)
+    w = rng.integers(1, 6, 300)
+    cond = {"temperature": 700.0, "param_chem_pot": [0.07]}
+    st = MonteCarloState((8, 8, 8), np.zeros(n, np.int8), cond)
+    nf = object.__new__(MonteCalculator)
+    nf.method, nf.system = "nfold", system
+    sg = object.__new__(MonteCalculator)
+    sg.method, sg.system = "semigrand_canonical", system
+    blk = np.zeros((30, 6))
+    for b in range(30):
+        s = slice(10 * b, 10 * b + 10)
+        ww, ee, xx = w[s].astype(float), E[s], N1[s].astype(float)
+        blk[b] = [ww.sum(), (ww * ee).sum(), (ww * xx).sum(), (ww * ee * ee).sum(), (ww * xx * xx).sum(),
+                  (ww * ee * xx).sum()]
+    got = nf._nfold_summary(st, blk, n, 3000, 0.0, None)
+    Er, Nr = np.repeat(E, w), np.repeat(N1, w)
+    ref = sg._summary(st, Er, np.stack([n - Nr, Nr], axis=1), 1, 1, 0.0)
+    for q in ("clex.formation_energy", "potential_energy"):
+        assert got["statistics"][q]["value"]["mean"][0] == pytest.approx(ref["statistics"][q]["value"]["mean"][0],
+                                                                         rel=1e-12, abs=1e-12)
+    for k in ("A", "B"):
+        assert got["statistics"]["mol_composition"][k]["mean"][0] == pytest.approx(
+            ref["statistics"]["mol_composition"][k]["mean"][0], rel=1e-12)
+    assert got["statistics"]["param_composition"]["a"]["mean"][0] == pytest.approx(
+        ref["statistics"]["param_composition"]["a"]["mean"][0], rel=1e-12)
+    for key, comp in (("heat_capacity", None), ("mol_susc", "B,B"), ("param_susc", "a,a"),
+                      ("mol_thermochem_susc", "S,B"), ("param_thermochem_susc", "S,a")):
+        g = got["analysis"][key]["value" if comp is None else comp][0]
+        r = ref["analysis"][key]["value" if comp is None else comp][0]
+        assert g == pytest.approx(r, rel=1e-8), key
+    assert got["conditions"]["param_chem_pot"]["a"] == [0.07]
+
+
+@pytest.mark.gpu
+def test_python_nfold_matches_sgc(system):
+    """MonteCalculator("nfold") (residence-time-weighted rejection-free
+    events) and the semi-grand canonical calculator sample the same
+    ensemble: formation energy and composition means agree within 4 combined
+    standard errors at every chemical potential."""
+    print("[case] python calculator nfold vs sgc, fcc binary 8^3, T 900 K", flush=True)
+    mus = [-0.06, 0.0, 0.06]
+
+    def states():
+        return [MonteCarloState((8, 8, 8), random_occupation(512, 0.5, 40 + i, exact=False),
+                                {"temperature": 900.0, "param_chem_pot": [mu]}) for i, mu in enumerate(mus)]
+    sg = MonteCalculator("semigrand_canonical", system)
+    p = sg.make_default_sampling_fixture_params(n_equilibration_passes=200, n_passes=3000)
+    ref = sg.run_multistate(states(), p)
+    nf = MonteCalculator("nfold", system)
+    got = nf.run_multistate(states(), nf.make_default_sampling_fixture_params(n_equilibration_passes=200,
+                                                                               n_passes=1500))
+    validate_summary(merge_summaries(got), len(mus), False)
+    for r, g in zip(ref, got):
+        for sec, key in (("clex.formation_energy", "value"), ("param_composition", "a")):
+            a, b = r["statistics"][sec][key], g["statistics"][sec][key]
+            se = np.hypot(a["calculated_precision"][0], b["calculated_precision"][0]) / 2.0
+            assert abs(a["mean"][0] - b["mean"][0]) < 4 * se + 1e-6, (sec, a, b)
+        assert g["completion_check_results"]["count"][0] == 1500 * 512

@@ -1,7 +1,7 @@
 """The N-fold way (rejection-free) semi-grand canonical kernel.
 
 * Exact enumeration (FCC binary 2x2x4, 16 sites, T = 1200 K, mu = 0.03 eV):
-  residence-weighted averages of E, E^2, N_B, N_B^2 over 128 chains within
+  residence-weighted averages of E, E^2, N_B, N_B^2, E N_B over 128 chains within
   3 sigma of the Boltzmann averages (error: standard error over chains).
 * State bookkeeping: after the events, each chain's running energy equals
   the oracle's energy of its final occupation (exact counts, 1e-9), and the
@@ -13,7 +13,7 @@
   seeds: identical block sums and final states, bit for bit.
 * General (table) expansions: ternary HCP 2x2x1 (2 sublattices, 3^8
   states) and quaternary BCC 2x2x2 (4^8 states), T = 1000 K: weighted
-  averages of E, E^2, N_1, N_1^2 over 128 chains within 3 sigma of exact
+  averages of E, E^2, N_1, N_1^2, E N_1 over 128 chains within 3 sigma of exact
   enumeration; final energies equal the oracle's within the FP32
   accumulation bound (4e-6 eV x sqrt(events)); occupant counts follow the
   occupations.
@@ -46,7 +46,7 @@ def small_flat(seed=0):
 def _weighted(blocks, burn_blocks):
     b = blocks[:, burn_blocks:, :].sum(axis=1)
     W = b[:, 0]
-    return {"E": b[:, 1] / W, "E2": b[:, 3] / W, "n1": b[:, 2] / W, "n12": b[:, 4] / W}
+    return {"E": b[:, 1] / W, "E2": b[:, 3] / W, "n1": b[:, 2] / W, "n12": b[:, 4] / W, "En1": b[:, 5] / W}
 
 
 def test_nfold_exact_enumeration():
@@ -62,9 +62,9 @@ def test_nfold_exact_enumeration():
     occs = [random_occupation(sc.n_sites, 0.5, 100 + c, exact=False) for c in range(nch)]
     eng.load_chains([sid] * nch, occs, [T] * nch, 1000 + np.arange(nch), chem=[chem] * nch)
     blocks = eng.nfold_run(20000, 500)
-    assert blocks.shape == (nch, 40, 5) and np.all(blocks[:, :, 0] > 0)
+    assert blocks.shape == (nch, 40, 6) and np.all(blocks[:, :, 0] > 0)
     per = _weighted(blocks, 2)
-    for k in ("E", "E2", "n1", "n12"):
+    for k in ("E", "E2", "n1", "n12", "En1"):
         m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
         print(f"{k}: nfold {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
         assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
@@ -159,9 +159,9 @@ def test_nfold_general_exact_enumeration(system):
     eng.load_chains([sid] * nch, occs, [T] * nch, 2000 + np.arange(nch), chem=[chem] * nch)
     n_events = 20000
     blocks = eng.nfold_run(n_events, 500)
-    assert blocks.shape == (nch, 40, 5) and np.all(blocks[:, :, 0] > 0)
+    assert blocks.shape == (nch, 40, 6) and np.all(blocks[:, :, 0] > 0)
     per = _weighted(blocks, 2)
-    for k in ("E", "E2", "n1", "n12"):
+    for k in ("E", "E2", "n1", "n12", "En1"):
         m, se = per[k].mean(), per[k].std(ddof=1) / np.sqrt(nch)
         print(f"{k}: nfold {m:.6f} +- {se:.6f} exact {exact[k]:.6f}")
         assert abs(m - exact[k]) <= NSIG * se + 1e-12, (k, m, exact[k], se)
