@@ -15,7 +15,7 @@ namespace py = pybind11;
 using namespace CASM::clexmonte::gpu;
 
 PYBIND11_MODULE(_clexmc_plugin, m) {
-  m.doc() = "C++ GPU Monte Carlo calculators (GpuCanonicalCalculator, GpuSemiGrandCanonicalCalculator)";
+  m.doc() = "C++ GPU Monte Carlo calculators (GpuCanonicalCalculator, GpuSemiGrandCanonicalCalculator, GpuNfoldCalculator)";
   m.attr("KB") = KB;
 
   py::class_<System, std::shared_ptr<System>>(m, "System")
@@ -101,6 +101,7 @@ PYBIND11_MODULE(_clexmc_plugin, m) {
         []() { return std::shared_ptr<BaseMonteCalculator>(make_GpuCanonicalCalculator()); });
   m.def("make_semigrand_canonical_calculator",
         []() { return std::shared_ptr<BaseMonteCalculator>(make_GpuSemiGrandCanonicalCalculator()); });
+  m.def("make_nfold_calculator", []() { return std::shared_ptr<BaseMonteCalculator>(make_GpuNfoldCalculator()); });
 
   py::class_<MonteCalculator, std::shared_ptr<MonteCalculator>>(m, "MonteCalculator")
       .def_property_readonly("calculator_name", &MonteCalculator::calculator_name)

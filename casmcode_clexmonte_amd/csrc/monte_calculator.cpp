@@ -374,16 +374,17 @@ double required_scalar(State const &s, std::string const &k) {
 
 }  // namespace
 
-void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &states,
-                                      std::vector<std::vector<double>> const &chem, RunManager &run_manager,
-                                      std::vector<Results> &results) {
+void BaseMonteCalculator::_load_chains(int kind, std::vector<State *> const &states,
+                                       std::vector<std::vector<double>> const &chem, RunManager const &run_manager,
+                                       std::vector<double> &temps) {
   clexmc_handle *h = handle();
   SamplingFixtureParams const &P = run_manager.params;
   const int n = (int)states.size();
   if (n == 0) throw std::runtime_error("Error in " + calculator_name + "::run: no states");
   std::vector<int> ids(n);
   std::vector<int8_t> occ;
-  std::vector<double> temps(n), chem_all;
+  std::vector<double> chem_all;
+  temps.assign(n, 0.0);
   std::vector<uint64_t> seeds(n);
   for (int c = 0; c < n; ++c) {
     Configuration const &cfg = states[c]->configuration;
@@ -411,6 +412,16 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
   }
   check(h, clexmc_load_chains(h, n, ids.data(), occ.data(), temps.data(), chem_all.data(), seeds.data(), KB),
         calculator_name + "::run (load)");
+}
+
+void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &states,
+                                      std::vector<std::vector<double>> const &chem, RunManager &run_manager,
+                                      std::vector<Results> &results) {
+  clexmc_handle *h = handle();
+  SamplingFixtureParams const &P = run_manager.params;
+  const int n = (int)states.size();
+  std::vector<double> temps;
+  _load_chains(kind, states, chem, run_manager, temps);
   auto t0 = std::chrono::steady_clock::now();
   double kms = 0.0, ms = 0.0;
   if (P.n_equilibration_passes > 0) {
@@ -717,6 +728,172 @@ void GpuSemiGrandCanonicalCalculator::run(int current_state, std::vector<State> 
   for (size_t i = 0; i < states.size(); ++i) occ_locations[i].initialize(*system, states[i].configuration);
 }
 
+// ---------------------------------------------------------------------------
+// GpuNfoldCalculator
+// ---------------------------------------------------------------------------
+void BaseMonteCalculator::_run_nfold(std::vector<State *> const &states, std::vector<std::vector<double>> const &chem,
+                                     RunManager &run_manager, std::vector<Results> &results) {
+  const std::string where = calculator_name + "::run";
+  SamplingFixtureParams const &P = run_manager.params;
+  const auto &sys = *system;
+  const int n = (int)states.size();
+  if (n == 0) throw std::runtime_error("Error in " + where + ": no states");
+  int n_var_sub = 0, bvar = -1;
+  for (int b = 0; b < sys.n_sublat; ++b) {
+    if (sys.n_occ[b] > 2) throw std::runtime_error("Error in " + where + ": nfold runs take binary expansions only");
+    if (sys.n_occ[b] == 2) {
+      ++n_var_sub;
+      if (bvar < 0) bvar = b;
+    }
+  }
+  if (bvar < 0) throw std::runtime_error("Error in " + where + ": no variable sites");
+  for (auto *st : states)
+    if (!std::equal(st->configuration.L, st->configuration.L + 3, states[0]->configuration.L))
+      throw std::runtime_error("Error in " + where + ": nfold states must share one supercell");
+  const Index nu = states[0]->configuration.n_unitcells();
+  const long long n_var = (long long)nu * n_var_sub;
+  const long long n_events = (long long)P.n_passes * n_var;
+  if (P.n_passes < 1 || n_events > 2147483647LL)
+    throw std::runtime_error("Error in " + where + ": n_passes x mutable sites must lie in [1, 2^31)");
+  clexmc_handle *h = handle();
+  std::vector<double> temps;
+  _load_chains(1, states, chem, run_manager, temps);
+  auto t0 = std::chrono::steady_clock::now();
+  double kms = 0.0, ms = 0.0;
+  if (P.n_equilibration_passes > 0) {
+    check(h, clexmc_run_passes(h, 1, (int)P.n_equilibration_passes, (int)P.n_equilibration_passes, 0), where);
+    clexmc_last_run_ms(h, &ms);
+    kms += ms;
+  }
+  std::vector<long long> att0(n);
+  std::vector<int8_t> o(states[0]->configuration.occupation.size());
+  double Er;
+  long long na, nacc;
+  for (int c = 0; c < n; ++c) {
+    check(h, clexmc_read_state(h, c, o.data(), &Er, &na, &nacc), where + " (state)");
+    att0[c] = na;
+  }
+  const int nb = (int)P.n_passes;
+  std::vector<double> blocks((size_t)n * nb * 6);
+  check(h, clexmc_nfold_run(h, (int)n_events, (int)n_var, blocks.data()), where + " (nfold)");
+  double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // composition as an affine function of N1 (sites holding occupant 1)
+  std::vector<double> fixed(sys.n_components, 0.0);
+  for (int b = 0; b < sys.n_sublat; ++b)
+    if (sys.n_occ[b] < 2) fixed[sys.occ_component[b * sys.n_occ_max]] += 1.0;
+  auto mol_of = [&](double N1) {
+    std::vector<double> mol(fixed);
+    const double cnt[2] = {(double)n_var - N1, N1};
+    for (int q = 0; q < 2; ++q) {
+      int comp = sys.occ_component[bvar * sys.n_occ_max + q];
+      if (comp >= 0) mol[comp] += cnt[q] / (double)nu;
+    }
+    return mol;
+  };
+  const std::vector<double> m0 = mol_of(0.0), m1 = mol_of(1.0);
+  const std::vector<double> x0 = sys.param_composition(m0), x1 = sys.param_composition(m1);
+  results.assign(n, Results());
+  for (int c = 0; c < n; ++c) {
+    State &st = *states[c];
+    Results &R = results[c];
+    const double T = temps[c];
+    const auto mu = required_vector(st, "param_chem_pot");
+    // q = a + b E + g N1 for every sampled component (E per supercell)
+    struct Aff {
+      double a, b, g;
+    };
+    std::map<std::string, std::vector<Aff>> aff;
+    double pe_a = 0.0, pe_g = 0.0;
+    for (size_t i = 0; i < x0.size(); ++i) {
+      pe_a -= mu[i] * x0[i];
+      pe_g -= mu[i] * (x1[i] - x0[i]);
+    }
+    aff["clex.formation_energy"] = {{0.0, 1.0 / (double)nu, 0.0}};
+    aff["potential_energy"] = {{pe_a, 1.0 / (double)nu, pe_g}};
+    for (size_t i = 0; i < m0.size(); ++i) aff["mol_composition"].push_back({m0[i], 0.0, m1[i] - m0[i]});
+    for (size_t i = 0; i < x0.size(); ++i) aff["param_composition"].push_back({x0[i], 0.0, x1[i] - x0[i]});
+    const double *bk = blocks.data() + (size_t)c * nb * 6;
+    double S[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> Eb(nb), Nb(nb);
+    for (int k = 0; k < nb; ++k) {
+      for (int j = 0; j < 6; ++j) S[j] += bk[k * 6 + j];
+      Eb[k] = bk[k * 6 + 1] / bk[k * 6];
+      Nb[k] = bk[k * 6 + 2] / bk[k * 6];
+    }
+    const double E1 = S[1] / S[0], N1 = S[2] / S[0];
+    const double vEE = S[3] / S[0] - E1 * E1, vNN = S[4] / S[0] - N1 * N1, vEN = S[5] / S[0] - E1 * N1;
+    auto cov = [&](Aff const &p, Aff const &q) {
+      return p.b * q.b * vEE + (p.b * q.g + p.g * q.b) * vEN + p.g * q.g * vNN;
+    };
+    // one sample per pass: the pass's residence-weighted means
+    for (auto const &kv : aff) {
+      auto &ser = R.samplers[kv.first];
+      ser.assign(nb, std::vector<double>(kv.second.size()));
+      for (int k = 0; k < nb; ++k)
+        for (size_t i = 0; i < kv.second.size(); ++i)
+          ser[k][i] = kv.second[i].a + kv.second[i].b * Eb[k] + kv.second[i].g * Nb[k];
+      for (size_t i = 0; i < kv.second.size(); ++i) {
+        auto col = column(ser, i);
+        double v = cov_of(col, col);
+        double err = std::sqrt(std::max(0.0, v) * 2.0 * tau_int(col) / col.size());
+        R.statistics[kv.first + "(" + std::to_string(i) + ")"] = {
+            kv.second[i].a + kv.second[i].b * E1 + kv.second[i].g * N1, err};
+      }
+    }
+    R.samplers["temperature"].assign(nb, {T});
+    R.samplers["param_chem_pot"].assign(nb, mu);
+    // analysis functions (analysis_functions.cc:17-173) on the weighted moments
+    const Aff pe = aff["potential_energy"][0];
+    const auto &M = aff["mol_composition"];
+    const auto &X = aff["param_composition"];
+    const double kT = KB * T;
+    R.analysis["heat_capacity"] = {cov(pe, pe) * (double)nu / (kT * T)};
+    std::vector<double> mol_susc, param_susc, mol_tc, param_tc;
+    for (size_t j = 0; j < M.size(); ++j)
+      for (size_t i = 0; i < M.size(); ++i) mol_susc.push_back(cov(M[i], M[j]) * (double)nu / kT);
+    for (size_t j = 0; j < X.size(); ++j)
+      for (size_t i = 0; i < X.size(); ++i) param_susc.push_back(cov(X[i], X[j]) * (double)nu / kT);
+    for (size_t j = 0; j < M.size(); ++j) mol_tc.push_back(cov(pe, M[j]) * (double)nu / kT);
+    for (size_t j = 0; j < X.size(); ++j) param_tc.push_back(cov(pe, X[j]) * (double)nu / kT);
+    R.analysis["mol_susc"] = mol_susc;
+    R.analysis["param_susc"] = param_susc;
+    R.analysis["mol_thermochem_susc"] = mol_tc;
+    R.analysis["param_thermochem_susc"] = param_tc;
+    check(h, clexmc_read_state(h, c, o.data(), &Er, &na, &nacc), where + " (state)");
+    st.configuration.occupation.assign(o.begin(), o.end());
+    R.n_accept = na - att0[c];  // every event changes the state
+    R.n_reject = 0;
+    R.elapsed_s = elapsed;
+    R.kernel_ms = kms;
+  }
+}
+
+void GpuNfoldCalculator::run(State &state, OccLocation &occ_location, RunManager &run_manager) {
+  set_state_and_potential(state, &occ_location);
+  std::vector<State *> v = {&state};
+  std::vector<std::vector<double>> chem = {system->chem_weights(required_vector(state, "param_chem_pot"))};
+  std::vector<Results> res;
+  _run_nfold(v, chem, run_manager, res);
+  run_manager.results = res[0];
+  occ_location.initialize(*system, state.configuration);
+}
+
+void GpuNfoldCalculator::run(int current_state, std::vector<State> &states, std::vector<OccLocation> &occ_locations,
+                             RunManager &run_manager) {
+  std::vector<State *> v;
+  std::vector<std::vector<double>> chem;
+  for (auto &s : states) {
+    auto err = validate_state(s);
+    if (!err.empty()) throw std::runtime_error("Error in GpuNfoldCalculator::run: Invalid state: " + err[0]);
+    v.push_back(&s);
+    chem.push_back(system->chem_weights(required_vector(s, "param_chem_pot")));
+  }
+  _run_nfold(v, chem, run_manager, run_manager.multistate_results);
+  if (current_state >= 0 && current_state < (int)states.size()) run_manager.results = run_manager.multistate_results[current_state];
+  occ_locations.resize(states.size());
+  for (size_t i = 0; i < states.size(); ++i) occ_locations[i].initialize(*system, states[i].configuration);
+}
+
 }  // namespace gpu
 }  // namespace clexmonte
 }  // namespace CASM
@@ -729,5 +906,9 @@ CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuCanonicalCalculator() {
 
 CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuSemiGrandCanonicalCalculator() {
   return new CASM::clexmonte::gpu::GpuSemiGrandCanonicalCalculator();
+}
+
+CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuNfoldCalculator() {
+  return new CASM::clexmonte::gpu::GpuNfoldCalculator();
 }
 }

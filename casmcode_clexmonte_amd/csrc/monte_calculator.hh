@@ -203,6 +203,13 @@ class BaseMonteCalculator {
   /// run n chains through the C ABI; kind 0 canonical, 1 semi-grand
   void _run_chains(int kind, std::vector<State *> const &states, std::vector<std::vector<double>> const &chem,
                    RunManager &run_manager, std::vector<Results> &results);
+  /// load the states as chains of the handle (one per state); temperatures out
+  void _load_chains(int kind, std::vector<State *> const &states, std::vector<std::vector<double>> const &chem,
+                    RunManager const &run_manager, std::vector<double> &temps);
+  /// N-fold way (rejection-free) semi-grand canonical runs with
+  /// residence-time-weighted statistics; binary expansions, one supercell shape
+  void _run_nfold(std::vector<State *> const &states, std::vector<std::vector<double>> const &chem,
+                  RunManager &run_manager, std::vector<Results> &results);
 
   clexmc_handle *m_handle = nullptr;
   std::map<std::array<int, 3>, int> m_sc_ids;
@@ -234,6 +241,7 @@ class GpuCanonicalCalculator : public BaseMonteCalculator {
 class GpuSemiGrandCanonicalCalculator : public BaseMonteCalculator {
  public:
   GpuSemiGrandCanonicalCalculator() : BaseMonteCalculator("GpuSemiGrandCanonicalCalculator", false) {}
+  explicit GpuSemiGrandCanonicalCalculator(std::string name) : BaseMonteCalculator(std::move(name), false) {}
   std::vector<std::string> standard_sampling_function_names() const override;
   std::vector<std::string> standard_analysis_function_names() const override;
   SamplingFixtureParams make_default_sampling_fixture_params() const override;
@@ -246,6 +254,23 @@ class GpuSemiGrandCanonicalCalculator : public BaseMonteCalculator {
 
  protected:
   BaseMonteCalculator *_clone() const override { return new GpuSemiGrandCanonicalCalculator(*this); }
+};
+
+/// N-fold way (rejection-free) semi-grand canonical runs on the GPU (the
+/// reference's nfold method, include/casm/clexmonte/nfold/nfold_impl.hh:18-98):
+/// the semi-grand canonical conditions, potential and samplers; equilibration
+/// by Metropolis passes, then n_passes x mol_size events, one sample per pass
+/// (the pass's residence-weighted means).  Statistics and analysis are
+/// residence-time weighted.  Binary expansions; all states on one supercell.
+class GpuNfoldCalculator : public GpuSemiGrandCanonicalCalculator {
+ public:
+  GpuNfoldCalculator() : GpuSemiGrandCanonicalCalculator("GpuNfoldCalculator") {}
+  void run(State &state, OccLocation &occ_location, RunManager &run_manager) override;
+  void run(int current_state, std::vector<State> &states, std::vector<OccLocation> &occ_locations,
+           RunManager &run_manager) override;
+
+ protected:
+  BaseMonteCalculator *_clone() const override { return new GpuNfoldCalculator(*this); }
 };
 
 /// MonteCalculator wrapper (MonteCalculator.hh:44-403)
@@ -287,4 +312,5 @@ extern "C" {
 /// Factories with the reference's plugin convention (CanonicalCalculator.cc:580-586)
 CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuCanonicalCalculator();
 CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuSemiGrandCanonicalCalculator();
+CASM::clexmonte::gpu::BaseMonteCalculator *make_GpuNfoldCalculator();
 }

@@ -6,8 +6,9 @@ clones a calculator instance made by an ``extern "C"`` factory and runs it
 python/src/clexmonte_monte_calculator.cpp:34-46,140-173,216-269).
 
 Here the instance comes from ``make_canonical_calculator()`` /
-``make_semigrand_canonical_calculator()`` (the library's
-``make_GpuCanonicalCalculator`` / ``make_GpuSemiGrandCanonicalCalculator``),
+``make_semigrand_canonical_calculator()`` / ``make_nfold_calculator()`` (the
+library's ``make_GpuCanonicalCalculator`` / ``make_GpuSemiGrandCanonicalCalculator``
+/ ``make_GpuNfoldCalculator``),
 the system from the flat file written by ``system.save_flat``, and runs go
 through the C++ ``GpuCanonicalCalculator::run`` / multi-state ``run``.
 
@@ -38,14 +39,17 @@ RunManager = _p.RunManager
 BaseMonteCalculator = _p.BaseMonteCalculator
 make_canonical_calculator = _p.make_canonical_calculator
 make_semigrand_canonical_calculator = _p.make_semigrand_canonical_calculator
+make_nfold_calculator = _p.make_nfold_calculator
 
-_FACTORIES = {"canonical": make_canonical_calculator, "semigrand_canonical": make_semigrand_canonical_calculator}
+_FACTORIES = {"canonical": make_canonical_calculator, "semigrand_canonical": make_semigrand_canonical_calculator,
+              "nfold": make_nfold_calculator}
 
 
 def MonteCalculator(method: Union[str, "BaseMonteCalculator"], system: "System",
                     params: Optional[Dict[str, str]] = None):
     """A C++ MonteCalculator: `method` is a BaseMonteCalculator instance
-    (cloned, as the reference does) or "canonical" / "semigrand_canonical"."""
+    (cloned, as the reference does) or "canonical" / "semigrand_canonical" /
+    "nfold"."""
     if isinstance(method, str):
         if method not in _FACTORIES:
             raise ValueError(f"Error in MonteCalculator: unknown method '{method}'")

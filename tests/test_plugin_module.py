@@ -37,8 +37,11 @@ def test_factories_and_clone_on_cpu(flat, tmp_path):
     assert calc.calculator_name == calc2.calculator_name == "GpuCanonicalCalculator"
     sgc = P.MonteCalculator("semigrand_canonical", system)
     assert sgc.calculator_name == "GpuSemiGrandCanonicalCalculator"
+    nf = P.MonteCalculator("nfold", system)
+    assert nf.calculator_name == "GpuNfoldCalculator"
+    assert P.make_nfold_calculator().calculator_name == "GpuNfoldCalculator"
     with pytest.raises(ValueError):
-        P.MonteCalculator("nfold", system)
+        P.MonteCalculator("kinetic", system)
     st = P.make_state((4, 4, 4), np.zeros(64, np.int8), {"temperature": 600.0, "param_chem_pot": [0.1]})
     assert st.configuration.L == [4, 4, 4] and st.conditions.vector_values == {"param_chem_pot": [0.1]}
     assert system.mol_composition(st.configuration) == [1.0, 0.0]
@@ -100,6 +103,40 @@ def test_plugin_calculator_vs_oracle(flat, tmp_path, method):
     for s0, s1 in zip(states, out):
         if method == "canonical":
             assert sum(s1.configuration.occupation) == sum(s0.configuration.occupation)
+
+
+@pytest.mark.gpu
+def test_plugin_nfold_matches_sgc(flat, tmp_path):
+    """The C++ N-fold calculator (rejection-free events, residence-time
+    weighted statistics) against the C++ semi-grand canonical calculator:
+    formation energy and composition means within 4 combined standard errors
+    at three chemical potentials (FCC 8^3, T 900 K); one sample per pass."""
+    print("[case] plugin module nfold vs sgc", flush=True)
+    system = P.system_from_flat(flat, str(tmp_path / "system.flat"))
+    dims = (8, 8, 8)
+    mus = [-0.06, 0.0, 0.06]
+
+    def run(method, n_passes, seed):
+        calc = P.MonteCalculator(method, system)
+        states = [P.make_state(dims, random_occupation(512, 0.5, 60 + i, exact=False),
+                               {"temperature": 900.0, "param_chem_pot": [mu]}) for i, mu in enumerate(mus)]
+        rm = P.RunManager()
+        rm.seed = seed
+        params = P.SamplingFixtureParams()
+        params.n_equilibration_passes = 200
+        params.n_passes = n_passes
+        rm.params = params
+        calc.run_multistate(states, [P.OccLocation() for _ in states], rm)
+        return rm.multistate_results
+    ref = run("semigrand_canonical", 3000, 11)
+    got = run("nfold", 1500, 12)
+    for r, g in zip(ref, got):
+        assert len(g.samplers["potential_energy"]) == 1500 and g.n_reject == 0 and g.n_accept == 1500 * 512
+        for key in ("clex.formation_energy(0)", "param_composition(0)"):
+            (ma, ea), (mb, eb) = r.statistics[key], g.statistics[key]
+            assert abs(ma - mb) < 4 * np.hypot(ea, eb) + 1e-6, (key, ma, ea, mb, eb)
+        for key in ("heat_capacity", "param_susc"):
+            assert np.isfinite(g.analysis[key][0]) and g.analysis[key][0] > 0
 
 
 def test_no_device_fails_loudly(flat, tmp_path):
