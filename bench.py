@@ -319,9 +319,15 @@ def main():
     ap.add_argument("--c4-rounds", type=int, default=32)
     ap.add_argument("--c4-train-samples", type=int, default=512)
     ap.add_argument("--c4-train-steps", type=int, default=200)
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="GPU_MAX_HW_QUEUES for this process, at most 8 (set before the first HIP call; "
+                         "0 keeps the environment's value): the uneven-shape batch runs one stream per "
+                         "shape, up to 8, which share the hardware queues")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="copy N MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 8))
     if args.calibrate_copy:
         # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this run
         # calibrate the counters read on the Metropolis kernel
@@ -403,7 +409,7 @@ def main():
                               "achieved_lane_ops_per_s": attempts * VALU_OPS_PER_ATTEMPT / (kernel_ms * 1e-3),
                               "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
                               "frac": attempts * VALU_OPS_PER_ATTEMPT / (kernel_ms * 1e-3) / VALU_PEAK_LANE_OPS}},
-        "launch": launch,
+        "launch": dict(launch, gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")),
         "ess_per_s": ess_total / wall,
         "tau_int_E_median_passes": float(np.median(taus)),
         "acceptance": float(sum(eng.read_state(c)["n_accept"] for c in range(n_chains)) / max(1, att1)),
