@@ -139,6 +139,47 @@ def test_plugin_nfold_matches_sgc(flat, tmp_path):
             assert np.isfinite(g.analysis[key][0]) and g.analysis[key][0] > 0
 
 
+@pytest.mark.gpu
+def test_plugin_nfold_equals_python_nfold(flat, tmp_path):
+    """Same state, same chain seed (0) and no equilibration: the C++ and the
+    Python N-fold calculators launch the same deterministic events, so their
+    residence-weighted means and analysis values agree to rounding.  The
+    Python summary is checked against replicated samples on the CPU; this
+    ties the C++ one to it."""
+    print("[case] plugin nfold vs python nfold, same seed", flush=True)
+    from casmcode_clexmonte_amd.calculator import MonteCalculator as PyCalc, MonteCarloState, System
+    dims = (8, 8, 8)
+    occ = random_occupation(512, 0.5, 77, exact=False)
+    cond = {"temperature": 800.0, "param_chem_pot": [0.03]}
+    pc = PyCalc("nfold", System(flat.ce))
+    py = pc.run_multistate([MonteCarloState(dims, occ.copy(), dict(cond))],
+                           pc.make_default_sampling_fixture_params(n_equilibration_passes=0, n_passes=400))[0]
+    system = P.system_from_flat(flat, str(tmp_path / "system.flat"))
+    calc = P.MonteCalculator("nfold", system)
+    state = P.make_state(dims, occ.copy(), cond)
+    rm = P.RunManager()
+    rm.seed = 0
+    params = P.SamplingFixtureParams()
+    params.n_equilibration_passes = 0
+    params.n_passes = 400
+    rm.params = params
+    calc.run(state, P.OccLocation(), rm)
+    res = rm.results
+    st = py["statistics"]
+    pairs = [(res.statistics["clex.formation_energy(0)"][0], st["clex.formation_energy"]["value"]["mean"][0]),
+             (res.statistics["potential_energy(0)"][0], st["potential_energy"]["value"]["mean"][0]),
+             (res.statistics["param_composition(0)"][0], st["param_composition"]["a"]["mean"][0]),
+             (res.statistics["mol_composition(1)"][0], st["mol_composition"]["B"]["mean"][0]),
+             (res.analysis["heat_capacity"][0], py["analysis"]["heat_capacity"]["value"][0]),
+             (res.analysis["param_susc"][0], py["analysis"]["param_susc"]["a,a"][0]),
+             (res.analysis["mol_susc"][3], py["analysis"]["mol_susc"]["B,B"][0]),
+             (res.analysis["mol_thermochem_susc"][1], py["analysis"]["mol_thermochem_susc"]["S,B"][0]),
+             (res.analysis["param_thermochem_susc"][0], py["analysis"]["param_thermochem_susc"]["S,a"][0])]
+    for c, p in pairs:
+        assert c == pytest.approx(p, rel=1e-9, abs=1e-12), pairs
+    assert np.array_equal(np.array(state.configuration.occupation), pc.engine.read_state(0)["occ"])
+
+
 def test_no_device_fails_loudly(flat, tmp_path):
     """Without a GPU the C++ calculators raise (no CPU fallback); with one,
     this test has nothing to check."""
