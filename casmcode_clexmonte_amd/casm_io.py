@@ -309,7 +309,7 @@ def states_from_run_params(run_params, system, method: str, seed: int = 0):
             occ = enforce_composition(occ, axes, prim, n_cells, mol, rng)
             cond["mol_composition"] = axes.mol_composition(occ, n_cells)
             cond["param_composition"] = axes.param_composition(cond["mol_composition"])
-        elif method == "semigrand_canonical":
+        elif method in ("semigrand_canonical", "nfold"):  # the N-fold way takes semi-grand canonical conditions
             mu = _axis_vector(c0["param_chem_pot"], axis_names)
             if "param_chem_pot" in dc:
                 mu = mu + i * _axis_vector(dc["param_chem_pot"], axis_names)

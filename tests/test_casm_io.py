@@ -147,6 +147,9 @@ def test_run_params_states(zro_ce):
     sgc = states_from_run_params(ZRO["run_params_sgc_complete"], system, "semigrand_canonical")
     assert [s.conditions["temperature"] for s in sgc] == [1000.0] * 11
     np.testing.assert_allclose([s.conditions["param_chem_pot"][0] for s in sgc], -4.0 + 0.4 * np.arange(11))
+    nf = states_from_run_params(ZRO["run_params_sgc_complete"], system, "nfold")  # same conditions
+    assert [s.conditions["temperature"] for s in nf] == [1000.0] * 11
+    np.testing.assert_allclose([s.conditions["param_chem_pot"][0] for s in nf], -4.0 + 0.4 * np.arange(11))
 
 
 def test_planted_mistakes_are_refused():
