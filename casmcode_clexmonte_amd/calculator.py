@@ -74,8 +74,9 @@ class SamplingFixtureParams:
         if smp.get("sample_by", "pass") != "pass" or smp.get("spacing", "linear") != "linear":
             raise ValueError("only linear sampling by pass is supported")
         names = []
+        legacy = {"formation_energy": "clex.formation_energy", "formation_energy_corr": "corr.formation_energy"}
         for q in smp.get("quantities", []):
-            names.append("clex.formation_energy" if q == "formation_energy" else q)
+            names.append(legacy.get(q, q))
         return cls(label=label, sampler_names=names or cls().sampler_names,
                    sample_period=int(smp.get("period", 1)), n_equilibration_passes=0,
                    completion=CompletionCheckParams.from_casm(d.get("completion_check", {})), **kw)
@@ -162,6 +163,7 @@ class MonteCalculator:
         self.engine.load_chains(ids, [np.asarray(s.occupation, np.int8) for s in states],
                                 [float(s.conditions["temperature"]) for s in states], seeds,
                                 chem=np.stack([self._chem(s) for s in states]))
+        self._gpass = 0  # passes since the load: the kernels sample after passes g with (g + 1) % period == 0
 
     # -- running -----------------------------------------------------------------
     def make_default_sampling_fixture_params(self, label: str = "thermo", **kw) -> SamplingFixtureParams:
@@ -182,13 +184,15 @@ class MonteCalculator:
         t0 = time.perf_counter()
         if p.n_equilibration_passes > 0:
             self.engine.run_passes(kind, p.n_equilibration_passes, 1, p.canonical_rounds)
+            self._gpass += p.n_equilibration_passes
             self.engine.clear_samples()
             # sampling starts from energies recomputed in FP64 (table-path CEs accumulate FP32 dE)
             self.engine.reanchor_energies()
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
         acc0 = [self.engine.read_state(c)["n_accept"] for c in range(len(states))]
+        self._corr = [[] for _ in states] if "corr.formation_energy" in p.sampler_names else None
         if p.completion is None:
-            self.engine.run_passes(kind, p.n_passes, p.sample_period, p.canonical_rounds)
+            self._sampled_passes(kind, p.n_passes, p)
             self.engine.reanchor_energies()
             E, cnt = self.engine.read_samples()
             ends = [(E.shape[1], None, self.engine.read_state(c), time.perf_counter() - t0)
@@ -199,9 +203,32 @@ class MonteCalculator:
         for c, s in enumerate(states):
             n_s, res, st, elapsed = ends[c]
             s.occupation = st["occ"]
+            corr = np.asarray(self._corr[c][:n_s]) if self._corr is not None else None
             out.append(self._summary(s, E[c, :n_s], cnt[c, :n_s], st["n_attempt"] - att0[c],
-                                     st["n_accept"] - acc0[c], elapsed, res, p))
+                                     st["n_accept"] - acc0[c], elapsed, res, p, corr))
         return out
+
+    def _sampled_passes(self, kind: str, n_passes: int, p: SamplingFixtureParams):
+        """n_passes passes sampled every p.sample_period.  With the
+        "corr.formation_energy" sampler (sampling_functions.cc:121-135: all
+        correlations per unit cell), the launches end at the sampled passes
+        (g + 1 a multiple of the period, g counted from the load) and every
+        chain's FP64 correlations are read there."""
+        per = p.sample_period
+        if self._corr is None:
+            self.engine.run_passes(kind, n_passes, per, p.canonical_rounds)
+            self._gpass += n_passes
+            return
+        end = self._gpass + n_passes
+        while self._gpass < end:
+            due = self._gpass + (per - 1 - self._gpass % per)  # next sampled pass
+            stop = min(end, due + 1)
+            self.engine.run_passes(kind, stop - self._gpass, per, p.canonical_rounds)
+            self._gpass = stop
+            if stop == due + 1:
+                for c in range(len(self._corr)):
+                    n = int(np.prod(self.engine.supercells[self.engine.chain_sc[c]].dims))
+                    self._corr[c].append(self.engine.correlations(c) / n)
 
     def _run_to_completion(self, states, p: SamplingFixtureParams, t0: float):
         cc = p.completion
@@ -215,7 +242,7 @@ class MonteCalculator:
         hard_cap = int(cc.max_count) if cc.max_count is not None else 10 ** 8
         while any(e is None for e in ends):
             target = min(cc.next_check(count), hard_cap)
-            self.engine.run_passes(kind, target - count, p.sample_period, p.canonical_rounds)
+            self._sampled_passes(kind, target - count, p)
             count = target
             E, cnt = self.engine.read_samples()
             for c in range(n):
@@ -373,7 +400,7 @@ class MonteCalculator:
         return out
 
     def _summary(self, state, E_sc, counts, n_attempt, n_accept, elapsed, res=None,
-                 p: Optional[SamplingFixtureParams] = None) -> dict:
+                 p: Optional[SamplingFixtureParams] = None, corr: Optional[np.ndarray] = None) -> dict:
         axes = self.system.composition_axes
         n = int(np.prod(state.supercell))
         T = float(state.conditions["temperature"])
@@ -404,6 +431,13 @@ class MonteCalculator:
             for i, k in enumerate(cn):
                 d[k] = stat(f"{name}.{k}", arr[:, i])
             stats[name] = d
+        if corr is not None:
+            corr = corr[n_eq:]
+            names = [str(i) for i in range(corr.shape[1])]
+            d = {"component_names": names, "shape": [len(names)]}
+            for i, k in enumerate(names):
+                d[k] = stat(f"corr.formation_energy.{k}", corr[:, i])
+            stats["corr.formation_energy"] = d
         analysis = {"heat_capacity": {"shape": [], "value": [A.heat_capacity(pe, T, n)]}}
         if self.method in _SGC:
             mu = np.asarray(state.conditions["param_chem_pot"], float)

@@ -168,3 +168,39 @@ def test_python_nfold_matches_sgc(system):
             se = np.hypot(a["calculated_precision"][0], b["calculated_precision"][0]) / 2.0
             assert abs(a["mean"][0] - b["mean"][0]) < 4 * se + 1e-6, (sec, a, b)
         assert g["completion_check_results"]["count"][0] == 1500 * 512
+
+
+def test_fixture_legacy_quantity_names():
+    """CASM run params name the samplers formation_energy / formation_energy_corr."""
+    from casmcode_clexmonte_amd.calculator import SamplingFixtureParams
+    p = SamplingFixtureParams.from_casm("thermo", {"sampling": {"period": 2, "quantities": [
+        "formation_energy", "formation_energy_corr", "potential_energy"]}})
+    assert p.sampler_names == ["clex.formation_energy", "corr.formation_energy", "potential_energy"]
+    assert p.sample_period == 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["canonical", "semigrand_canonical"])
+def test_python_corr_sampler(system, method):
+    """The corr.formation_energy sampler: all correlations per unit cell at
+    every sample.  The energy is linear in them, so ECI . <corr> equals the
+    mean formation energy of the same samples (1e-9) only if the correlation
+    reads line up with the kernel's energy samples; the empty-cluster
+    correlation is 1."""
+    print(f"[case] python calculator corr sampler {method}", flush=True)
+    calc = MonteCalculator(method, system)
+    cond = {"temperature": 1000.0}
+    if method != "canonical":
+        cond["param_chem_pot"] = [0.02]
+    states = [MonteCarloState((6, 6, 6), random_occupation(216, 0.5, 70 + i), dict(cond)) for i in range(2)]
+    p = calc.make_default_sampling_fixture_params(n_equilibration_passes=20, n_passes=60, sample_period=3)
+    p.sampler_names = p.sampler_names + ["corr.formation_energy"]
+    out = calc.run_multistate(states, p)
+    eci = np.asarray(system.flat.eci)
+    for o in out:
+        d = o["statistics"]["corr.formation_energy"]
+        assert d["shape"] == [len(eci)] and o["completion_check_results"]["N_samples"] == [20]
+        mc = np.array([d[k]["mean"][0] for k in d["component_names"]])
+        assert mc[0] == pytest.approx(1.0, abs=1e-12)
+        e = o["statistics"]["clex.formation_energy"]["value"]["mean"][0]
+        assert float(eci @ mc) == pytest.approx(e, rel=1e-9, abs=1e-12)
