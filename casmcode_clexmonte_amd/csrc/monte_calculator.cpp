@@ -432,9 +432,38 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
     // sampling starts from energies recomputed in FP64 (table-path CEs accumulate FP32 dE)
     check(h, clexmc_reanchor_energies(h, nullptr), calculator_name + "::run");
   }
-  check(h, clexmc_run_passes(h, kind, (int)P.n_passes, P.sample_period, P.canonical_rounds), calculator_name + "::run");
-  clexmc_last_run_ms(h, &ms);
-  kms += ms;
+  // "corr.<key>" sampler (sampling_functions.cc:121-135): launches end at the
+  // sampled passes (g + 1 a multiple of the period, g counted from the load)
+  // and every chain's FP64 correlations per unit cell are read there
+  const bool want_corr = std::find(P.sampler_names.begin(), P.sampler_names.end(), "corr.formation_energy") !=
+                         P.sampler_names.end();
+  std::vector<std::vector<std::vector<double>>> corr(n);
+  if (!want_corr) {
+    check(h, clexmc_run_passes(h, kind, (int)P.n_passes, P.sample_period, P.canonical_rounds), calculator_name + "::run");
+    clexmc_last_run_ms(h, &ms);
+    kms += ms;
+  } else {
+    const long long per = std::max(1, P.sample_period);
+    long long g = std::max<Index>(P.n_equilibration_passes, 0);
+    const long long end = g + P.n_passes;
+    std::vector<double> cv(system->n_corr);
+    while (g < end) {
+      const long long due = g + (per - 1 - g % per);
+      const long long stop = std::min(end, due + 1);
+      check(h, clexmc_run_passes(h, kind, (int)(stop - g), (int)per, P.canonical_rounds), calculator_name + "::run");
+      clexmc_last_run_ms(h, &ms);
+      kms += ms;
+      g = stop;
+      if (stop != due + 1) continue;
+      for (int c = 0; c < n; ++c) {
+        check(h, clexmc_correlations(h, c, cv.data()), calculator_name + "::run (corr)");
+        const double nu = (double)states[c]->configuration.n_unitcells();
+        std::vector<double> row(cv.size());
+        for (size_t i = 0; i < cv.size(); ++i) row[i] = cv[i] / nu;
+        corr[c].push_back(std::move(row));
+      }
+    }
+  }
   check(h, clexmc_reanchor_energies(h, nullptr), calculator_name + "::run");
   double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   int ns = 0;
@@ -476,6 +505,10 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
       R.samplers["param_composition"].push_back(x);
       R.samplers["temperature"].push_back({T});
       if (kind == 1) R.samplers["param_chem_pot"].push_back(mu);
+    }
+    if (want_corr) {
+      if ((int)corr[c].size() < ns) throw std::runtime_error("Error in " + calculator_name + "::run: corr samples");
+      R.samplers["corr.formation_energy"].assign(corr[c].begin(), corr[c].begin() + ns);
     }
     // statistics (mean, autocorrelation-corrected standard error)
     for (auto const &kv : R.samplers) {

@@ -180,6 +180,39 @@ def test_plugin_nfold_equals_python_nfold(flat, tmp_path):
     assert np.array_equal(np.array(state.configuration.occupation), pc.engine.read_state(0)["occ"])
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["canonical", "semigrand_canonical"])
+def test_plugin_corr_sampler(flat, tmp_path, method):
+    """corr.formation_energy in the C++ calculators: one row of correlations
+    per unit cell at every sample; ECI . corr equals that sample's formation
+    energy (1e-9), which holds only when the reads line up with the kernel's
+    samples (period 3 after 20 equilibration passes: the first sampled pass
+    is the first after equilibration)."""
+    print(f"[case] plugin corr sampler {method}", flush=True)
+    system = P.system_from_flat(flat, str(tmp_path / "system.flat"))
+    calc = P.MonteCalculator(method, system)
+    cond = {"temperature": 1000.0}
+    if method != "canonical":
+        cond["param_chem_pot"] = [0.02]
+    state = P.make_state((6, 6, 6), random_occupation(216, 0.5, 5), cond)
+    rm = P.RunManager()
+    rm.seed = 4
+    params = P.SamplingFixtureParams()
+    params.sampler_names = ["clex.formation_energy", "potential_energy", "mol_composition", "param_composition",
+                            "corr.formation_energy"]
+    params.n_equilibration_passes = 20
+    params.n_passes = 60
+    params.sample_period = 3
+    rm.params = params
+    calc.run(state, P.OccLocation(), rm)
+    res = rm.results
+    corr = np.array(res.samplers["corr.formation_energy"])
+    e = np.array(res.samplers["clex.formation_energy"])[:, 0]
+    assert corr.shape == (20, flat.n_corr) and np.allclose(corr[:, 0], 1.0, atol=1e-12)
+    np.testing.assert_allclose(corr @ np.asarray(flat.eci), e, rtol=1e-9, atol=1e-12)
+    assert "corr.formation_energy(1)" in res.statistics
+
+
 def test_no_device_fails_loudly(flat, tmp_path):
     """Without a GPU the C++ calculators raise (no CPU fallback); with one,
     this test has nothing to check."""
