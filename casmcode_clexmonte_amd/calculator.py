@@ -271,6 +271,9 @@ class MonteCalculator:
         flat = self.system.flat
         if not flat.binary:
             raise ValueError("Error in MonteCalculator: the nfold method reports binary cluster expansions only")
+        if "corr.formation_energy" in p.sampler_names:
+            raise ValueError("Error in MonteCalculator: the nfold method samples energies and compositions only "
+                             "(no corr.formation_energy)")
         if p.completion is not None:
             raise ValueError("Error in MonteCalculator: nfold runs take a fixed number of passes")
         shapes = {tuple(int(x) for x in s.supercell) for s in states}

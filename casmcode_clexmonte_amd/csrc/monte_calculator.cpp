@@ -780,6 +780,8 @@ void BaseMonteCalculator::_run_nfold(std::vector<State *> const &states, std::ve
     }
   }
   if (bvar < 0) throw std::runtime_error("Error in " + where + ": no variable sites");
+  if (std::find(P.sampler_names.begin(), P.sampler_names.end(), "corr.formation_energy") != P.sampler_names.end())
+    throw std::runtime_error("Error in " + where + ": nfold runs sample energies and compositions only");
   for (auto *st : states)
     if (!std::equal(st->configuration.L, st->configuration.L + 3, states[0]->configuration.L))
       throw std::runtime_error("Error in " + where + ": nfold states must share one supercell");

@@ -204,3 +204,15 @@ def test_python_corr_sampler(system, method):
         assert mc[0] == pytest.approx(1.0, abs=1e-12)
         e = o["statistics"]["clex.formation_energy"]["value"]["mean"][0]
         assert float(eci @ mc) == pytest.approx(e, rel=1e-9, abs=1e-12)
+
+
+def test_nfold_refuses_corr_sampler(system):
+    """The N-fold method's block sums hold energies and compositions only: a
+    fixture asking for correlations is refused before any device work."""
+    nf = object.__new__(MonteCalculator)
+    nf.method, nf.system = "nfold", system
+    p = nf.make_default_sampling_fixture_params(n_passes=10)
+    p.sampler_names = p.sampler_names + ["corr.formation_energy"]
+    st = MonteCarloState((4, 4, 4), np.zeros(64, np.int8), {"temperature": 800.0, "param_chem_pot": [0.0]})
+    with pytest.raises(ValueError, match="corr"):
+        nf.run_multistate([st], p)

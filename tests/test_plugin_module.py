@@ -211,6 +211,10 @@ def test_plugin_corr_sampler(flat, tmp_path, method):
     assert corr.shape == (20, flat.n_corr) and np.allclose(corr[:, 0], 1.0, atol=1e-12)
     np.testing.assert_allclose(corr @ np.asarray(flat.eci), e, rtol=1e-9, atol=1e-12)
     assert "corr.formation_energy(1)" in res.statistics
+    if method == "semigrand_canonical":  # the N-fold method refuses the correlation sampler
+        nf = P.MonteCalculator("nfold", system)
+        with pytest.raises(RuntimeError, match="energies and compositions only"):
+            nf.run(P.make_state((6, 6, 6), random_occupation(216, 0.5, 5), cond), P.OccLocation(), rm)
 
 
 def test_no_device_fails_loudly(flat, tmp_path):
