@@ -84,6 +84,10 @@ struct clexmc_handle {
   int *d_sampN = nullptr;
   int *d_sample_count = nullptr;
   int sample_cap = 0;
+  // upper bound of the samples any chain holds since the last load or clear
+  // (kept on the host so that planning a run needs no device read)
+  long long samples_bound = 0;
+  int max_lds = 0, n_cu = 0;        // device attributes, read at creation
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
@@ -190,6 +194,8 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
   HIP_TRY(hipEventCreate(&h->ev1));
+  HIP_TRY(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  HIP_TRY(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device));
 
   ARG_CHECK(s->n_sublat >= 1 && s->n_sublat <= kMaxSublat, "n_sublat out of range (1..8)");
   ARG_CHECK(s->n_occ_max >= 1 && s->n_occ_max <= kMaxOcc, "n_occ_max out of range (1..4)");
@@ -1000,6 +1006,7 @@ clexmc_status clexmc_load_chains(clexmc_handle *h, int n_chains, const int *supe
   HIP_TRY(hipMalloc(&h->d_binC, sizeof(long long) * (size_t)n_chains * h->n_orbits));
   HIP_TRY(hipMalloc(&h->d_sample_count, sizeof(int) * n_chains));
   HIP_TRY(hipMemset(h->d_sample_count, 0, sizeof(int) * n_chains));
+  h->samples_bound = 0;
   HIP_TRY(hipMemcpy(h->d_params, params.data(), sizeof(ChainParams) * n_chains, hipMemcpyHostToDevice));
   h->n_chains = n_chains;
   // initial energies (and exact binary counts) from full correlations
@@ -1172,6 +1179,27 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra) {
 
 static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes, int threads);
 
+// Chains whose N-fold run stopped because their total rate vanished: the
+// flags are cleared and the run reported as failed (its blocks past the stop
+// are zero).
+static clexmc_status nfold_check_stalls(clexmc_handle *h) {
+  std::vector<ChainState> st(h->n_chains);
+  HIP_TRY(hipMemcpy(st.data(), h->d_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToHost));
+  int first = -1, n_bad = 0;
+  for (int c = 0; c < h->n_chains; ++c)
+    if (st[c].flags & kChainFlagNfoldStalled) {
+      if (first < 0) first = c;
+      ++n_bad;
+      st[c].flags &= ~kChainFlagNfoldStalled;
+    }
+  if (n_bad == 0) return CLEXMC_OK;
+  HIP_TRY(hipMemcpy(h->d_state, st.data(), sizeof(ChainState) * h->n_chains, hipMemcpyHostToDevice));
+  h->err = std::to_string(n_bad) + " chain(s), first " + std::to_string(first) +
+           ", stopped: the total N-fold event rate fell below 1e-30 (every change has "
+           "beta*dE above ~69; FP32 rates cannot resolve them); raise the temperature";
+  return CLEXMC_ERR_STATE;
+}
+
 clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
@@ -1236,7 +1264,7 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_blocks);
   HIP_TRY(e);
-  return CLEXMC_OK;
+  return nfold_check_stalls(h);
 }
 
 clexmc_status clexmc_set_specialization(clexmc_handle *h, int mode) {
@@ -1332,14 +1360,18 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
   return state == 1 ? fn : nullptr;
 }
 
-static clexmc_status grow_samples(clexmc_handle *h, int n_new_samples) {
-  std::vector<int> cnt(h->n_chains);
-  HIP_TRY(hipMemcpy(cnt.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
-  int used = 0;
-  for (int v : cnt) used = std::max(used, v);
-  const int need = used + n_new_samples + 2;
+// Sample storage for n_new_samples more samples per chain.  The count in use
+// is the host's upper bound (no device read); capacity at least doubles when
+// it grows and starts at kMinSampleCap samples per chain, so short runs
+// after a warm-up do not reallocate.
+static constexpr long long kMinSampleCap = 256;
+static clexmc_status grow_samples(clexmc_handle *h, long long n_new_samples) {
+  const long long need = h->samples_bound + n_new_samples + 2;
+  h->samples_bound += n_new_samples + 1;
+  ARG_CHECK(need < (1ll << 31), "sample storage beyond 2^31 samples per chain; clear the samples");
   if (need > h->sample_cap) {
-    const int cap = std::max(need, 2 * h->sample_cap);
+    const int cap = (int)std::min<long long>((1ll << 31) - 1,
+                                             std::max({need, 2ll * h->sample_cap, kMinSampleCap}));
     double *nE = nullptr;
     int *nN = nullptr;
     HIP_TRY(hipMalloc(&nE, sizeof(double) * (size_t)h->n_chains * cap));
@@ -1380,8 +1412,7 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   if (threads == 0) threads = std::min(kMaxJitThreads, std::max(64, ((max_class + 63) / 64) * 64));
   const int packing = packing_override ? packing_override : h->packing;
   const int n_tab = h->binary ? 0 : h->n_flower_tables;
-  int max_lds = 0;
-  HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  const int max_lds = h->max_lds;
   // int8 image when it fits (or is forced), else 2 bits per site (<= 4 occupants)
   bool packed = packing == 2;
   size_t lds = metropolis_lds_bytes(max_pad, n_tab, h->binary != 0, h->n_func_max, max_class, packed);
@@ -1403,7 +1434,7 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     h->err = "chain image does not fit in LDS (" + std::to_string(lds) + " > " + std::to_string(max_lds) + " bytes)";
     return CLEXMC_ERR_UNSUPPORTED;
   }
-  clexmc_status st = grow_samples(h, total_passes / sample_period);
+  clexmc_status st = grow_samples(h, (long long)total_passes / sample_period);
   if (st != CLEXMC_OK) return st;
   P.kind = kind;
   P.lanes = 1;
@@ -1532,8 +1563,7 @@ static constexpr int kMinSlabSites = 8192;
 static constexpr int kMaxSlabs = 16;
 
 static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &plans) {
-  int n_cu = 0;
-  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) n_cu = 1;
+  const int n_cu = std::max(1, h->n_cu);
   long long total = 0;
   for (int c = 0; c < h->n_chains; ++c) total += h->scs[h->chain_sc[c]].dev.n_sites;
   const long long per_wg = std::max<long long>(kMinSlabSites, total / std::max(1, n_cu));
@@ -1949,6 +1979,7 @@ clexmc_status clexmc_clear_samples(clexmc_handle *h) {
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
   HIP_TRY(hipSetDevice(h->device));
   HIP_TRY(hipMemset(h->d_sample_count, 0, sizeof(int) * h->n_chains));
+  h->samples_bound = 0;
   return CLEXMC_OK;
 }
 

@@ -204,7 +204,12 @@ struct ChainState {
   double sumE, sumE2, sumN1, sumN12, sumEN1;
   long long n_samples_acc;
   unsigned long long draw_hi;  // slab launches: largest Philox draw counter used
+  unsigned long long flags;    // kChainFlag* conditions raised by a kernel, cleared by the host
 };
+
+// N-fold way: the chain's total rate fell below the smallest the kernel
+// accepts (no event can be drawn reliably); the run stopped early.
+constexpr unsigned long long kChainFlagNfoldStalled = 1ull;
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
@@ -230,7 +235,26 @@ __host__ __device__ inline uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
   return c;
 }
 
-// 24 random bits -> float in [0,1)
+// 24 random bits -> float in [0,1) (site and occupant choices)
 __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// Acceptance with probability p from a 32-bit uniform word x: the test
+// u < p for u = (x + y / 2^32) / 2^32 with y a second uniform word.  With
+// t = p * 2^32, x < floor(t) accepts and x > floor(t) rejects; only
+// x == floor(t) (chance 2^-32) calls more() for y and accepts if
+// y < (t - floor(t)) * 2^32.  The acceptance probability is p (to FP32
+// rounding of p) plus at most 2^-64, for every p down to the smallest
+// normal float (1.2e-38, beta dE ~ 87); a single 24-bit uniform would accept
+// any change with probability at least 2^-24.  p >= 1 accepts; p == 0 or NaN
+// rejects.
+template <class MORE>
+__host__ __device__ inline bool accept_prob(float p, uint32_t x, MORE more) {
+  const float t = p * 4294967296.0f;
+  if (!(t > 0.0f)) return false;
+  if (t >= 4294967296.0f) return true;
+  const uint32_t ti = (uint32_t)t;
+  if (x != ti) return x < ti;
+  return (float)more() < (t - (float)ti) * 4294967296.0f;
+}
 
 }  // namespace clexmc

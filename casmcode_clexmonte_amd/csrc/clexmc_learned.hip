@@ -359,9 +359,12 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
                                           (uint32_t)chain), k0, k1);
             draw += 64ull;
           }
-          const float u1 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.x, slot));
-          const float u2 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.y, slot));
-          const float u3 = u01((uint32_t)__builtin_amdgcn_readlane((int)ul.z, slot));
+          const uint32_t x1 = (uint32_t)__builtin_amdgcn_readlane((int)ul.x, slot);
+          const uint32_t x2 = (uint32_t)__builtin_amdgcn_readlane((int)ul.y, slot);
+          const uint32_t x3 = (uint32_t)__builtin_amdgcn_readlane((int)ul.z, slot);
+          // the fourth word refines an acceptance test whose 32-bit word ties
+          auto more = [&]() { return (uint32_t)__builtin_amdgcn_readlane((int)ul.w, slot); };
+          const float u1 = u01(x1);
           if (KIND == 0) {
             const float vA = (o == 0) ? w : 0.0f;
             const float vB = (o == 1) ? w : 0.0f;
@@ -369,14 +372,14 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
             const float ZA = lane_f(sA, 63), ZB = lane_f(sB, 63);
             if (!(ZA > 0.0f) || !(ZB > 0.0f)) break;
             const int la_ = wave_pick(vA, sA, u1 * ZA);
-            const int lb_ = wave_pick(vB, sB, u2 * ZB);
+            const int lb_ = wave_pick(vB, sB, u01(x2) * ZB);
             if (la_ < 0 || lb_ < 0) break;
             const float wa = lane_f(w, la_), wna = lane_f(wn, la_), ea = lane_f(e, la_);
             const float wb = lane_f(w, lb_), wnb = lane_f(wn, lb_), eb = lane_f(e, lb_);
             const float dE = ea + eb;
             const float ZA2 = ZA - wa + wnb, ZB2 = ZB - wb + wna;
             const float ratio = __fdividef(wnb * wna * ZA * ZB, wa * wb * ZA2 * ZB2);
-            const bool acc = u3 < __expf(-beta * dE) * ratio;
+            const bool acc = accept_prob(__expf(-beta * dE) * ratio, x3, more);
             rAtt += 1;
             if (acc) {
               rAcc += 1;
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
             const float ws = lane_f(w, ls), wns = lane_f(wn, ls), es = lane_f(e, ls);
             const float Z2 = Z - ws + wns;
             const float ratio = __fdividef(wns * Z, ws * Z2);
-            const bool acc = u2 < __expf(-beta * es) * ratio;
+            const bool acc = accept_prob(__expf(-beta * es) * ratio, x2, more);
             rAtt += 1;
             if (acc) {
               rAcc += 1;
@@ -409,7 +412,6 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
                 wn = t;
               }
             }
-            (void)u3;
           }
         }
         // sites of one class share no cluster: apply the group's net changes
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
         const float Fq = binary_field_l(img, pq.y, oq, cst(sc->bin_sub + (bpq >> 8)), bin_off, n_slots, Kq);
         const float dp = op ? -2.0f : 2.0f;
         const float dE = dp * Fp - dp * Fq;
-        const bool acc = dE <= 0.0f || u01(ur.x) < __expf(-beta * dE);
+        const bool acc = dE <= 0.0f || accept_prob(__expf(-beta * dE), ur.x, [&]() { return ur.y; });
         accAtt += 1;
         if (acc) {
           img[pq.x] = (uint8_t)oq;
@@ -514,7 +516,7 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
                                                                   n_slots, Kt);
           const float dE = dE1 + dE2;
           accAtt += 1;
-          if (dE <= 0.0f || u01(ug.z) < __expf(-beta * dE)) {
+          if (dE <= 0.0f || accept_prob(__expf(-beta * dE), ug.z, [&]() { return ug.w; })) {
             set_all(ta, os);
             accAcc += 1;
             const int ss = os ? -2 : 2, st = ot ? -2 : 2;

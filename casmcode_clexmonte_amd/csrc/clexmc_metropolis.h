@@ -34,6 +34,17 @@ struct LaneRng {
     --left;
     return u01(x);
   }
+  // one 32-bit word of the stream (acceptance tests)
+  __device__ __forceinline__ uint32_t next_u32() {
+    if (left == 0) {
+      buf = philox4x32_10(make_uint4(lane, (uint32_t)draw, (uint32_t)(draw >> 32), chain), k0, k1);
+      ++draw;
+      left = 4;
+    }
+    uint32_t x = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
+    --left;
+    return x;
+  }
 };
 
 // block-uniform random integer in [0, n) for (pass, phase)
@@ -240,8 +251,8 @@ __device__ __forceinline__ float general_dE(cptr<DevSystem> sys, int b, int o, i
   return d;
 }
 
-__device__ __forceinline__ bool metropolis_accept(float dE, float beta, float u) {
-  return dE <= 0.0f || u < __expf(-beta * dE);
+__device__ __forceinline__ bool metropolis_accept(float dE, float beta, uint32_t x, LaneRng &rng) {
+  return dE <= 0.0f || accept_prob(__expf(-beta * dE), x, [&]() { return rng.next_u32(); });
 }
 
 // Binary field policy.  FieldCtx carries what the runtime site programs
@@ -491,10 +502,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             for (int a = 0; a < kMaxSlots; ++a) X[a] += pair_swap(X[a]);
             const float F = FIELD::bin_finish(fctx, b, X, K);
             if (part == 0) {
-              const float u = rng.next();
+              const uint32_t u = rng.next_u32();
               const float dphi = o ? -2.0f : 2.0f;
               const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
-              const bool acc = metropolis_accept(dE, beta, u);
+              const bool acc = metropolis_accept(dE, beta, u, rng);
               accAtt += 1;
               const int sk = acc ? (o ? -2 : 2) : 0;
 #pragma unroll
@@ -509,13 +520,13 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         for (int v = tid; v < nR; v += nthr) {
           const int pa = sc->class_addr[R.at(v)];
           const int o = img.get(pa);
-          const float u = rng.next();
+          const uint32_t u = rng.next_u32();
           if (BIN) {
             int K[kMaxSlots];
             const float F = FIELD::bin(fctx, img, b, pa, o, K);
             const float dphi = o ? -2.0f : 2.0f;
             const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
-            const bool acc = metropolis_accept(dE, beta, u);
+            const bool acc = metropolis_accept(dE, beta, u, rng);
             accAtt += 1;
             const int sk = acc ? (o ? -2 : 2) : 0;
 #pragma unroll
@@ -534,7 +545,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             const int nw = r >= o ? r + 1 : r;
             const float dEf = general_dE(sys, b, o, nw, G);
             const float dE = dEf - (cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o]);
-            const bool acc = metropolis_accept(dE, beta, u);
+            const bool acc = metropolis_accept(dE, beta, u, rng);
             accAtt += 1;
             if (acc) {
               img.set(pa, o, nw);
@@ -593,7 +604,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
             const int pa = sc->class_addr[R.at(p)], qa = sc->class_addr[R.at(q)];
             const int op = img.get(pa), oq = img.get(qa);
             if (op == oq) continue;
-            const float u = rng.next();
+            const uint32_t u = rng.next_u32();
             float dE;
             if (BIN) {
               const float dp = op ? -2.0f : 2.0f;
@@ -603,7 +614,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
               float Gq[kMaxFunc] = {cF[q], nfs > 1 ? cF[max_class + q] : 0.f, nfs > 2 ? cF[2 * max_class + q] : 0.f};
               dE = general_dE(sys, b, op, oq, Gp) + general_dE(sys, b, oq, op, Gq);
             }
-            const bool acc = metropolis_accept(dE, beta, u);
+            const bool acc = metropolis_accept(dE, beta, u, rng);
             accAtt += 1;
             if (acc) {
               img.set(pa, op, oq);
@@ -660,7 +671,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         const int bp = bpq & 0xFF, bq = bpq >> 8;
         const int op = img.get(pq.x), oq = img.get(pq.y);
         if (op == oq) continue;
-        const float u = rng.next();
+        const uint32_t u = rng.next_u32();
         float dE;
         int Kp[kMaxSlots], Kq[kMaxSlots];
         if (BIN) {
@@ -674,7 +685,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           FIELD::gen(fctx, img, bq, pq.y, tabL, Gq);
           dE = general_dE(sys, bp, op, oq, Gp) + general_dE(sys, bq, oq, op, Gq);
         }
-        const bool acc = metropolis_accept(dE, beta, u);
+        const bool acc = metropolis_accept(dE, beta, u, rng);
         accAtt += 1;
         if (acc) {
           img.set(pq.x, op, oq);
@@ -727,7 +738,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           }
         };
         for (int g = 0; g < args.global_swaps; ++g) {
-          const float u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
+          const float u1 = rng.next(), u2 = rng.next();
+          const uint32_t u3 = rng.next_u32();
           const int is = min((int)(u1 * (float)n_var), n_var - 1);
           const int it = min((int)(u2 * (float)n_var), n_var - 1);
           const int sa = sc->class_addr[is], ta = sc->class_addr[it];
@@ -755,7 +767,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
           }
           const float dE = dE1 + dE2;
           accAtt += 1;
-          if (metropolis_accept(dE, beta, u3)) {
+          if (metropolis_accept(dE, beta, u3, rng)) {
             set_all(ta, ot, os);
             accAcc += 1;
             if (BIN) {

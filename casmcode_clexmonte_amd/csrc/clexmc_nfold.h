@@ -10,21 +10,26 @@
 // Metropolis acceptance of its changes, (1/(n_occ-1)) sum_o' min(1,
 // exp(-beta dOmega)); binary expansions have one change per site.
 //
-// One workgroup per chain.  LDS holds the padded int8 image, the rate of
-// every site and its sums over groups of 64 sites.  Per event:
-//  * wavefront 0 forms the total rate R, draws u, finds the group by a scan
-//    over per-lane chunks of group sums, then the site by a scan of the
-//    group's 64 rates (all prefix sums are wavefront scans);
+// One workgroup per chain.  LDS holds the padded int8 image, the FP32 rate
+// of every site and FP64 sums of the rates over groups of 64 sites.  Per
+// event:
+//  * wavefront 0 forms the total rate R (FP64), draws a 53-bit uniform u,
+//    finds the group by a scan over per-lane chunks of group sums, then the
+//    site by a scan of the group's 64 rates (FP64 wavefront scans), so a rate
+//    is drawn in proportion to its value however small it is next to R;
 //  * the current state is weighted by its expected residence time
 //    w = n_mutable / R in Metropolis steps (the jump chain visits states in
 //    proportion to pi(x) R(x); weights 1/R recover pi);
 //  * one lane applies the change (the new occupant drawn in proportion to
 //    its rate; binary: exact integer cluster counts, general: FP64 energy;
-//    every periodic copy of the site), then the lanes recompute the rates of the site and
-//    of its interaction neighbours (each neighbour once) and add the changes
-//    to the group sums with LDS float atomics;
-//  * every 128 events the group sums are summed again from the rates, so
-//    rounding from the atomic updates does not accumulate.
+//    every periodic copy of the site), then the lanes recompute the rates of
+//    the site and of its interaction neighbours (each neighbour once), and
+//    the sum of every group holding one of them is formed again from its 64
+//    rates.  Group sums are never updated by differences, so they carry no
+//    cancellation error when rates of very different size come and go;
+//  * if R falls below 1e-30 (every rate of the chain within a few decades of
+//    FP32 underflow, beta dE > ~69 for all changes) the chain stops and
+//    records the condition in its state; the host reports it as an error.
 //
 // nfold_body<BIN, FIELD>: FIELD is GenericField (interpreted site programs,
 // clexmc_nfold.hip) or the run-time specialised JitField (clexmc_jit.cpp);
@@ -36,33 +41,48 @@ namespace clexmc {
 
 __host__ __device__ inline size_t nfold_align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-
-__device__ __forceinline__ float wave_scan_incl_f(float v, int lane) {
+__device__ __forceinline__ double wave_scan_incl_d(double v, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const float y = __shfl_up(v, o, 64);
+    const double y = __shfl_up(v, o, 64);
     if (lane >= o) v += y;
   }
   return v;
 }
 
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // first lane whose inclusive prefix exceeds t; if rounding leaves none, the
 // last lane with a positive value (-1 if there is none)
-__device__ __forceinline__ int pick_lane(float incl, float v, float t) {
-  const unsigned long long m = __ballot(incl > t && v > 0.0f);
+__device__ __forceinline__ int pick_lane(double incl, double v, double t) {
+  const unsigned long long m = __ballot(incl > t && v > 0.0);
   if (m) return __ffsll((long long)m) - 1;
-  const unsigned long long m2 = __ballot(v > 0.0f);
+  const unsigned long long m2 = __ballot(v > 0.0);
   return m2 ? 63 - __clzll((long long)m2) : -1;
 }
 
+// uniform double in [0, 1) from 53 random bits of two words
+__device__ __forceinline__ double u01_53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
 
+// sum of the 64 FP32 rates of a group in FP64, in a fixed order (four
+// interleaved partial sums of 16), so every lane that forms it gets the same
+// value
+__device__ __forceinline__ double group_rate_sum(const float *rate, int g) {
+  const float4 *r4 = (const float4 *)(rate + (size_t)g * 64);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const float4 v = r4[q];
+    a0 += (double)v.x;
+    a1 += (double)v.y;
+    a2 += (double)v.z;
+    a3 += (double)v.w;
+  }
+  return (a0 + a1) + (a2 + a3);
+}
 
+// smallest total rate a chain may reach before it stops with an error
+constexpr double kNfoldMinTotalRate = 1e-30;
 
 template <bool BIN, class FIELD>
 __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
@@ -88,10 +108,10 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   ImgU8 img;
   img.p = (uint8_t *)smem;
   size_t off = nfold_align16((size_t)image_bytes(pad_bytes, false));
-  float *rate = (float *)(smem + off);
-  off += nfold_align16((size_t)n_sites * 4);
-  float *G = (float *)(smem + off);
-  off += nfold_align16((size_t)n_groups * 4);
+  float *rate = (float *)(smem + off);  // [n_groups * 64], zero past n_sites
+  off += nfold_align16((size_t)n_groups * 64 * 4);
+  double *G = (double *)(smem + off);   // [n_groups]
+  off += nfold_align16((size_t)n_groups * 8);
   int *aff = (int *)(smem + off);    // linear indices of the sites whose rates change
   off += nfold_align16((size_t)sc->nb_max * 4);
   int *affp = (int *)(smem + off);   // their padded addresses
@@ -148,17 +168,9 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     int K[kMaxSlots];
     return site_rate_at(l / n_cells, main_addr(l), K);
   };
-  auto refresh_groups = [&]() {
-    for (int g = wave; g < n_groups; g += n_waves) {
-      const int l = g * 64 + lane;
-      const float s = wave_sum_f(l < n_sites ? rate[l] : 0.0f);
-      if (lane == 0) G[g] = s;
-    }
-  };
-
-  for (int l = tid; l < n_sites; l += nthr) rate[l] = site_rate(l);
+  for (int l = tid; l < n_groups * 64; l += nthr) rate[l] = l < n_sites ? site_rate(l) : 0.0f;
   __syncthreads();
-  refresh_groups();
+  for (int g = tid; g < n_groups; g += nthr) G[g] = group_rate_sum(rate, g);
   __syncthreads();
 
   // per-chain running quantities (lane 0 of the workgroup)
@@ -178,44 +190,44 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   }
   const unsigned long long ev0 = cs->pass_base;
 
-  bool retried = false;
   for (int ev = 0; ev < na.n_events; ++ev) {
     // ---- select an event (wavefront 0)
     if (wave == 0) {
       const int chunk = (n_groups + 63) / 64;
       const int g0 = min(lane * chunk, n_groups), g1 = min(g0 + chunk, n_groups);
-      float ls = 0.0f;
+      double ls = 0.0;
       for (int g = g0; g < g1; ++g) ls += G[g];
-      const float incl = wave_scan_incl_f(ls, lane);
-      const float R = __shfl(incl, 63, 64);
+      const double incl = wave_scan_incl_d(ls, lane);
+      const double R = __shfl(incl, 63, 64);
       const unsigned long long gev = ev0 + (unsigned long long)ev;
       const uint4 r = philox4x32_10(make_uint4(0x4E464C44u, (uint32_t)gev, (uint32_t)(gev >> 32), (uint32_t)chain),
                                     cp->key0, cp->key1);
-      const float t = u01(r.x) * R;
+      const double t = u01_53(r.x, r.y) * R;
       int site = -1;
-      const int L = R > 0.0f ? pick_lane(incl, ls, t) : -1;
+      const bool live = R >= kNfoldMinTotalRate;
+      const int L = live ? pick_lane(incl, ls, t) : -1;
       if (L >= 0) {
-        const float before = __shfl(incl, L, 64) - __shfl(ls, L, 64);
+        const double before = __shfl(incl, L, 64) - __shfl(ls, L, 64);
         const int gl0 = __shfl(g0, L, 64), gl1 = __shfl(g1, L, 64);
-        const float gv = (gl0 + lane < gl1) ? G[gl0 + lane] : 0.0f;
-        const float gi = wave_scan_incl_f(gv, lane);
-        const float t2 = t - before;
+        const double gv = (gl0 + lane < gl1) ? G[gl0 + lane] : 0.0;
+        const double gi = wave_scan_incl_d(gv, lane);
+        const double t2 = t - before;
         const int lg = pick_lane(gi, gv, t2);
         if (lg >= 0) {
           const int g = gl0 + lg;
-          const float before2 = __shfl(gi, lg, 64) - __shfl(gv, lg, 64);
-          const int l = g * 64 + lane;
-          const float rv = l < n_sites ? rate[l] : 0.0f;
-          const float ri = wave_scan_incl_f(rv, lane);
+          const double before2 = __shfl(gi, lg, 64) - __shfl(gv, lg, 64);
+          const double rv = (double)rate[g * 64 + lane];
+          const double ri = wave_scan_incl_d(rv, lane);
           const int ls2 = pick_lane(ri, rv, t2 - before2);
           if (ls2 >= 0) site = g * 64 + ls2;
         }
       }
       if (lane == 0) {
-        // a group sum left positive by rounding with no positive rate in the
-        // group: sum the groups again and draw once more
-        sel[0] = (site < 0 && R > 0.0f && !retried) ? -2 : site;
-        sel[1] = __float_as_int(u01(r.y));  // draws the new occupant (general sites)
+        // a chain whose total rate vanished (or a group sum without a
+        // positive rate, which exact group sums rule out) stops: -2
+        sel[0] = site >= 0 ? site : -2;
+        sel[1] = (int)r.z;  // the new occupant (general sites): 53-bit uniform
+        sel[2] = (int)r.w;
         if (site >= 0) {
           // weight of the current state: expected Metropolis steps spent in it
           double E = E0;
@@ -236,15 +248,11 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     }
     __syncthreads();
     const int site = sel[0];
-    if (site == -2) {
-      refresh_groups();
-      __syncthreads();
-      retried = true;
-      --ev;
-      continue;
+    if (site < 0) {
+      // uniform across the workgroup: record the stall for the host
+      if (tid == 0) cs->flags |= kChainFlagNfoldStalled;
+      break;
     }
-    retried = false;
-    if (site < 0) break;  // no event has a positive rate (uniform across the workgroup)
 
     // ---- apply the change (one lane); binary: its cluster sums K come with
     // its new rate below; general: the new occupant in proportion to its rate
@@ -259,14 +267,14 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         const int no = sys->n_occ[b];
         float G[kMaxFunc];
         FIELD::gen(fctx, img, b, pa, tab, G);
-        float rq[kMaxOcc] = {0.0f, 0.0f, 0.0f, 0.0f}, tot = 0.0f;
+        double rq[kMaxOcc] = {0.0, 0.0, 0.0, 0.0}, tot = 0.0;
         for (int q = 0; q < no; ++q)
           if (q != o) {
-            rq[q] = change_rate(b, o, q, G);
+            rq[q] = (double)change_rate(b, o, q, G);
             tot += rq[q];
           }
-        const float t = __int_as_float(sel[1]) * tot;
-        float acc = 0.0f;
+        const double t = u01_53((uint32_t)sel[1], (uint32_t)sel[2]) * tot;
+        double acc = 0.0;
         nw = -1;
         for (int q = 0; q < no; ++q) {
           if (q == o) continue;
@@ -313,12 +321,10 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         bool dup = false;
         if (!distinct)
           for (int q = 0; q < t; ++q) dup = dup || (aff[q] == lt);
+        if (distinct) aff[t] = lt;  // the groups to sum again below
         if (!dup) {
           int K[kMaxSlots];
-          const float nr = site_rate_at(sc->nb_cell[nb0 + t].x, pt, K);
-          const float d = nr - rate[lt];
-          rate[lt] = nr;
-          atomicAdd(&G[lt >> 6], d);
+          rate[lt] = site_rate_at(sc->nb_cell[nb0 + t].x, pt, K);
           if (BIN && t == 0) {
             // the flipped site itself (first in the list, lane 0): exact cluster-count change
             const int sk = o_flip ? -2 : 2;
@@ -328,9 +334,12 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         }
       }
       __syncthreads();
-    }
-    if ((ev & 127) == 127) {
-      refresh_groups();
+      // the groups of the changed rates, summed again from their 64 rates
+      // (several lanes may form one group's sum: they write the same value)
+      for (int t = tid; t < nb1 - nb0; t += nthr) {
+        const int g = aff[t] >> 6;
+        G[g] = group_rate_sum(rate, g);
+      }
       __syncthreads();
     }
     if (tid == 0 && ((ev + 1) % na.events_per_block) == 0) {

@@ -105,6 +105,12 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+def _check_cint(name: str, v) -> None:
+    """Counts cross the C ABI as C ints: refuse values that would wrap."""
+    if not 0 <= int(v) < 2 ** 31:
+        raise ValueError(f"{name} = {v} is outside [0, 2^31)")
+
+
 class ClexmcError(RuntimeError):
     pass
 
@@ -333,6 +339,8 @@ class Engine:
         0 for the automatic count (about one attempt between different
         occupants per mutable site and pass, as the reference's passes)."""
         k = 0 if kind == "canonical" else 1
+        _check_cint("n_passes", n_passes)
+        _check_cint("sample_period", sample_period)
         self._check(lib().clexmc_run_passes(self.h, C.c_int(k), C.c_int(n_passes), C.c_int(sample_period),
                                             C.c_int(canonical_rounds)), "clexmc_run_passes")
         ms = C.c_double()
@@ -464,6 +472,8 @@ class Engine:
         w the expected residence time of each visited state in Metropolis
         steps, E the formation energy per supercell, N1 the sites holding
         occupant 1."""
+        _check_cint("n_events", n_events)
+        _check_cint("events_per_block", events_per_block)
         n_blocks = n_events // events_per_block
         out = np.zeros((self.n_chains, n_blocks, 6))
         self._check(lib().clexmc_nfold_run(self.h, C.c_int(n_events), C.c_int(events_per_block), _p(out, f64p)),

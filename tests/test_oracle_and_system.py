@@ -299,6 +299,83 @@ def test_oracle_metropolis_detailed_balance_chi2():
     assert chi2 < dof + 5 * math.sqrt(2 * dof), (chi2, dof)
 
 
+def _canonical_histogram(osys, occ0, T, n_samples, thin, seed0):
+    """States visited by the oracle's canonical reference loop, one sample
+    every `thin` passes (each pass a separate run continuing the last)."""
+    counts = {}
+    occ = occ0
+    for it in range(n_samples):
+        r = osys.run(occ, "canonical", T, 0, thin, seed0 + it, full_recompute_samplers=False)
+        occ = r["occ"]
+        st = int(sum(int(v) << i for i, v in enumerate(occ)))
+        counts[st] = counts.get(st, 0) + 1
+    return counts
+
+
+def test_oracle_canonical_detailed_balance_chi2():
+    """The oracle's canonical loop (swap proposals among the candidate lists,
+    sequential-flip swap energy, Metropolis acceptance;
+    CanonicalCalculator.cc:78-88) visits the states of FCC 2x2x2 at 3 B of 8
+    (56 states) in proportion to their exact Boltzmann weights at 400 K
+    (chi-square, one sample per 6 passes), and the same histogram rejects
+    the weights of 1.2 T (power check)."""
+    flat = small_flat()
+    dims = (2, 2, 2)
+    osys = O.OracleSystem(flat, dims)
+    T, nb = 400.0, 3
+    ex = osys.enumerate("canonical", T, fixed_counts=[8 - nb, nb], want_probs=True)
+    occ0 = np.array([1, 1, 1, 0, 0, 0, 0, 0], np.int8)
+    counts = _canonical_histogram(osys, occ0, T, 20000, 6, 5000)
+    assert all(bin(st).count("1") == nb for st in counts)
+    n = sum(counts.values())
+
+    def chi2_of(probs):
+        cnt = np.array([counts.get(st, 0) for st in range(256)], dtype=float)
+        expct = probs * n
+        mask = expct > 5
+        return float(np.sum((cnt[mask] - expct[mask]) ** 2 / expct[mask])), int(mask.sum()) - 1
+
+    chi2, dof = chi2_of(ex["probs"])
+    print(f"canonical chi2 {chi2:.1f} dof {dof}")
+    assert dof >= 50
+    assert chi2 < dof + 5 * math.sqrt(2 * dof), (chi2, dof)
+    hot = osys.enumerate("canonical", 1.2 * T, fixed_counts=[8 - nb, nb], want_probs=True)
+    chi2_hot, _ = chi2_of(hot["probs"])
+    print(f"against 1.2 T: chi2 {chi2_hot:.1f}")
+    assert chi2_hot > dof + 10 * math.sqrt(2 * dof), chi2_hot
+
+
+def test_l12_correlations_closed_form():
+    """L1_2 (x_B = 1/4: B on the cube corners, A on the face centres of the
+    conventional cell), Chebyshev s = -1 (A), +1 (B), correlations per unit
+    cell counted by hand:
+    * point: (1 - 3) / 4 = -1/2;
+    * 1st-neighbour pairs: each B has 12 A neighbours (12 B-A of the 24 bonds
+      per conventional cell), the other 12 bonds are A-A: 0;
+    * 2nd-neighbour pairs (a <100>): corner to corner, face centre to a face
+      centre of the same kind: all alike, +1;
+    * 3rd-neighbour pairs (a <1 1/2 1/2>): each B has 24 A; 24 B-A of 48
+      bonds, the rest A-A: 0;
+    * 4th-neighbour pairs (a <110>): all alike, +1;
+    * nearest-neighbour triangles (8 per site, 32 per cell): each B is in
+      24, all A-A-B (B has no B neighbour), the other 8 are A-A-A:
+      (24 - 8) / 32 = +1/2."""
+    from casmcode_clexmonte_amd.workloads import fcc_binary_basis, l12_occupation
+    bs = fcc_binary_basis()
+    flat = FlatCE(ClusterExpansion(bs, np.zeros(bs.n_functions)))
+    sc = Supercell(bs.prim, (8, 8, 8))
+    occ = l12_occupation(sc)
+    assert occ.sum() * 4 == sc.n_sites
+    corr = O.OracleSystem(flat, sc.dims).corr(occ) / sc.n_cells
+    pairs = sorted((o for o in bs.orbits if o.size == 2), key=lambda o: o.max_length)
+    tri = sorted((o for o in bs.orbits if o.size == 3), key=lambda o: o.max_length)
+    point = [o for o in bs.orbits if o.size == 1][0]
+    expect = {point.functions[0]: -0.5, pairs[0].functions[0]: 0.0, pairs[1].functions[0]: 1.0,
+              pairs[2].functions[0]: 0.0, pairs[3].functions[0]: 1.0, tri[0].functions[0]: 0.5}
+    for f, v in expect.items():
+        assert abs(corr[f] - v) <= 1e-12, (f, corr[f], v)
+
+
 def test_oracle_canonical_conserves_composition(fcc):
     osys = O.OracleSystem(fcc, (6, 6, 6))
     occ = random_occupation(osys.n_sites, 0.25, 3)

@@ -66,17 +66,21 @@ def test_plugin_calculator_vs_oracle(flat, tmp_path, method):
     osys = O.OracleSystem(flat, dims)
     pot = calc.potential
     E = osys.energy(occ)
-    if method == "canonical":
-        assert abs(pot.per_supercell() - E) <= 1e-9 * max(1.0, abs(E))
+    # semi-grand canonical potential (SemiGrandCanonicalCalculator.cc:171-213):
+    # Omega = E - n_cells mu x with x = n_B / n_cells, so Omega = E - mu n_B and
+    # a change of n_B by dn adds -mu dn
+    mu = cond["param_chem_pot"][0] if method != "canonical" else 0.0
+    Omega = E - mu * float(occ.sum())
+    assert abs(pot.per_supercell() - Omega) <= 1e-9 * max(1.0, abs(E))
     rng = np.random.default_rng(2)
     dE, ref = [], []
     for _ in range(20):
         s = int(rng.integers(sc.n_sites))
         dE.append(pot.occ_delta_per_supercell([s], [1 - int(occ[s])]))
-        ref.append(osys.delta(occ, [s], [1 - int(occ[s])]))
+        dn = 1 if occ[s] == 0 else -1
+        ref.append(osys.delta(occ, [s], [1 - int(occ[s])]) - mu * dn)
     dE, ref = np.array(dE), np.array(ref)
-    if method == "canonical":
-        assert np.all(np.abs(dE - ref) <= 1e-5 * np.abs(ref).max() + 1e-6)
+    assert np.all(np.abs(dE - ref) <= 1e-5 * np.abs(ref).max() + 1e-6), (dE, ref)
     rm = P.RunManager()
     rm.seed = 7
     prm = calc.potential  # noqa: F841 (potential stays valid across runs)
