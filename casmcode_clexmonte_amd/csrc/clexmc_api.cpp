@@ -88,6 +88,7 @@ struct clexmc_handle {
   // (kept on the host so that planning a run needs no device read)
   long long samples_bound = 0;
   int max_lds = 0, n_cu = 0;        // device attributes, read at creation
+  unsigned char *noalloc = nullptr; // the device's staging array (clexmc_delta_energy_batch_noalloc)
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
@@ -196,6 +197,7 @@ clexmc_status clexmc_create(const clexmc_system_desc *s, int device, clexmc_hand
   HIP_TRY(hipEventCreate(&h->ev1));
   HIP_TRY(hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
   HIP_TRY(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  HIP_TRY(noalloc_scratch(device, &h->noalloc));
 
   ARG_CHECK(s->n_sublat >= 1 && s->n_sublat <= kMaxSublat, "n_sublat out of range (1..8)");
   ARG_CHECK(s->n_occ_max >= 1 && s->n_occ_max <= kMaxOcc, "n_occ_max out of range (1..4)");
@@ -1083,16 +1085,12 @@ clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out)
   return run_corr(h, chain, 1, corr_out);
 }
 
-clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
-                                        const int32_t *new_occ, int sites_per_prop, int include_chem,
-                                        float *dE_out) {
-  if (!h) return CLEXMC_ERR_ARG;
-  ARG_CHECK(n_prop >= 0, "n_prop < 0");
-  if (n_prop == 0) return CLEXMC_OK;
-  ARG_CHECK(chain && sites && new_occ && dE_out, "null argument");
+// Proposal arguments of the delta-energy entry points, checked on the host.
+static clexmc_status check_proposals(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
+                                     const int32_t *new_occ, int sites_per_prop) {
+  ARG_CHECK(chain && sites && new_occ, "null argument");
   ARG_CHECK(sites_per_prop >= 1 && sites_per_prop <= kMaxSitesPerProp, "sites_per_prop out of range (1..4)");
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
-  HIP_TRY(hipSetDevice(h->device));
   for (int p = 0; p < n_prop; ++p) {
     ARG_CHECK(chain[p] >= 0 && chain[p] < h->n_chains, "proposal chain out of range");
     const DevSupercell &d = h->scs[h->chain_sc[chain[p]]].dev;
@@ -1104,13 +1102,15 @@ clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int3
       ARG_CHECK(v >= 0 && v < h->n_occ[b], "proposal occupant out of range");
     }
   }
+  return CLEXMC_OK;
+}
+
+// Copies the proposals into the device buffers given, evaluates them and
+// copies dE back (the handle's stream, synchronised before returning).
+static hipError_t delta_batch_staged(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
+                                     const int32_t *new_occ, int sites_per_prop, int include_chem, float *dE_out,
+                                     int *d_chain, int *d_sites, int *d_new, float *d_dE) {
   const size_t ns = (size_t)n_prop * sites_per_prop;
-  int *d_chain = nullptr, *d_sites = nullptr, *d_new = nullptr;
-  float *d_dE = nullptr;
-  HIP_TRY(hipMalloc(&d_chain, sizeof(int) * n_prop));
-  HIP_TRY(hipMalloc(&d_sites, sizeof(int) * ns));
-  HIP_TRY(hipMalloc(&d_new, sizeof(int) * ns));
-  HIP_TRY(hipMalloc(&d_dE, sizeof(float) * n_prop));
   hipError_t e = hipMemcpyAsync(d_chain, chain, sizeof(int) * n_prop, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(d_sites, sites, sizeof(int) * ns, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(d_new, new_occ, sizeof(int) * ns, hipMemcpyHostToDevice, h->stream);
@@ -1131,12 +1131,65 @@ clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int3
   }
   if (e == hipSuccess) e = hipMemcpyAsync(dE_out, d_dE, sizeof(float) * n_prop, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return e;
+}
+
+clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int32_t *chain, const int32_t *sites,
+                                        const int32_t *new_occ, int sites_per_prop, int include_chem, float *dE_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_prop >= 0, "n_prop < 0");
+  if (n_prop == 0) return CLEXMC_OK;
+  ARG_CHECK(dE_out, "null argument");
+  const clexmc_status st = check_proposals(h, n_prop, chain, sites, new_occ, sites_per_prop);
+  if (st != CLEXMC_OK) return st;
+  HIP_TRY(hipSetDevice(h->device));
+  const size_t ns = (size_t)n_prop * sites_per_prop;
+  int *d_chain = nullptr, *d_sites = nullptr, *d_new = nullptr;
+  float *d_dE = nullptr;
+  HIP_TRY(hipMalloc(&d_chain, sizeof(int) * n_prop));
+  HIP_TRY(hipMalloc(&d_sites, sizeof(int) * ns));
+  HIP_TRY(hipMalloc(&d_new, sizeof(int) * ns));
+  HIP_TRY(hipMalloc(&d_dE, sizeof(float) * n_prop));
+  const hipError_t e = delta_batch_staged(h, n_prop, chain, sites, new_occ, sites_per_prop, include_chem, dE_out,
+                                          d_chain, d_sites, d_new, d_dE);
   hipFree(d_chain);
   hipFree(d_sites);
   hipFree(d_new);
   hipFree(d_dE);
   if (e != hipSuccess) {
     h->err = std::string("delta_energy_batch: ") + hipGetErrorString(e);
+    return CLEXMC_ERR_HIP;
+  }
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_delta_energy_batch_noalloc(clexmc_handle *h, int n_prop, const int32_t *chain,
+                                                const int32_t *sites, const int32_t *new_occ, int sites_per_prop,
+                                                int include_chem, float *dE_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_prop >= 0, "n_prop < 0");
+  if (n_prop == 0) return CLEXMC_OK;
+  if (n_prop > kNoAllocMaxProposals) {
+    h->err = "clexmc_delta_energy_batch_noalloc: " + std::to_string(n_prop) + " proposals exceed the maximum of " +
+             std::to_string(kNoAllocMaxProposals) + " per call (use clexmc_delta_energy_batch)";
+    return CLEXMC_ERR_ARG;
+  }
+  ARG_CHECK(dE_out, "null argument");
+  const clexmc_status st = check_proposals(h, n_prop, chain, sites, new_occ, sites_per_prop);
+  if (st != CLEXMC_OK) return st;
+  ARG_CHECK(h->noalloc, "no staging array on this device");
+  HIP_TRY(hipSetDevice(h->device));
+  // fixed slices of the device's staging array, sized for the maximum call;
+  // every slice read by the kernel is written by this call's copies first
+  unsigned char *base = h->noalloc;
+  int *d_chain = (int *)base;
+  int *d_sites = (int *)(base + 4 * (size_t)kNoAllocMaxProposals);
+  int *d_new = (int *)(base + (4 + 4 * kMaxSitesPerProp) * (size_t)kNoAllocMaxProposals);
+  float *d_dE = (float *)(base + (4 + 8 * kMaxSitesPerProp) * (size_t)kNoAllocMaxProposals);
+  const hipError_t e = delta_batch_staged(h, n_prop, chain, sites, new_occ, sites_per_prop, include_chem, dE_out,
+                                          d_chain, d_sites, d_new, d_dE);
+  if (e != hipSuccess) {
+    h->err = std::string("delta_energy_batch_noalloc: ") + hipGetErrorString(e);
     return CLEXMC_ERR_HIP;
   }
   return CLEXMC_OK;

@@ -111,7 +111,16 @@ struct NfoldArgs {
                            // sum w E^2, sum w N1^2, sum w E N1
 };
 
+// clexmc_delta_energy_batch_noalloc: proposals per call, and the layout of
+// its file-scope staging array (per proposal: chain, up to kMaxSitesPerProp
+// sites and new occupants, one dE), 40 bytes per proposal, 2.5 MiB in all.
+constexpr int kNoAllocMaxProposals = 65536;
+constexpr size_t kNoAllocScratchBytes = (size_t)kNoAllocMaxProposals * (4 + 8 * kMaxSitesPerProp + 4);
+
 #ifndef __HIPCC_RTC__
+// Device address of the staging array on the current device, looked up once
+// per device (hipGetSymbolAddress) and kept for the life of the process.
+hipError_t noalloc_scratch(int device, unsigned char **ptr);
 hipError_t launch_learned(const RunArgs &args, const LearnArgs &la, int kind, int weights, int n_chains,
                           int threads, size_t lds, hipStream_t st, bool launch);
 size_t learned_lds_bytes(int pad_bytes);

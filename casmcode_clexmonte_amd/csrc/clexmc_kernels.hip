@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "clexmc_device.h"
@@ -318,6 +320,29 @@ __global__ __launch_bounds__(512) void lds_probe_kernel(int mode, int iters, uns
 hipError_t launch_lds_probe(int mode, int iters, int blocks, unsigned *sink, hipStream_t st) {
   hipLaunchKernelGGL(lds_probe_kernel, dim3(blocks), dim3(512), 32768, st, mode, iters, sink);
   return hipGetLastError();
+}
+
+// Staging array of clexmc_delta_energy_batch_noalloc: each GPU holds its own
+// copy at its own address until the process ends.  Host code never passes
+// the name itself to a kernel (in host code it is a stand-in, not a device
+// address); it uses the address hipGetSymbolAddress returns for the device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, unsigned char **ptr) {
+  static std::mutex mu;
+  static std::map<int, unsigned char *> by_device;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = by_device.find(device);
+  if (it != by_device.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  by_device[device] = (unsigned char *)p;
+  *ptr = (unsigned char *)p;
+  return hipSuccess;
 }
 
 hipError_t launch_delta_batch(const DeltaArgs &a, hipStream_t st) {

@@ -66,6 +66,7 @@ def lib():
         L.clexmc_version.restype = C.c_char_p
         L.clexmc_create.argtypes = [C.POINTER(SystemDesc), C.c_int, C.POINTER(C.c_void_p)]
         for name in ["clexmc_add_supercell", "clexmc_load_chains", "clexmc_delta_energy_batch",
+                     "clexmc_delta_energy_batch_noalloc",
                      "clexmc_correlations", "clexmc_run_passes", "clexmc_read_samples",
                      "clexmc_clear_samples", "clexmc_read_state", "clexmc_read_binary_counts",
                      "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device",
@@ -87,7 +88,8 @@ def lib():
 def exported_symbols() -> List[str]:
     """Names of the C-ABI entry points declared in include/clexmc.h."""
     return ["clexmc_create", "clexmc_add_supercell", "clexmc_load_chains",
-            "clexmc_delta_energy_batch", "clexmc_correlations", "clexmc_run_passes",
+            "clexmc_delta_energy_batch", "clexmc_delta_energy_batch_noalloc", "clexmc_correlations",
+            "clexmc_run_passes",
             "clexmc_read_samples", "clexmc_clear_samples", "clexmc_read_state",
             "clexmc_read_binary_counts", "clexmc_last_run_ms", "clexmc_set_block_size",
             "clexmc_philox_device", "clexmc_last_error", "clexmc_destroy", "clexmc_version",

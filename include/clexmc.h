@@ -152,6 +152,20 @@ clexmc_status clexmc_delta_energy_batch(clexmc_handle *h, int n_prop, const int3
                                         const int32_t *sites, const int32_t *new_occ,
                                         int sites_per_prop, int include_chem, float *dE_out);
 
+/* clexmc_delta_energy_batch without any allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops that call
+ * occ_delta_per_supercell per proposal, harnesses that forbid hipMalloc).
+ * Same arguments and results as clexmc_delta_energy_batch, but the proposals
+ * and results are staged in a fixed array in device memory that the library
+ * holds on each GPU for the life of the process (2.5 MiB), so at most 65,536
+ * proposals per call (more is rejected with CLEXMC_ERR_ARG).  Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU (from several
+ * handles or threads at once) race and must not be made. */
+clexmc_status clexmc_delta_energy_batch_noalloc(clexmc_handle *h, int n_prop, const int32_t *chain,
+                                                const int32_t *sites, const int32_t *new_occ, int sites_per_prop,
+                                                int include_chem, float *dE_out);
+
 /* Full per-supercell correlations of one chain in FP64 (Correlations::
  * per_supercell, sampling_functions.cc:121-135).  corr_out[n_corr]. */
 clexmc_status clexmc_correlations(clexmc_handle *h, int chain, double *corr_out);
