@@ -12,15 +12,16 @@ before the timed region.
 
 Second workload (effective samples per second, learned versus plain
 proposal): binary FCC ordering CE (nearest-neighbour pair +0.04 eV), x_B =
-0.25, 32x32x32 supercell (32,768 sites), 128 canonical chains at temperatures
-bracketing the L1_2 ordering temperature T_c (0.7..1.3 T_c; T_c from the CPU
-oracle's heat-capacity scan in tests/golden/fcc_ordering_tc_scan.json).  The
-plain sampler equilibrates the chains and generates the training data; the
+0.25, 32x32x32 supercell (32,768 sites), 128 canonical chains: 16
+temperatures bracketing the L1_2 ordering temperature T_c (0.7..1.3 T_c;
+T_c from the CPU oracle's heat-capacity scan in
+tests/golden/fcc_ordering_tc_scan.json) x 8 independent replicas.  The plain
+sampler equilibrates the chains and generates the training data; the
 network is trained on the device; then each proposal runs the same number of
 sampled passes and ESS/s = sum over chains of n_samples / (2 tau_int(E)) per
 second of sampling.  Every proposal starts from the same equilibrated
-configurations; the <E> of every chain is compared between proposals (3
-sigma, tau-corrected errors).
+configurations; per temperature, <E> is compared between proposals with
+errors from the spread over replicas.
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
 build measures one GPU.
@@ -67,9 +68,40 @@ def tau_int(x: np.ndarray, c: float = 5.0) -> float:
     return max(tau, 0.5)
 
 
-def cpu_baseline(flat, sc, batch, n_chains: int, n_passes: int, threads: int):
+def host_cpu():
+    """The host CPUs the CPU baselines run on: model, CPUs visible to this
+    process, threads used (the process's CPU share: OMP_NUM_THREADS when set,
+    which the GPU hosts set to the CPUs granted per GPU, else every visible
+    CPU) and the oracle's compiler."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(visible, int(env)) if env.isdigit() and int(env) > 0 else visible
+    try:
+        import subprocess
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        cc = "gcc"
+    return {"model": model, "cpus_visible": visible, "threads": threads,
+            "compiler": f"{cc}; oracle/Makefile: -O3 -march=x86-64-v3 (built where the library is built, run here)"}
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # CPU restatement of the reference loop (timed baseline)
+    return O
+
+
+def cpu_baseline(flat, sc, batch, n_chains: int, n_passes: int, threads: int):
+    O = _oracle()
     osys = O.OracleSystem(flat, sc.dims)
     occ = np.stack([batch.occ[i] for i in range(n_chains)])
     r = osys.run_batch(occ, "sgc", batch.temperatures[:n_chains], batch.chem[:n_chains], n_passes,
@@ -77,9 +109,45 @@ def cpu_baseline(flat, sc, batch, n_chains: int, n_passes: int, threads: int):
     return r["n_attempt"] / r["elapsed_s"], r
 
 
+def cpu_baseline_chains(flat, jobs, threads: int):
+    """Independent chains of the oracle's reference loop spread over host
+    threads (one chain per task, no shared state; the foreign calls release
+    the GIL).  jobs: (dims, occ, kind, temperature, chem, n_passes, seed).
+    Returns site-attempts/s over the wall time of all chains and the wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+    O = _oracle()
+    systems = {}
+    for j in jobs:
+        if j[0] not in systems:
+            systems[j[0]] = O.OracleSystem(flat, j[0])
+
+    def one(j):
+        dims, occ, kind, T, chem, n_passes, seed = j
+        r = systems[dims].run(occ, kind, float(T), 0, n_passes, int(seed), chem=chem, full_recompute_samplers=True)
+        return r["n_attempt"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        att = sum(ex.map(one, jobs))
+    wall = time.perf_counter() - t0
+    return att / wall, wall
+
+
 def learned_comparison(args):
     """ESS/s of the plain, learned and informed proposals on the ordering
-    workload (see the module docstring)."""
+    workload (see the module docstring), and their agreement on <E>.
+
+    The 128 chains are 16 temperatures x 8 independent replicas (own initial
+    state, own plain-sampler equilibration, own seeds).  Per temperature and
+    proposal, the mean energy is the mean over replicas of each replica's
+    time average and its error the replicas' standard error, so the error
+    needs no autocorrelation estimate.  Two proposals agree at a temperature
+    when Welch's t of their means stays below the two-sided critical value
+    for a family-wise error of 1 % over the 16 temperatures (Bonferroni).
+    ESS per chain is n / (2 tau_int(E)) with Sokal's window; the run length
+    is reported in units of the longest tau_int."""
+    from scipy import stats
+
     from casmcode_clexmonte_amd.engine import Engine
     from casmcode_clexmonte_amd.system import FlatCE, Supercell
     from casmcode_clexmonte_amd.workloads import fcc_ordering_ce, l12_occupation, random_occupation
@@ -90,28 +158,39 @@ def learned_comparison(args):
     flat = FlatCE(ce)
     L = args.c4_size
     sc = Supercell(ce.prim, (L, L, L))
-    nch = args.c4_chains
-    temps = Tc * np.linspace(0.7, 1.3, nch)
+    nT, nrep = args.c4_temps, args.c4_replicas
+    nch = nT * nrep
+    tgrid = Tc * np.linspace(0.7, 1.3, nT)
+    temps = np.repeat(tgrid, nrep)
     occ_ord = l12_occupation(sc)
     occs = [occ_ord if T < Tc else random_occupation(sc.n_sites, 0.25, 4000 + c) for c, T in enumerate(temps)]
     eng = Engine(flat, device=0)
     sid = eng.add_supercell(sc, n_pair_phases=32, seed=11)
     eng.load_chains([sid] * nch, occs, temps, 77000 + np.arange(nch))
-    eng.run_passes("canonical", args.c4_equil, args.c4_equil, canonical_rounds=2)
+    # automatic matchings and pair phases (the calculators' default)
+    eng.run_passes("canonical", args.c4_equil, args.c4_equil, canonical_rounds=0)
     t0 = time.perf_counter()
     loss = eng.mlp_train(n_per_chain=args.c4_train_samples, n_steps=args.c4_train_steps, lr=3e-3, seed=5)
     train_s = time.perf_counter() - t0
     # every proposal starts from the same equilibrated configurations
     snap = [eng.read_state(c)["occ"].copy() for c in range(nch)]
     res = {}
-    means = {}
+    per_T = {}
+    cpu_line = None
+    if not args.no_cpu:
+        cpu = host_cpu()
+        P = args.cpu_extra_passes
+        jobs = [(sc.dims, snap[c], "canonical", temps[c], None, P, 5150 + c) for c in range(nch)]
+        v, w = cpu_baseline_chains(flat, jobs, cpu["threads"])
+        cpu_line = {"value": v, "unit": "site-attempts/s", "cores": cpu["threads"], "kind": "port",
+                    "sample": f"plain proposal, all {nch} chains x {P} passes from the equilibrated states, {w:.1f} s"}
     for i, prop in enumerate(("plain", "learned", "informed")):
         eng.load_chains([sid] * nch, snap, temps, 91000 + 1000 * i + np.arange(nch))
         eng.set_proposal(prop, rounds_per_group=args.c4_rounds)
         eng.clear_samples()
         a0 = [eng.read_state(c) for c in range(nch)]
         t0 = time.perf_counter()
-        eng.run_passes("canonical", args.c4_passes, 1, canonical_rounds=2)
+        eng.run_passes("canonical", args.c4_passes, 1, canonical_rounds=0)
         wall = time.perf_counter() - t0
         a1 = [eng.read_state(c) for c in range(nch)]
         E, _ = eng.read_samples()
@@ -122,14 +201,14 @@ def learned_comparison(args):
         ess = float(np.sum(n / (2.0 * taus)))
         att = sum(b["n_attempt"] - a["n_attempt"] for a, b in zip(a0, a1))
         acc = sum(b["n_accept"] - a["n_accept"] for a, b in zip(a0, a1))
-        m = E[:, burn:].mean(1)
-        se = E[:, burn:].std(1) * np.sqrt(2.0 * taus / n)
-        means[prop] = (m, se)
+        m = E[:, burn:].mean(1).reshape(nT, nrep)
+        per_T[prop] = (m.mean(1), m.std(1, ddof=1) / np.sqrt(nrep))
         # ESS over the post-burn-in samples per second of the passes that made them
         res[prop] = {"ess_per_s": ess / (wall * n / E.shape[1]),
                      "passes_per_s": args.c4_passes / wall, "attempts_per_s": att / wall,
                      "acceptance": acc / max(1, att), "tau_int_E_median_passes": float(np.median(taus)),
-                     "tau_int_E_max_passes": float(np.max(taus))}
+                     "tau_int_E_max_passes": float(np.max(taus)),
+                     "sampled_passes_over_max_tau": float(n / np.max(taus))}
         # ESS/s by temperature band (fractions of T_c)
         bands = {}
         for lo, hi in ((0.0, 0.9), (0.9, 1.0), (1.0, 1.1), (1.1, 9.9)):
@@ -137,18 +216,27 @@ def learned_comparison(args):
             if sel.any():
                 bands[f"{lo:.1f}-{hi:.1f}Tc"] = float(np.sum(n / (2.0 * taus[sel])) / (wall * n / E.shape[1]))
         res[prop]["ess_per_s_by_T_band"] = bands
+    dof = 2 * (nrep - 1)
+    t_crit = float(stats.t.ppf(1.0 - 0.01 / (2 * nT), dof))
     for prop in ("learned", "informed"):
-        (m0, s0), (m1, s1) = means["plain"], means[prop]
-        z = np.abs(m1 - m0) / np.sqrt(s0 ** 2 + s1 ** 2 + 1e-30)
-        res[prop]["E_agree_3sigma_frac"] = float(np.mean(z <= 3.0))
-        res[prop]["E_max_z"] = float(z.max())
+        (m0, s0), (m1, s1) = per_T["plain"], per_T[prop]
+        t = np.abs(m1 - m0) / np.sqrt(s0 ** 2 + s1 ** 2 + 1e-30)
+        res[prop]["E_agreement"] = {
+            "temperatures": nT, "replicas_per_temperature": nrep, "error": "standard error over replicas",
+            "welch_t_max": float(t.max()), "t_critical_1pct_familywise": t_crit,
+            "temperatures_agreeing": int(np.sum(t <= t_crit)),
+            "frac_within_3_sigma": float(np.mean(t <= 3.0))}
         res[prop]["ess_per_s_vs_plain"] = res[prop]["ess_per_s"] / res["plain"]["ess_per_s"]
     eng.close()
-    return {"workload": f"fcc_ordering_{L}x{L}x{L}_canonical_xB0.25_{nch}chains_T{temps[0]:.0f}-{temps[-1]:.0f}K",
+    return {"workload": f"fcc_ordering_{L}x{L}x{L}_canonical_xB0.25_{nch}chains_{nT}T_x_{nrep}replicas_"
+                        f"T{tgrid[0]:.0f}-{tgrid[-1]:.0f}K",
             "Tc_K": Tc, "sampled_passes": args.c4_passes, "equilibration_passes": args.c4_equil,
             "rounds_per_group": args.c4_rounds,
             "training": {"samples": args.c4_train_samples * nch, "adam_steps": args.c4_train_steps,
                          "seconds": train_s, "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
+            "cpu_baseline_plain": cpu_line,
+            "mean_E_per_cell_by_T": {p: [float(x) for x in per_T[p][0]] for p in per_T},
+            "err_E_per_cell_by_T": {p: [float(x) for x in per_T[p][1]] for p in per_T},
             **res}
 
 
@@ -188,8 +276,7 @@ def workload_c1(args):
            "E_per_cell_mean": float(e.mean()), "E_per_cell_err": float(e.std() * np.sqrt(2 * tau / len(e))),
            "tau_int_E_passes": tau}
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # CPU restatement of the reference loop (timed baseline)
+        O = _oracle()
         osys = O.OracleSystem(flat, sc.dims)
         t0 = time.perf_counter()
         r = osys.run(occ, "canonical", 600.0, 0, 1000, 11, full_recompute_samplers=True)
@@ -225,9 +312,18 @@ def workload_c3(args):
     drift = eng.reanchor_energies()
     sites = sum(len(o) for o in b.occ)
     eng.close()
-    return {"workload": "hcp_ternary_canonical_64chains_16x16x10-48x48x32", "site_attempts_per_s": rate,
-            "sites_total": sites, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
-            "launch": info, "max_energy_drift_eV": drift}
+    out = {"workload": "hcp_ternary_canonical_64chains_16x16x10-48x48x32", "site_attempts_per_s": rate,
+           "sites_total": sites, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
+           "launch": info, "max_energy_drift_eV": drift}
+    if not args.no_cpu:
+        cpu = host_cpu()
+        P = args.cpu_extra_passes
+        jobs = [(tuple(d), b.occ[i], "canonical", b.temperatures[i], None, P, b.seeds[i]) for i, d in enumerate(b.dims)]
+        v, w = cpu_baseline_chains(flat, jobs, cpu["threads"])
+        out["cpu_baseline"] = {"value": v, "unit": "site-attempts/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"all 64 chains x {P} passes, one chain per host thread task, {w:.1f} s"}
+        out["speedup_vs_cpu"] = rate / v
+    return out
 
 
 def workload_c5(args):
@@ -249,9 +345,18 @@ def workload_c5(args):
     tune["slabs_per_chain"] = eng.last_run_slabs()
     drift = eng.reanchor_energies()
     eng.close()
-    return {"workload": "bcc_quaternary_64x64x64_canonical_equiatomic_32chains_T300-2000",
-            "site_attempts_per_s": rate, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
-            "autotune": tune, "max_energy_drift_eV": drift}
+    out = {"workload": "bcc_quaternary_64x64x64_canonical_equiatomic_32chains_T300-2000",
+           "site_attempts_per_s": rate, "passes": args.extra_passes, "wall_s": wall, "kernel_ms": kms,
+           "autotune": tune, "max_energy_drift_eV": drift}
+    if not args.no_cpu:
+        cpu = host_cpu()
+        P = max(1, args.cpu_extra_passes // 4)
+        jobs = [(sc.dims, b.occ[i], "canonical", b.temperatures[i], None, P, b.seeds[i]) for i in range(n)]
+        v, w = cpu_baseline_chains(flat, jobs, cpu["threads"])
+        out["cpu_baseline"] = {"value": v, "unit": "site-attempts/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": f"all 32 chains x {P} passes, one chain per host thread task, {w:.1f} s"}
+        out["speedup_vs_cpu"] = rate / v
+    return out
 
 
 def workload_nfold(args):
@@ -304,8 +409,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-chains", type=int, default=32)
+    ap.add_argument("--cpu-chains", type=int, default=0, help="chains of the CPU baseline (0: all 256)")
     ap.add_argument("--cpu-passes", type=int, default=40)
+    ap.add_argument("--cpu-extra-passes", type=int, default=4,
+                    help="passes per chain of the CPU baselines of the HCP, ordering and BCC workloads")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal ESS comparison")
     ap.add_argument("--no-extra", action="store_true", help="skip the single-chain, HCP and BCC workloads")
     ap.add_argument("--extra-passes", type=int, default=200)
@@ -313,9 +420,10 @@ def main():
     ap.add_argument("--nfold-generic", action="store_true",
                     help="run the N-fold workload on the generic (interpreted) kernel")
     ap.add_argument("--c4-size", type=int, default=32)
-    ap.add_argument("--c4-chains", type=int, default=128)
-    ap.add_argument("--c4-equil", type=int, default=1000)
-    ap.add_argument("--c4-passes", type=int, default=2000)
+    ap.add_argument("--c4-temps", type=int, default=16)
+    ap.add_argument("--c4-replicas", type=int, default=8)
+    ap.add_argument("--c4-equil", type=int, default=2000)
+    ap.add_argument("--c4-passes", type=int, default=12000)
     ap.add_argument("--c4-rounds", type=int, default=32)
     ap.add_argument("--c4-train-samples", type=int, default=512)
     ap.add_argument("--c4-train-steps", type=int, default=200)
@@ -373,10 +481,14 @@ def main():
     expected = n_chains * sc.n_sites * args.steps
     assert attempts == expected, (attempts, expected)
     value = attempts / wall
-    # effective sample size of E per chain from the per-pass samples
-    E, cnt = eng.read_samples()
-    taus = np.array([tau_int(E[c]) for c in range(n_chains)])
-    ess_total = float(np.sum(E.shape[1] / (2.0 * taus)))
+    # effective sample size of E per chain from the per-pass samples (only
+    # for runs long enough to estimate tau_int: at least 500 passes)
+    ess_total, tau_med = None, None
+    if args.steps >= 500:
+        E, cnt = eng.read_samples()
+        taus = np.array([tau_int(E[c]) for c in range(n_chains)])
+        ess_total = float(np.sum(E.shape[1] / (2.0 * taus)))
+        tau_med = float(np.median(taus))
     bytes_per_attempt = 1 + 54  # the site and its 54 interaction neighbours, int8 in LDS
     achieved = attempts * bytes_per_attempt / (kernel_ms * 1e-3) / 1e9
     # measured LDS read throughput of this device: byte gathers and 16-byte reads
@@ -410,8 +522,8 @@ def main():
                               "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
                               "frac": attempts * VALU_OPS_PER_ATTEMPT / (kernel_ms * 1e-3) / VALU_PEAK_LANE_OPS}},
         "launch": dict(launch, gpu_max_hw_queues=os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")),
-        "ess_per_s": ess_total / wall,
-        "tau_int_E_median_passes": float(np.median(taus)),
+        "ess_per_s": ess_total / wall if ess_total is not None else None,
+        "tau_int_E_median_passes": tau_med,
         "acceptance": float(sum(eng.read_state(c)["n_accept"] for c in range(n_chains)) / max(1, att1)),
     }
     eng.close()
@@ -421,13 +533,16 @@ def main():
         out["workloads"] = {"single_chain": workload_c1(args), "hcp_ternary_uneven": workload_c3(args),
                             "bcc_quaternary_64": workload_c5(args), "nfold_low_T": workload_nfold(args)}
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu_rate, r = cpu_baseline(flat, sc, batch, args.cpu_chains, args.cpu_passes, threads)
+        cpu = host_cpu()
+        threads = cpu["threads"]
+        n_cpu = min(args.cpu_chains, n_chains) if args.cpu_chains > 0 else n_chains
+        cpu_rate, r = cpu_baseline(flat, sc, batch, n_cpu, args.cpu_passes, threads)
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "site-attempts/s", "cores": threads, "kind": "port",
-                               "sample": f"{args.cpu_chains} chains x {args.cpu_passes} passes of the same "
+                               "sample": f"{n_cpu} chains x {args.cpu_passes} passes of the same "
                                          f"workload, oracle restatement of occupation_metropolis_v2 "
-                                         f"(FP64, mt19937_64, per-pass full-recompute samplers), "
-                                         f"{r['elapsed_s']:.1f} s wall"}
+                                         f"(FP64, mt19937_64, per-pass full-recompute samplers), chains "
+                                         f"spread over {threads} threads, {r['elapsed_s']:.1f} s wall"}
+        out["cpu_host"] = cpu
         out["speedup_vs_cpu"] = value / cpu_rate
     print(json.dumps(out))
 

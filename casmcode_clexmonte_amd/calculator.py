@@ -64,6 +64,11 @@ class SamplingFixtureParams:
     canonical_rounds: int = 0       # 0: automatic, about one swap attempt per mutable site and pass
     n_pair_phases: int = 32
     completion: Optional[CompletionCheckParams] = None
+    # results_io "json" kwargs: run_series writes <output_dir>/<label>/run.<i>/
+    # observations.json (every sample of every sampler) and trajectory.json
+    # (the configuration at every sample)
+    write_observations: bool = False
+    write_trajectory: bool = False
 
     @classmethod
     def from_casm(cls, label: str, d: dict, **kw) -> "SamplingFixtureParams":
@@ -77,9 +82,19 @@ class SamplingFixtureParams:
         legacy = {"formation_energy": "clex.formation_energy", "formation_energy_corr": "corr.formation_energy"}
         for q in smp.get("quantities", []):
             names.append(legacy.get(q, q))
+        rio = d.get("results_io", {})
+        rkw = rio.get("kwargs", {}) if rio.get("method", "json") == "json" else {}
+        kw.setdefault("write_observations", bool(rkw.get("write_observations", False)))
+        kw.setdefault("write_trajectory", bool(rkw.get("write_trajectory", False)) or
+                      bool(smp.get("sample_trajectory", False)))
         return cls(label=label, sampler_names=names or cls().sampler_names,
                    sample_period=int(smp.get("period", 1)), n_equilibration_passes=0,
                    completion=CompletionCheckParams.from_casm(d.get("completion_check", {})), **kw)
+
+    def wants_correlations(self) -> bool:
+        """Samplers that read every chain's correlations at each sample."""
+        return "corr.formation_energy" in self.sampler_names or \
+            "clex.formation_energy.sparse_corr" in self.sampler_names
 
 
 class MontePotential:
@@ -124,6 +139,17 @@ class MonteCalculator:
         self.engine = Engine(system.flat, device=device)
         self._sc_ids: Dict[tuple, int] = {}
         self.seed = int(self.params.get("seed", 0))
+        # semi-grand canonical multiswaps (the system's
+        # "semigrand_canonical_multiswaps": a list of types, each a list of
+        # (unit, from, to, count), or the largest number of single swaps to
+        # combine); composition then changes through multiswaps only
+        msw = self.params.get("semigrand_canonical_multiswaps")
+        if msw is not None and method in _SGC:
+            if method == "nfold":
+                raise ValueError("Error in MonteCalculator: the nfold method takes single-site changes only")
+            from .system import make_multiswaps
+            types = make_multiswaps(system.flat, int(msw)) if isinstance(msw, int) else msw
+            self.engine.set_multiswaps(types, int(self.params.get("multiswaps_per_pass", 0)))
 
     # -- conditions ------------------------------------------------------------
     def _validate(self, state: MonteCarloState):
@@ -190,7 +216,11 @@ class MonteCalculator:
             self.engine.reanchor_energies()
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
         acc0 = [self.engine.read_state(c)["n_accept"] for c in range(len(states))]
-        self._corr = [[] for _ in states] if "corr.formation_energy" in p.sampler_names else None
+        self._corr = [[] for _ in states] if p.wants_correlations() else None
+        self._traj = [[] for _ in states] if p.write_trajectory else None
+        # passes from the start of sampling to its first sample (the kernels
+        # sample after passes g, counted from the load, with (g + 1) % period == 0)
+        self._count0 = p.sample_period - (max(p.n_equilibration_passes, 0) % p.sample_period)
         if p.completion is None:
             self._sampled_passes(kind, p.n_passes, p)
             self.engine.reanchor_energies()
@@ -200,12 +230,16 @@ class MonteCalculator:
         else:
             E, cnt, ends = self._run_to_completion(states, p, t0)
         out = []
+        self.last_observations, self.last_trajectories = [], []
         for c, s in enumerate(states):
             n_s, res, st, elapsed = ends[c]
             s.occupation = st["occ"]
             corr = np.asarray(self._corr[c][:n_s]) if self._corr is not None else None
             out.append(self._summary(s, E[c, :n_s], cnt[c, :n_s], st["n_attempt"] - att0[c],
                                      st["n_accept"] - acc0[c], elapsed, res, p, corr))
+            self.last_observations.append(self._observations(s, E[c, :n_s], cnt[c, :n_s], p, corr))
+            self.last_trajectories.append(self._trajectory(s, self._traj[c][:n_s], p)
+                                          if self._traj is not None else None)
         return out
 
     def _sampled_passes(self, kind: str, n_passes: int, p: SamplingFixtureParams):
@@ -215,7 +249,7 @@ class MonteCalculator:
         (g + 1 a multiple of the period, g counted from the load) and every
         chain's FP64 correlations are read there."""
         per = p.sample_period
-        if self._corr is None:
+        if self._corr is None and getattr(self, "_traj", None) is None:
             self.engine.run_passes(kind, n_passes, per, p.canonical_rounds)
             self._gpass += n_passes
             return
@@ -226,9 +260,12 @@ class MonteCalculator:
             self.engine.run_passes(kind, stop - self._gpass, per, p.canonical_rounds)
             self._gpass = stop
             if stop == due + 1:
-                for c in range(len(self._corr)):
-                    n = int(np.prod(self.engine.supercells[self.engine.chain_sc[c]].dims))
-                    self._corr[c].append(self.engine.correlations(c) / n)
+                for c in range(len(self._corr or self._traj)):
+                    if self._corr is not None:
+                        n = int(np.prod(self.engine.supercells[self.engine.chain_sc[c]].dims))
+                        self._corr[c].append(self.engine.correlations(c) / n)
+                    if self._traj is not None:
+                        self._traj[c].append(self.engine.read_state(c)["occ"].copy())
 
     def _run_to_completion(self, states, p: SamplingFixtureParams, t0: float):
         cc = p.completion
@@ -271,9 +308,9 @@ class MonteCalculator:
         flat = self.system.flat
         if not flat.binary:
             raise ValueError("Error in MonteCalculator: the nfold method reports binary cluster expansions only")
-        if "corr.formation_energy" in p.sampler_names:
+        if p.wants_correlations():
             raise ValueError("Error in MonteCalculator: the nfold method samples energies and compositions only "
-                             "(no corr.formation_energy)")
+                             "(no corr.formation_energy or clex.formation_energy.sparse_corr)")
         if p.completion is not None:
             raise ValueError("Error in MonteCalculator: nfold runs take a fixed number of passes")
         shapes = {tuple(int(x) for x in s.supercell) for s in states}
@@ -375,6 +412,66 @@ class MonteCalculator:
                 "completion_check_results": completion}
 
     # -- results -----------------------------------------------------------------
+    def _corr_samplers(self, p: SamplingFixtureParams) -> Dict[str, tuple]:
+        """The requested correlation samplers with (local correlation indices,
+        component names): "corr.formation_energy", every correlation
+        (sampling_functions.cc:121-135), and "clex.formation_energy.sparse_corr",
+        the correlations of the non-zero ECIs (sampling_functions.cc:185-219).
+        Components are named and ordered by the ECI file's linear function
+        index for an expansion read from CASM files (casm_io), else by index."""
+        ce = self.system.flat.ce
+        eci = self.system.flat.eci
+        c2l = getattr(ce, "casm_to_local", None)
+        if c2l and len(c2l) == len(eci):
+            order = [(int(cf), int(lf)) for cf, lf in sorted(c2l.items())]
+        else:
+            order = [(i, i) for i in range(len(eci))]
+        out = {}
+        if "corr.formation_energy" in p.sampler_names:
+            out["corr.formation_energy"] = ([lf for _, lf in order], [str(cf) for cf, _ in order])
+        if "clex.formation_energy.sparse_corr" in p.sampler_names:
+            nz = [(cf, lf) for cf, lf in order if eci[lf] != 0.0]
+            out["clex.formation_energy.sparse_corr"] = ([lf for _, lf in nz], [str(cf) for cf, _ in nz])
+        return out
+
+    def _observations(self, state, E_sc, counts, p: SamplingFixtureParams, corr=None) -> dict:
+        """Every sample of every sampler of one run (the observations file):
+        "count" (pass of each sample since sampling began) and, per sampler,
+        component names, shape and one row of values per sample."""
+        ser = self._series(state, E_sc, counts)
+        axes = self.system.composition_axes
+        ns = len(E_sc)
+        var_names = [chr(ord("a") + i) for i in range(axes.n_independent)]
+        out = {"count": [self._count0 + k * p.sample_period for k in range(ns)]}
+
+        def put(name, names, shape, rows):
+            out[name] = {"component_names": names, "shape": shape, "value": [[float(v) for v in r] for r in rows]}
+
+        put("clex.formation_energy", [], [], ser["clex.formation_energy"][:, None])
+        put("potential_energy", [], [], ser["potential_energy"][:, None])
+        put("mol_composition", list(axes.components), [len(axes.components)],
+            np.stack([ser[f"mol_composition.{k}"] for k in axes.components], axis=1))
+        put("param_composition", var_names, [len(var_names)],
+            np.stack([ser[f"param_composition.{k}"] for k in var_names], axis=1) if var_names
+            else np.zeros((ns, 0)))
+        put("temperature", [], [], np.full((ns, 1), float(state.conditions["temperature"])))
+        if self.method in _SGC:
+            mu = np.asarray(state.conditions["param_chem_pot"], float)
+            put("param_chem_pot", var_names, [len(var_names)], np.tile(mu, (ns, 1)))
+        if corr is not None:
+            for name, (idx, names) in self._corr_samplers(p).items():
+                put(name, names, [len(names)], np.asarray(corr)[:, idx])
+        return out
+
+    def _trajectory(self, state, occs, p: SamplingFixtureParams) -> list:
+        """The configuration at every sample (the trajectory file): diagonal
+        supercell transformation matrix and occupation (site order
+        b * n_cells + i + L1 * (j + L2 * k))."""
+        L = [int(x) for x in state.supercell]
+        T = [[L[0], 0, 0], [0, L[1], 0], [0, 0, L[2]]]
+        return [{"count": self._count0 + k * p.sample_period, "transformation_matrix_to_super": T,
+                 "dof": {"occ": [int(v) for v in o]}} for k, o in enumerate(occs)]
+
     def _series(self, state, E_sc, counts) -> Dict[str, np.ndarray]:
         """Per-sample values of the samplers, per unit cell: clex.formation_energy,
         potential_energy, mol_composition.<component>, param_composition.<axis>."""
@@ -436,11 +533,11 @@ class MonteCalculator:
             stats[name] = d
         if corr is not None:
             corr = corr[n_eq:]
-            names = [str(i) for i in range(corr.shape[1])]
-            d = {"component_names": names, "shape": [len(names)]}
-            for i, k in enumerate(names):
-                d[k] = stat(f"corr.formation_energy.{k}", corr[:, i])
-            stats["corr.formation_energy"] = d
+            for name, (idx, names) in self._corr_samplers(p).items():
+                d = {"component_names": names, "shape": [len(names)]}
+                for i, k in zip(idx, names):
+                    d[k] = stat(f"{name}.{k}", corr[:, i])
+                stats[name] = d
         analysis = {"heat_capacity": {"shape": [], "value": [A.heat_capacity(pe, T, n)]}}
         if self.method in _SGC:
             mu = np.asarray(state.conditions["param_chem_pot"], float)
@@ -526,6 +623,17 @@ def run_series(calculator: MonteCalculator, run_params, output_dir: Optional[str
                     old_summary = json.load(f)
     todo = states[len(done):]
     summaries = calculator.run_multistate(todo, params) if todo else []
+    if output_dir is not None and todo and (params.write_observations or params.write_trajectory):
+        # <output_dir>/<fixture>/run.<i>/, i counting every run of the series from 1
+        for k in range(len(todo)):
+            run_dir = os.path.join(output_dir, fixture, f"run.{len(done) + k + 1}")
+            os.makedirs(run_dir, exist_ok=True)
+            if params.write_observations:
+                with open(os.path.join(run_dir, "observations.json"), "w") as f:
+                    json.dump(calculator.last_observations[k], f)
+            if params.write_trajectory:
+                with open(os.path.join(run_dir, "trajectory.json"), "w") as f:
+                    json.dump(calculator.last_trajectories[k], f)
     merged = merge_summaries(([old_summary] if old_summary else []) + summaries) if (old_summary or summaries) \
         else {}
     if output_dir is not None:

@@ -97,6 +97,10 @@ struct clexmc_handle {
   SlabRecord *d_slab_rec = nullptr;  // [n_chains][kSlabMax][kSlabRecords] (slab launches)
   int slab_rec_chains = 0;
   int global_swaps = 1;             // canonical serial random-pair swaps per pass
+  // semi-grand canonical multiswaps (device tables, kept until replaced)
+  int n_mswaps = 0, mswap_per_pass = 0;
+  int4 *d_mswap_parts = nullptr;
+  int *d_mswap_begin = nullptr, *d_mswap_rev = nullptr;
   int row_pitch = -1;               // padded image row pitch: -1 automatic, else extra bytes per row
   int n_slots = 0;                  // binary path: orbits with >= 1 site
   int lanes_per_site = 0;           // 0 automatic (1; the autotuner may pick 2), 1, 2
@@ -394,9 +398,43 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
       ARG_CHECK(std::adjacent_find(sites.begin(), sites.end()) == sites.end(),
                 "supercell too small for the cluster expansion: a cluster wraps onto itself");
     }
-  // periodic colouring
-  long best_n = -1;
-  int bm[3] = {0, 0, 0};
+  // periodic colouring of the cells: diagonal residues (i mod m1, j mod m2,
+  // k mod m3) or a linear form (i + a j + c k) mod M, conflict-free (no
+  // same-sublattice interaction offset inside a class) with at most
+  // kMaxClassSites cells per class.  Preferred: the most parities (mod 2) of
+  // nearest-neighbour offsets among the class translations (so period-2
+  // orderings driven by nearest neighbours, such as L1_2 and L1_0 on FCC or
+  // B2 on BCC, leave no class pure), then the fewest classes.
+  std::set<std::array<int, 3>> par;  // parities of same-sublattice offsets in the shortest pair orbit
+  for (int a = 0; a < h->n_orbits && par.empty(); ++a) {
+    if (h->orbit_k[a] != 2) continue;
+    for (int b = 0; b < nb; ++b)
+      for (int e = h->flower_begin[b]; e < h->flower_begin[b + 1]; ++e) {
+        if (h->entry_orbit[e] != a) continue;
+        const int q = h->entry_site_begin[e];
+        if (h->entry_sites[4 * q] != b) continue;
+        par.insert({((h->entry_sites[4 * q + 1] % 2) + 2) % 2, ((h->entry_sites[4 * q + 2] % 2) + 2) % 2,
+                    ((h->entry_sites[4 * q + 3] % 2) + 2) % 2});
+      }
+  }
+  std::vector<std::array<int, 3>> same;
+  for (int b = 0; b < nb; ++b)
+    for (auto const &o : offs[b])
+      if (o[0] == b) same.push_back({o[1], o[2], o[3]});
+  constexpr long kMaxClassSites = 4096;
+  long best_rank_score = -1, best_n = -1;
+  int best_kind = -1, bm[3] = {0, 0, 0};  // kind 0: residues bm; kind 1: linear (M, a, c) = bm
+  auto consider = [&](int kind, int p0, int p1, int p2, long n_classes, int score) {
+    if (nc / n_classes > kMaxClassSites) return;
+    if (best_kind < 0 || score > best_rank_score || (score == best_rank_score && n_classes < best_n)) {
+      best_kind = kind;
+      best_rank_score = score;
+      best_n = n_classes;
+      bm[0] = p0;
+      bm[1] = p1;
+      bm[2] = p2;
+    }
+  };
   for (int m1 = 1; m1 <= L[0]; ++m1) {
     if (L[0] % m1) continue;
     for (int m2 = 1; m2 <= L[1]; ++m2) {
@@ -404,35 +442,55 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
       for (int m3 = 1; m3 <= L[2]; ++m3) {
         if (L[2] % m3) continue;
         bool ok = true;
-        for (int b = 0; b < nb && ok; ++b)
-          for (auto const &o : offs[b])
-            if (o[0] == b && o[1] % m1 == 0 && o[2] % m2 == 0 && o[3] % m3 == 0) {
-              ok = false;
-              break;
-            }
-        long n = (long)m1 * m2 * m3;
-        if (ok && (best_n < 0 || n < best_n)) {
-          best_n = n;
-          bm[0] = m1;
-          bm[1] = m2;
-          bm[2] = m3;
-        }
+        for (auto const &d : same)
+          if (d[0] % m1 == 0 && d[1] % m2 == 0 && d[2] % m3 == 0) {
+            ok = false;
+            break;
+          }
+        if (!ok) continue;
+        int score = 0;
+        for (auto const &v : par)
+          score += ((m1 % 2 || v[0] == 0) && (m2 % 2 || v[1] == 0) && (m3 % 2 || v[2] == 0)) ? 1 : 0;
+        consider(0, m1, m2, m3, (long)m1 * m2 * m3, score);
+      }
+    }
+  }
+  for (int M = 2; M <= L[0]; ++M) {
+    if (L[0] % M) continue;
+    for (int a = 0; a < M; ++a) {
+      if (((long)a * L[1]) % M) continue;
+      for (int c = 0; c < M; ++c) {
+        if (((long)c * L[2]) % M) continue;
+        bool ok = true;
+        for (auto const &d : same)
+          if ((((long)d[0] + (long)a * d[1] + (long)c * d[2]) % M + M) % M == 0) {
+            ok = false;
+            break;
+          }
+        if (!ok) continue;
+        int score = 0;
+        for (auto const &v : par) score += (M % 2 || (v[0] + a * v[1] + c * v[2]) % 2 == 0) ? 1 : 0;
+        consider(1, M, a, c, M, score);
       }
     }
   }
   cb.assign(1, 0);
   cs.clear();
-  if (best_n > 0) {
+  if (best_kind >= 0) {
+    const long n_lab = best_kind == 0 ? (long)bm[0] * bm[1] * bm[2] : bm[0];
+    std::vector<long> lab(nc);
+    for (long cell = 0; cell < nc; ++cell) {
+      const int i = (int)(cell % L[0]), j = (int)((cell / L[0]) % L[1]), k = (int)(cell / ((long)L[0] * L[1]));
+      lab[cell] = best_kind == 0 ? (i % bm[0]) + (long)bm[0] * ((j % bm[1]) + (long)bm[1] * (k % bm[2]))
+                                 : ((long)i + (long)bm[1] * j + (long)bm[2] * k) % bm[0];
+    }
     for (int b = 0; b < nb; ++b) {
       if (h->n_occ[b] < 2) continue;
-      for (int r3 = 0; r3 < bm[2]; ++r3)
-        for (int r2 = 0; r2 < bm[1]; ++r2)
-          for (int r1 = 0; r1 < bm[0]; ++r1) {
-            for (int k = r3; k < L[2]; k += bm[2])
-              for (int j = r2; j < L[1]; j += bm[1])
-                for (int i = r1; i < L[0]; i += bm[0]) cs.push_back((int)lin(b, i, j, k));
-            cb.push_back((int)cs.size());
-          }
+      for (long r = 0; r < n_lab; ++r) {
+        for (long cell = 0; cell < nc; ++cell)
+          if (lab[cell] == r) cs.push_back((int)(b * nc + cell));
+        cb.push_back((int)cs.size());
+      }
     }
   } else {
     std::vector<std::set<long>> nbr(N);
@@ -1216,6 +1274,80 @@ clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs) {
   return CLEXMC_OK;
 }
 
+static void free_mswaps(clexmc_handle *h) {
+  hipFree(h->d_mswap_parts);
+  hipFree(h->d_mswap_begin);
+  hipFree(h->d_mswap_rev);
+  h->d_mswap_parts = nullptr;
+  h->d_mswap_begin = nullptr;
+  h->d_mswap_rev = nullptr;
+  h->n_mswaps = 0;
+}
+
+clexmc_status clexmc_set_multiswaps(clexmc_handle *h, int n_types, const int *type_begin, const int *parts,
+                                    int per_pass) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_types >= 0 && n_types <= kMaxMswapTypes, "n_types out of range (0..256)");
+  ARG_CHECK(per_pass >= 0 && per_pass <= (1 << 24), "multiswaps per pass out of range");
+  HIP_TRY(hipSetDevice(h->device));
+  if (n_types == 0) {
+    free_mswaps(h);
+    h->mswap_per_pass = 0;
+    return CLEXMC_OK;
+  }
+  ARG_CHECK(type_begin && parts && type_begin[0] == 0, "type_begin[0] must be 0");
+  // a part: (asymmetric unit, occupant from, occupant to, count)
+  std::vector<int4> P;
+  for (int t = 0; t < n_types; ++t) {
+    const int n_parts = type_begin[t + 1] - type_begin[t];
+    ARG_CHECK(n_parts >= 1 && n_parts <= kMaxMswapParts, "a multiswap type needs 1..4 parts");
+    int sites = 0;
+    for (int q = type_begin[t]; q < type_begin[t + 1]; ++q) {
+      const int4 pt = make_int4(parts[4 * q], parts[4 * q + 1], parts[4 * q + 2], parts[4 * q + 3]);
+      int n_occ_u = -1;
+      for (int b = 0; b < h->n_sublat; ++b)
+        if (h->asym[b] == pt.x && h->n_occ[b] >= 2) n_occ_u = h->n_occ[b];
+      ARG_CHECK(pt.x >= 0 && pt.x < kMaxSublat && n_occ_u >= 2, "multiswap part: no mutable sites in that unit");
+      ARG_CHECK(pt.y >= 0 && pt.z >= 0 && pt.y < n_occ_u && pt.z < n_occ_u && pt.y != pt.z,
+                "multiswap part: occupants out of range or equal");
+      ARG_CHECK(pt.w >= 1, "multiswap part: count < 1");
+      for (int q2 = type_begin[t]; q2 < q; ++q2)
+        ARG_CHECK(!(parts[4 * q2] == pt.x && parts[4 * q2 + 1] == pt.y && parts[4 * q2 + 2] == pt.z),
+                  "multiswap type repeats a part (merge the counts)");
+      sites += pt.w;
+      P.push_back(pt);
+    }
+    ARG_CHECK(sites <= kMaxMswapSites, "a multiswap changes at most 8 sites");
+  }
+  // the type undoing each type: the same parts with from and to exchanged
+  std::vector<int> rev(n_types, -1);
+  for (int t = 0; t < n_types; ++t) {
+    const int nt = type_begin[t + 1] - type_begin[t];
+    for (int r = 0; r < n_types && rev[t] < 0; ++r) {
+      if (type_begin[r + 1] - type_begin[r] != nt) continue;
+      bool all = true;
+      for (int q = type_begin[t]; q < type_begin[t + 1] && all; ++q) {
+        bool found = false;
+        for (int q2 = type_begin[r]; q2 < type_begin[r + 1] && !found; ++q2)
+          found = P[q2].x == P[q].x && P[q2].y == P[q].z && P[q2].z == P[q].y && P[q2].w == P[q].w;
+        all = found;
+      }
+      if (all) rev[t] = r;
+    }
+    ARG_CHECK(rev[t] >= 0, "the multiswap list must hold the type that undoes each type");
+  }
+  free_mswaps(h);
+  HIP_TRY(hipMalloc(&h->d_mswap_parts, sizeof(int4) * P.size()));
+  HIP_TRY(hipMalloc(&h->d_mswap_begin, sizeof(int) * (n_types + 1)));
+  HIP_TRY(hipMalloc(&h->d_mswap_rev, sizeof(int) * n_types));
+  HIP_TRY(hipMemcpy(h->d_mswap_parts, P.data(), sizeof(int4) * P.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_mswap_begin, type_begin, sizeof(int) * (n_types + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_mswap_rev, rev.data(), sizeof(int) * n_types, hipMemcpyHostToDevice));
+  h->n_mswaps = n_types;
+  h->mswap_per_pass = per_pass;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(per_pass >= 0 && per_pass <= 65536, "global swaps per pass out of range");
@@ -1269,6 +1401,7 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
     nb_max = std::max(nb_max, d.nb_max);
   }
   ARG_CHECK(max_sites <= 262144, "the N-fold kernel takes supercells of at most 262,144 sites");
+  ARG_CHECK(h->n_mswaps == 0, "the N-fold kernel's events are single-site changes: clear the multiswaps");
   const size_t lds = nfold_lds_bytes(max_pad, max_sites, nb_max);
   ARG_CHECK(lds <= 160 * 1024, "supercell too large for the N-fold kernel's LDS (image + one rate per site)");
   NfoldArgs na;
@@ -1489,6 +1622,14 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   }
   clexmc_status st = grow_samples(h, (long long)total_passes / sample_period);
   if (st != CLEXMC_OK) return st;
+  // semi-grand canonical runs with multiswaps: the fixed-composition
+  // (canonical) sweep followed by the multiswap attempts of every pass
+  const bool mswap_run = kind == 1 && h->n_mswaps > 0;
+  if (mswap_run && P.learned) {
+    h->err = "multiswaps run with the plain proposal only";
+    return CLEXMC_ERR_UNSUPPORTED;
+  }
+  kind = mswap_run ? 0 : kind;
   P.kind = kind;
   P.lanes = 1;
   const int lanes_req = lanes_override ? lanes_override : h->lanes_per_site;
@@ -1517,6 +1658,11 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
   a.chain_map = nullptr;
   a.slabs = 1;
   a.slab_rec = h->d_slab_rec;
+  a.n_mswaps = mswap_run ? h->n_mswaps : 0;
+  a.mswap_per_pass = h->mswap_per_pass;
+  a.mswap_parts = h->d_mswap_parts;
+  a.mswap_begin = h->d_mswap_begin;
+  a.mswap_rev = h->d_mswap_rev;
   P.n_blocks = n_sel;
   P.la.mlp = h->d_mlp;
   P.la.rounds_per_group = h->rounds_per_group;
@@ -1623,7 +1769,7 @@ static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &p
   for (RunPlan *P : plans) {
     P->slabs = 1;
     P->a.slabs = 1;
-    if (h->slabs == 1 || P->learned || P->lanes != 1 || !P->one_shape) continue;
+    if (h->slabs == 1 || P->learned || P->lanes != 1 || !P->one_shape || P->a.n_mswaps > 0) continue;
     const DevSupercell &d = h->scs[P->sc_id].dev;
     if (d.kreach < 0 || !d.class_kidx) continue;
     int S;
@@ -1729,8 +1875,36 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
   return CLEXMC_OK;
 }
 
+// Multiswap runs: a chain whose pool counts disagreed with its image (a
+// search found no site; never expected) reports an error; flags are cleared.
+static clexmc_status mswap_check_flags(clexmc_handle *h) {
+  std::vector<ChainState> st(h->n_chains);
+  HIP_TRY(hipMemcpy(st.data(), h->d_state, sizeof(ChainState) * h->n_chains, hipMemcpyDeviceToHost));
+  int first = -1;
+  for (int c = 0; c < h->n_chains; ++c)
+    if (st[c].flags & kChainFlagMswapCounts) {
+      if (first < 0) first = c;
+      st[c].flags &= ~kChainFlagMswapCounts;
+    }
+  if (first < 0) return CLEXMC_OK;
+  HIP_TRY(hipMemcpy(h->d_state, st.data(), sizeof(ChainState) * h->n_chains, hipMemcpyHostToDevice));
+  h->err = "multiswap site search found no candidate (chain " + std::to_string(first) +
+           "): occupant counts per asymmetric unit disagree with the occupation";
+  return CLEXMC_ERR_STATE;
+}
+
+static clexmc_status run_passes_impl(clexmc_handle *h, int kind, int n_passes, int sample_period,
+                                     int canonical_rounds);
+
 clexmc_status clexmc_run_passes(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
   if (!h) return CLEXMC_ERR_ARG;
+  const clexmc_status st = run_passes_impl(h, kind, n_passes, sample_period, canonical_rounds);
+  if (st != CLEXMC_OK || kind != 1 || h->n_mswaps == 0 || n_passes == 0) return st;
+  return mswap_check_flags(h);
+}
+
+static clexmc_status run_passes_impl(clexmc_handle *h, int kind, int n_passes, int sample_period,
+                                     int canonical_rounds) {
   ARG_CHECK(kind == 0 || kind == 1, "kind must be 0 (canonical) or 1 (semi-grand canonical)");
   ARG_CHECK(n_passes >= 0 && sample_period >= 1, "bad n_passes / sample_period");
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
@@ -2353,6 +2527,7 @@ void clexmc_destroy(clexmc_handle *h) {
     for (auto m : s.jit_mods) hipModuleUnload(m);
   }
   free_all(h->sys_allocs);
+  free_mswaps(h);
   if (h->d_scs) hipFree(h->d_scs);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);

@@ -27,6 +27,12 @@ constexpr int kMaxFunc = 3;
 constexpr int kMaxOrbits = 64;   // correlations kernel / exact counts storage
 constexpr int kMaxSlots = 8;     // binary path: orbits with >= 1 site (registers)
 constexpr int kMaxSitesPerProp = 4;
+constexpr int kMaxMswapSites = 8;    // sites changed by one multiswap
+constexpr int kMaxMswapParts = 4;    // (unit, from, to, count) parts of one multiswap type
+constexpr int kMaxMswapTypes = 256;
+// LDS of the multiswap phase: pool counts [kMaxSublat][kMaxOcc], 8 control
+// words and the picked sites
+constexpr int kMswapLdsBytes = ((kMaxSublat * kMaxOcc + 8 + kMaxMswapSites) * 4 + 15) & ~15;
 constexpr int kMaxAutoRounds = 1024; // canonical matchings per independent set and pass (automatic)
 constexpr int kAotMaxThreads = 512;  // launch bound of the ahead-of-time Metropolis kernels
 constexpr int kMaxJitThreads = 1024; // launch bound of run-time specialised kernels for large classes
@@ -210,6 +216,10 @@ struct ChainState {
 // N-fold way: the chain's total rate fell below the smallest the kernel
 // accepts (no event can be drawn reliably); the run stopped early.
 constexpr unsigned long long kChainFlagNfoldStalled = 1ull;
+// Multiswaps: a workgroup-wide search found fewer candidate sites than the
+// kernel's pool counts (never expected: the counts follow every move); the
+// attempt was rejected.
+constexpr unsigned long long kChainFlagMswapCounts = 2ull;
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as

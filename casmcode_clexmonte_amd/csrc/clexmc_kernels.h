@@ -34,6 +34,17 @@ struct RunArgs {
   // counters to slab_post_kernel
   int slabs;
   SlabRecord *slab_rec;  // [n_chains][kSlabMax][kSlabRecords]
+  // semi-grand canonical multiswaps (n_mswaps > 0; the sweep is then the
+  // fixed-composition canonical one): after every pass, mswap_per_pass serial
+  // attempts of a multiswap type drawn uniformly; type t has the parts
+  // mswap_parts[mswap_begin[t] .. mswap_begin[t + 1]) = (asymmetric unit,
+  // occupant from, occupant to, count), and mswap_rev[t] is the type that
+  // undoes it
+  int n_mswaps;
+  int mswap_per_pass;
+  const int4 *mswap_parts;
+  const int *mswap_begin;
+  const int *mswap_rev;
 };
 
 struct DeltaArgs {

@@ -431,11 +431,19 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
       __syncthreads();
     }
 
-    // canonical cross-class mixing with the plain pair phases
-    if (KIND == 0 && sc->n_pair_phases > 0) {
-      const uint32_t ph =
-          philox4x32_10(make_uint4(0xFFFFFFFFu, (uint32_t)gpass, 0xFFFFFFF0u, (uint32_t)chain ^ 0x9E3779B9u), k0, k1).x %
-          (uint32_t)sc->n_pair_phases;
+    // canonical cross-class mixing with the plain pair phases: as many per
+    // pass as the plain kernel's automatic count (about n_mutable / 4 pair
+    // attempts), each phase drawn at random
+    const int n_pair_rounds =
+        (KIND == 0 && sc->n_pair_phases > 0)
+            ? auto_pair_rounds(sc->class_begin[n_classes], sc->pair_begin[sc->n_pair_phases], sc->n_pair_phases)
+            : 0;
+    for (int pr = 0; pr < n_pair_rounds; ++pr) {
+      const uint32_t ph = philox4x32_10(make_uint4(0xFFFFFFFFu, (uint32_t)gpass, 0xFFFFFFF0u - (uint32_t)pr,
+                                                   (uint32_t)chain ^ 0x9E3779B9u),
+                                        k0, k1)
+                              .x %
+                          (uint32_t)sc->n_pair_phases;
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
       for (int t = pb + tid; t < pe; t += nthr) {
         const int2 pq = sc->pair_addr[t];

@@ -336,6 +336,11 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   long long *slabI = (long long *)(smem + off_b);        // [kMaxSlots + kMaxOcc + 2]
   double *slabE = (double *)(slabI + kMaxSlots + kMaxOcc + 2);
   off_b += 128;
+  // multiswaps: occupant counts per (asymmetric unit, occupant), the search
+  // request / result words and the sites picked by the current attempt
+  int *poolN = (int *)(smem + off_b);                    // [kMaxSublat][kMaxOcc]
+  int *msw = poolN + kMaxSublat * kMaxOcc;               // [8] control + [kMaxMswapSites] picks
+  off_b += kMswapLdsBytes;
   const int nfs = BIN ? 1 : sys->n_func_max;              // field values kept per class site
   float *cF = (float *)(smem + off_b);                   // [nfs][max_class]
   int8_t *cK = (int8_t *)(cF + nfs * max_class);         // [kMaxSlots][max_class] (binary)
@@ -376,6 +381,16 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     *maxDraw = 0ull;
     for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) slabI[q] = 0;
     *slabE = 0.0;
+  }
+  // (multiswap runs use the fixed-composition sweep: canonical kernels only)
+  const bool mswaps = KIND == 0 && args.n_mswaps > 0 && n_slabs == 1;
+  if (mswaps) {
+    if (tid < kMaxSublat * kMaxOcc) poolN[tid] = 0;
+    __syncthreads();
+    for (int l = tid; l < n_sites; l += nthr) {
+      const int b = l / n_cells;
+      if (sys->n_occ[b] >= 2) atomicAdd(&poolN[sys->asym[b] * kMaxOcc + occG[l]], 1);
+    }
   }
   __syncthreads();
 
@@ -436,6 +451,28 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     int d = k - ka;
     if (d < 0) d += L3;
     return d < U;
+  };
+  // the site at padded main address a and every periodic copy of it in the
+  // halo: old -> nw (serial moves that change sites one at a time)
+  auto set_all = [&](int a, int old, int nw) {
+    const int b = a / S;
+    const int r = a - b * S;
+    const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
+    const int P2 = P12 / P1, P3 = S / P12;
+    const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
+    for (int dk = -m3; dk <= m3; ++dk) {
+      const int qk = pk + dk * L3;
+      if (qk < 0 || qk >= P3) continue;
+      for (int dj = -m2; dj <= m2; ++dj) {
+        const int qj = pj + dj * L2;
+        if (qj < 0 || qj >= P2) continue;
+        for (int di = -m1; di <= m1; ++di) {
+          const int qi = pi + di * L1;
+          if (qi < 0 || qi >= P1) continue;
+          img.set(b * S + qk * P12 + qj * P1 + qi, old, nw);
+        }
+      }
+    }
   };
   const bool auto_rounds = args.rounds <= 0;
   const int rounds = auto_rounds ? 0 : args.rounds;
@@ -717,26 +754,6 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     if (KIND == 0 && args.global_swaps > 0 && n_slabs == 1) {
       if (tid == 0) {
         const int n_var = sc->class_begin[n_classes];
-        auto set_all = [&](int a, int old, int nw) {
-          const int b = a / S;
-          const int r = a - b * S;
-          const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
-          const int P2 = P12 / P1, P3 = S / P12;
-          const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
-          for (int dk = -m3; dk <= m3; ++dk) {
-            const int qk = pk + dk * L3;
-            if (qk < 0 || qk >= P3) continue;
-            for (int dj = -m2; dj <= m2; ++dj) {
-              const int qj = pj + dj * L2;
-              if (qj < 0 || qj >= P2) continue;
-              for (int di = -m1; di <= m1; ++di) {
-                const int qi = pi + di * L1;
-                if (qi < 0 || qi >= P1) continue;
-                img.set(b * S + qk * P12 + qj * P1 + qi, old, nw);
-              }
-            }
-          }
-        };
         for (int g = 0; g < args.global_swaps; ++g) {
           const float u1 = rng.next(), u2 = rng.next();
           const uint32_t u3 = rng.next_u32();
@@ -783,6 +800,246 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
         }
       }
       __syncthreads();
+    }
+
+    // ---- semi-grand canonical multiswaps (SemiGrandCanonicalCalculator.cc:
+    // 104-115 with the system's multiswaps, System_json_io.cc:955-1006).  An
+    // attempt draws a type uniformly and, for each of its parts (unit u,
+    // occupant from -> to, count c), c distinct sites uniformly among the
+    // sites of u holding `from` and not yet picked: by rejection from all
+    // sites of u (up to 32 draws per site), else by a workgroup-wide search
+    // for a uniformly drawn k-th such site (both uniform).  Thread 0 then
+    // changes the sites one after another (each site's field sees the earlier
+    // changes, every periodic copy written) and accepts with
+    // min(1, exp(-beta dOmega) q_rev / q_fwd), where q of a move is
+    // (1 / n_types) prod_parts 1 / C(r, c), r the part's pool size (minus the
+    // sites earlier parts took from the same pool) in the state the move
+    // starts from and the reverse type's parts for q_rev.  A part whose pool
+    // is too small makes the attempt a rejection.
+    if (KIND == 0 && mswaps) {
+      const int n_ty = args.n_mswaps;
+      auto unit_sites = [&](int u) -> int {
+        int m = 0;
+        for (int b = 0; b < sys->n_sublat; ++b) m += (sys->asym[b] == u && sys->n_occ[b] >= 2) ? 1 : 0;
+        return m * n_cells;
+      };
+      auto unit_addr = [&](int u, int j) -> int {
+        int q = j / n_cells;
+        const int cell = j - q * n_cells;
+        int bb = 0;
+        for (int b = 0; b < sys->n_sublat; ++b)
+          if (sys->asym[b] == u && sys->n_occ[b] >= 2) {
+            if (q == 0) {
+              bb = b;
+              break;
+            }
+            --q;
+          }
+        const int i = cell % L1, jj = (cell / L1) % L2, k = cell / (L1 * L2);
+        return bb * S + (k + h) * P12 + (jj + h) * P1 + (i + h);
+      };
+      auto binom = [](int r, int c) -> double {
+        double v = 1.0;
+        for (int t = 0; t < c; ++t) v *= (double)(r - t) / (double)(t + 1);
+        return v;
+      };
+      const int per_pass = args.mswap_per_pass > 0 ? args.mswap_per_pass : sc->class_begin[n_classes];
+      // thread 0's attempt: type, parts cursor, picked sites and target occupants
+      int ty = 0, part = 0, in_part = 0, n_pick = 0;
+      int pick_a[kMaxMswapSites], pick_to[kMaxMswapSites];
+      double fwd = 1.0;
+      bool failed = false;  // a search found no site: the pool counts disagree with the image
+      for (int g = 0; g < per_pass; ++g) {
+        if (tid == 0) {
+          // exactly uniform type: rejection of the top of the 32-bit range
+          const unsigned long long lim = 4294967296ull - (4294967296ull % (unsigned long long)n_ty);
+          uint32_t x = rng.next_u32();
+          for (int t = 0; t < 64 && (unsigned long long)x >= lim; ++t) x = rng.next_u32();
+          ty = (int)(x % (uint32_t)n_ty);
+          part = args.mswap_begin[ty];
+          in_part = 0;
+          n_pick = 0;
+          fwd = 1.0;
+          failed = false;
+          msw[0] = 0;
+        }
+        for (;;) {
+          if (tid == 0) {
+            int state = failed ? 3 : 2;  // 2: all sites picked, 3: impossible, 1: search requested
+            while (!failed && part < args.mswap_begin[ty + 1]) {
+              const int4 pt = args.mswap_parts[part];  // (unit, from, to, count)
+              int taken = 0;
+              for (int q = 0; q < n_pick; ++q)
+                taken += (sys->asym[pick_a[q] / S] == pt.x && img.get(pick_a[q]) == pt.y) ? 1 : 0;
+              const int r = poolN[pt.x * kMaxOcc + pt.y] - taken;
+              if (in_part == 0) {
+                if (r < pt.w) {
+                  state = 3;
+                  break;
+                }
+                fwd *= binom(r, pt.w);
+              }
+              if (in_part == pt.w) {
+                ++part;
+                in_part = 0;
+                continue;
+              }
+              const int nu = unit_sites(pt.x);
+              int a = -1;
+              for (int t = 0; t < 32 && a < 0; ++t) {
+                const int j = (int)(((unsigned long long)rng.next_u32() * (unsigned long long)nu) >> 32);
+                const int aj = unit_addr(pt.x, j);
+                bool fresh = img.get(aj) == pt.y;
+                for (int q = 0; q < n_pick; ++q) fresh = fresh && pick_a[q] != aj;
+                if (fresh) a = aj;
+              }
+              if (a < 0) {
+                // uniform k among the r remaining candidates: exact integer draw
+                const unsigned long long lim = 4294967296ull - (4294967296ull % (unsigned long long)r);
+                uint32_t x = rng.next_u32();
+                for (int t = 0; t < 64 && (unsigned long long)x >= lim; ++t) x = rng.next_u32();
+                msw[1] = pt.x;
+                msw[2] = pt.y;
+                msw[3] = (int)(x % (uint32_t)r);
+                msw[4] = -1;
+                state = 1;
+                break;
+              }
+              pick_a[n_pick] = a;
+              pick_to[n_pick] = pt.z;
+              ++n_pick;
+              ++in_part;
+            }
+            msw[0] = state;
+            msw[5] = n_pick;
+            for (int q = 0; q < n_pick; ++q) msw[8 + q] = pick_a[q];
+          }
+          __syncthreads();
+          const int state = msw[0];
+          if (state != 1) break;
+          // workgroup-wide search: the k-th site (in unit order) of unit u
+          // holding `from` that is not picked yet
+          {
+            const int u = msw[1], of = msw[2], kk = msw[3], np = msw[5];
+            const int nu = unit_sites(u);
+            const int chunk = (nu + nthr - 1) / nthr;
+            const int j0 = min(tid * chunk, nu), j1 = min(j0 + chunk, nu);
+            auto cand = [&](int j) -> int {
+              const int aj = unit_addr(u, j);
+              if (img.get(aj) != of) return -1;
+              for (int q = 0; q < np; ++q)
+                if (msw[8 + q] == aj) return -1;
+              return aj;
+            };
+            int cnt = 0;
+            for (int j = j0; j < j1; ++j) cnt += cand(j) >= 0 ? 1 : 0;
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+              const int y = __shfl_up(incl, o, 64);
+              if (lane >= o) incl += y;
+            }
+            if (lane == 63) redI[wave] = incl;
+            __syncthreads();
+            int before = incl - cnt;
+            for (int w = 0; w < wave; ++w) before += redI[w];
+            if (kk >= before && kk < before + cnt) {
+              int left = kk - before;
+              for (int j = j0; j < j1; ++j) {
+                const int aj = cand(j);
+                if (aj >= 0) {
+                  if (left == 0) {
+                    msw[4] = aj;
+                    break;
+                  }
+                  --left;
+                }
+              }
+            }
+            __syncthreads();
+          }
+          if (tid == 0) {
+            if (msw[4] < 0) {
+              failed = true;
+              cs->flags |= kChainFlagMswapCounts;
+            } else {
+              const int4 pt = args.mswap_parts[part];
+              pick_a[n_pick] = msw[4];
+              pick_to[n_pick] = pt.z;
+              ++n_pick;
+              ++in_part;
+            }
+          }
+        }
+        if (tid == 0) {
+          accAtt += 1;
+          if (msw[0] == 2) {
+            // sequential changes and the semi-grand energy change
+            float dEf = 0.0f, dchem = 0.0f;
+            int tC[kMaxSlots];
+#pragma unroll
+            for (int a = 0; a < kMaxSlots; ++a) tC[a] = 0;
+            int pick_from[kMaxMswapSites];
+            for (int q = 0; q < n_pick; ++q) {
+              const int a = pick_a[q], b = a / S, o = img.get(a), nw = pick_to[q];
+              pick_from[q] = o;
+              if (BIN) {
+                int K[kMaxSlots];
+                const float F = FIELD::bin(fctx, img, b, a, o, K);
+                dEf += (o ? -2.0f : 2.0f) * F;
+                const int sk = o ? -2 : 2;
+#pragma unroll
+                for (int t = 0; t < kMaxSlots; ++t) tC[t] += sk * K[t];
+              } else {
+                float G[kMaxFunc];
+                FIELD::gen(fctx, img, b, a, tabL, G);
+                dEf += general_dE(sys, b, o, nw, G);
+              }
+              dchem += cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o];
+              set_all(a, o, nw);
+            }
+            // reverse move's pool sizes in the new state
+            double rev = 1.0;
+            bool ok = true;
+            const int rt = args.mswap_rev[ty];
+            for (int p2 = args.mswap_begin[rt]; p2 < args.mswap_begin[rt + 1]; ++p2) {
+              const int4 pt = args.mswap_parts[p2];
+              int ny = poolN[pt.x * kMaxOcc + pt.y];
+              for (int p1 = args.mswap_begin[ty]; p1 < args.mswap_begin[ty + 1]; ++p1) {
+                const int4 f = args.mswap_parts[p1];
+                if (f.x == pt.x) ny += (f.z == pt.y ? f.w : 0) - (f.y == pt.y ? f.w : 0);
+              }
+              for (int p3 = args.mswap_begin[rt]; p3 < p2; ++p3) {
+                const int4 e = args.mswap_parts[p3];
+                if (e.x == pt.x && e.y == pt.y) ny -= e.w;
+              }
+              if (ny < pt.w) ok = false;
+              else rev *= binom(ny, pt.w);
+            }
+            const double pacc = ok ? (double)__expf(-beta * (dEf - dchem)) * fwd / rev : 0.0;
+            const bool acc = accept_prob((float)fmin(pacc, 2.0), rng.next_u32(), [&]() { return rng.next_u32(); });
+            if (acc) {
+              accAcc += 1;
+              if (BIN) {
+#pragma unroll
+                for (int t = 0; t < kMaxSlots; ++t) accC[t] += tC[t];
+              } else {
+                accE += (double)dEf;
+              }
+              for (int q = 0; q < n_pick; ++q) {
+                const int u = sys->asym[pick_a[q] / S];
+                poolN[u * kMaxOcc + pick_from[q]] -= 1;
+                poolN[u * kMaxOcc + pick_to[q]] += 1;
+#pragma unroll
+                for (int o = 0; o < kMaxOcc; ++o) accN[o] += (o == pick_to[q]) - (o == pick_from[q]);
+              }
+            } else {
+              for (int q = n_pick - 1; q >= 0; --q) set_all(pick_a[q], pick_to[q], pick_from[q]);
+            }
+          }
+        }
+        __syncthreads();
+      }
     }
 
     // ---- reduction of lane accumulators and sampling

@@ -435,8 +435,14 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
   // "corr.<key>" sampler (sampling_functions.cc:121-135): launches end at the
   // sampled passes (g + 1 a multiple of the period, g counted from the load)
   // and every chain's FP64 correlations per unit cell are read there
-  const bool want_corr = std::find(P.sampler_names.begin(), P.sampler_names.end(), "corr.formation_energy") !=
-                         P.sampler_names.end();
+  auto wants = [&](const char *name) {
+    return std::find(P.sampler_names.begin(), P.sampler_names.end(), name) != P.sampler_names.end();
+  };
+  // "corr.<key>" (all correlations) and "clex.<key>.sparse_corr" (those of the
+  // non-zero ECIs, sampling_functions.cc:185-219) read the correlations
+  const bool want_all_corr = wants("corr.formation_energy");
+  const bool want_sparse = wants("clex.formation_energy.sparse_corr");
+  const bool want_corr = want_all_corr || want_sparse;
   std::vector<std::vector<std::vector<double>>> corr(n);
   if (!want_corr) {
     check(h, clexmc_run_passes(h, kind, (int)P.n_passes, P.sample_period, P.canonical_rounds), calculator_name + "::run");
@@ -508,7 +514,16 @@ void BaseMonteCalculator::_run_chains(int kind, std::vector<State *> const &stat
     }
     if (want_corr) {
       if ((int)corr[c].size() < ns) throw std::runtime_error("Error in " + calculator_name + "::run: corr samples");
-      R.samplers["corr.formation_energy"].assign(corr[c].begin(), corr[c].begin() + ns);
+      if (want_all_corr) R.samplers["corr.formation_energy"].assign(corr[c].begin(), corr[c].begin() + ns);
+      if (want_sparse) {
+        auto &S = R.samplers["clex.formation_energy.sparse_corr"];
+        for (int q = 0; q < ns; ++q) {
+          std::vector<double> row;
+          for (int i = 0; i < sys.n_corr; ++i)
+            if (sys.eci[i] != 0.0) row.push_back(corr[c][q][i]);
+          S.push_back(std::move(row));
+        }
+      }
     }
     // statistics (mean, autocorrelation-corrected standard error)
     for (auto const &kv : R.samplers) {
@@ -564,7 +579,8 @@ std::shared_ptr<MonteCalculator> make_monte_calculator(std::map<std::string, std
 // GpuCanonicalCalculator
 // ---------------------------------------------------------------------------
 std::vector<std::string> GpuCanonicalCalculator::standard_sampling_function_names() const {
-  return {"temperature", "mol_composition", "param_composition", "clex.formation_energy", "potential_energy"};
+  return {"temperature",           "mol_composition", "param_composition", "clex.formation_energy",
+          "clex.formation_energy.sparse_corr", "corr.formation_energy", "potential_energy"};
 }
 std::vector<std::string> GpuCanonicalCalculator::standard_analysis_function_names() const { return {"heat_capacity"}; }
 
@@ -672,8 +688,9 @@ void GpuCanonicalCalculator::run(int current_state, std::vector<State> &states, 
 // GpuSemiGrandCanonicalCalculator
 // ---------------------------------------------------------------------------
 std::vector<std::string> GpuSemiGrandCanonicalCalculator::standard_sampling_function_names() const {
-  return {"temperature", "mol_composition", "param_composition", "param_chem_pot", "clex.formation_energy",
-          "potential_energy"};
+  return {"temperature",           "mol_composition",       "param_composition",
+          "param_chem_pot",        "clex.formation_energy", "clex.formation_energy.sparse_corr",
+          "corr.formation_energy", "potential_energy"};
 }
 std::vector<std::string> GpuSemiGrandCanonicalCalculator::standard_analysis_function_names() const {
   return {"heat_capacity", "mol_susc", "param_susc", "mol_thermochem_susc", "param_thermochem_susc"};
@@ -780,8 +797,9 @@ void BaseMonteCalculator::_run_nfold(std::vector<State *> const &states, std::ve
     }
   }
   if (bvar < 0) throw std::runtime_error("Error in " + where + ": no variable sites");
-  if (std::find(P.sampler_names.begin(), P.sampler_names.end(), "corr.formation_energy") != P.sampler_names.end())
-    throw std::runtime_error("Error in " + where + ": nfold runs sample energies and compositions only");
+  for (const char *name : {"corr.formation_energy", "clex.formation_energy.sparse_corr"})
+    if (std::find(P.sampler_names.begin(), P.sampler_names.end(), name) != P.sampler_names.end())
+      throw std::runtime_error("Error in " + where + ": nfold runs sample energies and compositions only");
   for (auto *st : states)
     if (!std::equal(st->configuration.L, st->configuration.L + 3, states[0]->configuration.L))
       throw std::runtime_error("Error in " + where + ": nfold states must share one supercell");

@@ -66,7 +66,7 @@ def lib():
         L.clexmc_version.restype = C.c_char_p
         L.clexmc_create.argtypes = [C.POINTER(SystemDesc), C.c_int, C.POINTER(C.c_void_p)]
         for name in ["clexmc_add_supercell", "clexmc_load_chains", "clexmc_delta_energy_batch",
-                     "clexmc_delta_energy_batch_noalloc",
+                     "clexmc_delta_energy_batch_noalloc", "clexmc_set_multiswaps",
                      "clexmc_correlations", "clexmc_run_passes", "clexmc_read_samples",
                      "clexmc_clear_samples", "clexmc_read_state", "clexmc_read_binary_counts",
                      "clexmc_last_run_ms", "clexmc_set_block_size", "clexmc_philox_device",
@@ -88,7 +88,8 @@ def lib():
 def exported_symbols() -> List[str]:
     """Names of the C-ABI entry points declared in include/clexmc.h."""
     return ["clexmc_create", "clexmc_add_supercell", "clexmc_load_chains",
-            "clexmc_delta_energy_batch", "clexmc_delta_energy_batch_noalloc", "clexmc_correlations",
+            "clexmc_delta_energy_batch", "clexmc_delta_energy_batch_noalloc", "clexmc_set_multiswaps",
+            "clexmc_correlations",
             "clexmc_run_passes",
             "clexmc_read_samples", "clexmc_clear_samples", "clexmc_read_state",
             "clexmc_read_binary_counts", "clexmc_last_run_ms", "clexmc_set_block_size",
@@ -495,6 +496,23 @@ class Engine:
         v = C.c_int()
         self._check(lib().clexmc_last_run_slabs(self.h, C.byref(v)), "clexmc_last_run_slabs")
         return v.value
+
+    def set_multiswaps(self, types, per_pass: int = 0):
+        """Semi-grand canonical multiswaps: `types` is a list of multiswap
+        types, each a list of parts (asymmetric unit, occupant from, occupant
+        to, count); the list must hold the type undoing each type
+        (make_multiswaps builds such lists).  An empty list restores
+        single-site changes.  per_pass: attempts per pass (0: one per
+        mutable site)."""
+        types = [list(t) for t in types]
+        begin = np.zeros(len(types) + 1, dtype=np.int32)
+        for i, t in enumerate(types):
+            begin[i + 1] = begin[i] + len(t)
+        parts = np.array([list(p) for t in types for p in t], dtype=np.int32).reshape(-1, 4)
+        if parts.size == 0:
+            parts = np.zeros((1, 4), dtype=np.int32)
+        self._check(lib().clexmc_set_multiswaps(self.h, C.c_int(len(types)), _p(begin, i32p), _p(parts, i32p),
+                                                C.c_int(per_pass)), "clexmc_set_multiswaps")
 
     def set_global_swaps(self, per_pass: int):
         self._check(lib().clexmc_set_global_swaps(self.h, C.c_int(per_pass)), "clexmc_set_global_swaps")

@@ -823,53 +823,109 @@ def interaction_offsets(flat: FlatCE) -> Dict[int, List[Tuple[int, int, int, int
     return {b: sorted(v) for b, v in out.items()}
 
 
-def _conflict_free_moduli(flat: FlatCE, dims, offs) -> Optional[Tuple[int, int, int]]:
-    """Smallest diagonal moduli (m1,m2,m3) dividing dims such that no
-    interaction offset connects two sites with equal (b, i mod m1, j mod m2,
-    k mod m3) in the periodic supercell."""
-    best = None
-    divs = [[m for m in range(1, d + 1) if d % m == 0] for d in dims]
-    for m in itertools.product(*divs):
-        ok = True
-        for b, lst in offs.items():
-            for (b2, di, dj, dk) in lst:
-                if b2 != b:
+# largest independent set (sites per class) the periodic colourings may use:
+# canonical sweeps keep one or more FP32 fields per class site in LDS
+MAX_CLASS_SITES = 4096
+
+
+def nn_parities(flat: FlatCE) -> set:
+    """Parities (di, dj, dk) mod 2 of the cell offsets of same-sublattice
+    neighbours in the shortest pair orbit that has them: the translations
+    across which nearest-neighbour-driven orderings put unlike occupants."""
+    for a in range(flat.n_orbits):
+        if flat.orbit_k[a] != 2:
+            continue
+        par = set()
+        for b in range(flat.n_sublat):
+            for e in range(flat.flower_begin[b], flat.flower_begin[b + 1]):
+                if flat.entry_orbit[e] != a:
                     continue
-                if di % m[0] == 0 and dj % m[1] == 0 and dk % m[2] == 0:
-                    ok = False
-                    break
-            if not ok:
-                break
-        if ok:
-            n = m[0] * m[1] * m[2]
-            if best is None or n < best[0] or (n == best[0] and m < best[1]):
-                best = (n, m)
+                q = flat.entry_site_begin[e]
+                b2, di, dj, dk = (int(x) for x in flat.entry_sites[q])
+                if b2 == b:
+                    par.add((di % 2, dj % 2, dk % 2))
+        if par:
+            return par
+    return set()
+
+
+def periodic_colouring(flat: FlatCE, dims, offs=None):
+    """The periodic colouring of the cells used for the independent sets:
+    ("residue", (m1, m2, m3)), class = (i mod m1, j mod m2, k mod m3), or
+    ("linear", (M, a, c)), class = (i + a j + c k) mod M; None if neither is
+    conflict-free (no interaction offset within one sublattice joins two
+    cells of a class).  Among the conflict-free ones with at most
+    MAX_CLASS_SITES cells per class it prefers the colouring whose class
+    translations hold the most parities of nearest-neighbour offsets (so a
+    period-2 ordering driven by nearest-neighbour pairs, such as L1_2 or L1_0
+    on FCC or B2 on BCC, does not make every class pure), then the fewest
+    classes.  Residue colourings with odd moduli hold every parity; for
+    example FCC 24^3 keeps 3 x 3 x 3 residues, FCC 32^3 takes
+    (i + 3 j + 7 k) mod 16, whose classes hold every L1_2 sublattice."""
+    if offs is None:
+        offs = interaction_offsets(flat)
+    same = [o[1:] for b, lst in offs.items() for o in lst if o[0] == b]
+    L1, L2, L3 = (int(x) for x in dims)
+    n_cells = L1 * L2 * L3
+    P = nn_parities(flat)
+    best = None
+
+    def consider(key, n_classes, score):
+        nonlocal best
+        if n_cells // n_classes > MAX_CLASS_SITES:
+            return
+        rank = (-score, n_classes)
+        if best is None or rank < best[0]:
+            best = (rank, key)
+
+    divs = [[m for m in range(1, d + 1) if d % m == 0] for d in (L1, L2, L3)]
+    for m in itertools.product(*divs):
+        if all(d[0] % m[0] or d[1] % m[1] or d[2] % m[2] for d in same):
+            odd = [x % 2 == 1 for x in m]
+            score = sum(1 for v in P if all(odd[t] or v[t] == 0 for t in range(3)))
+            consider(("residue", tuple(m)), m[0] * m[1] * m[2], score)
+    for M in range(2, L1 + 1):
+        if L1 % M:
+            continue
+        for a in range(M):
+            if (a * L2) % M:
+                continue
+            for c in range(M):
+                if (c * L3) % M:
+                    continue
+                if all((d[0] + a * d[1] + c * d[2]) % M for d in same):
+                    score = sum(1 for v in P if M % 2 == 1 or (v[0] + a * v[1] + c * v[2]) % 2 == 0)
+                    consider(("linear", (M, a, c)), M, score)
     return best[1] if best else None
 
 
 def color_classes(flat: FlatCE, sc: Supercell) -> List[np.ndarray]:
     """Partition of the variable sites into independent sets (no cluster
-    contains two sites of one class).  A periodic colouring by residues of
-    the cell coordinates is used when one exists (equal class sizes); the
-    fallback is a greedy colouring of the conflict graph.  Each class holds
-    sites of a single sublattice."""
+    contains two sites of one class): the periodic colouring of
+    periodic_colouring when one exists (equal class sizes, each class a
+    coset of a lattice of cell translations), else a greedy colouring of the
+    conflict graph.  Each class holds sites of a single sublattice, in
+    linear-index order."""
     prim = sc.prim
     offs = interaction_offsets(flat)
-    m = _conflict_free_moduli(flat, sc.dims, offs)
+    col = periodic_colouring(flat, sc.dims, offs)
     classes: List[np.ndarray] = []
-    if m is not None:
+    if col is not None:
+        L1, L2, L3 = sc.dims
+        cells = np.arange(sc.n_cells)
+        i, j, k = cells % L1, (cells // L1) % L2, cells // (L1 * L2)
+        kind, p = col
+        if kind == "residue":
+            lab = (i % p[0]) + p[0] * ((j % p[1]) + p[1] * (k % p[2]))
+            n_lab = p[0] * p[1] * p[2]
+        else:
+            lab = (i + p[1] * j + p[2] * k) % p[0]
+            n_lab = p[0]
         for b in range(prim.n_sublat):
             if not prim.variable(b):
                 continue
-            for r3 in range(m[2]):
-                for r2 in range(m[1]):
-                    for r1 in range(m[0]):
-                        sites = []
-                        for k in range(r3, sc.dims[2], m[2]):
-                            for j in range(r2, sc.dims[1], m[1]):
-                                for i in range(r1, sc.dims[0], m[0]):
-                                    sites.append(sc.linear_index(b, i, j, k))
-                        classes.append(np.array(sites, dtype=np.int32))
+            for r in range(n_lab):
+                classes.append((b * sc.n_cells + np.flatnonzero(lab == r)).astype(np.int32))
         return classes
     # greedy colouring
     N = sc.n_sites
@@ -1008,6 +1064,42 @@ class CompositionAxes:
             for o, c in enumerate(self.occ_component[b]):
                 w[b, o] = wc[c]
         return w
+
+
+def semigrand_canonical_swaps(flat: FlatCE) -> List[Tuple[int, int, int]]:
+    """Single semi-grand canonical changes (asymmetric unit, occupant from,
+    occupant to) of every unit with mutable sites, as the system's default
+    semigrand_canonical_swaps (src/casm/clexmonte/system/System.cc:57-58)."""
+    out = []
+    for u in sorted({int(flat.asym[b]) for b in range(flat.n_sublat) if flat.n_occ[b] >= 2}):
+        n = max(int(flat.n_occ[b]) for b in range(flat.n_sublat) if flat.asym[b] == u)
+        out += [(u, a, c) for a in range(n) for c in range(n) if a != c]
+    return out
+
+
+def make_multiswaps(flat: FlatCE, max_total_count: int, keep=None, single_swaps=None) -> List[List[Tuple]]:
+    """Multiswap types from single semi-grand canonical swaps: every multiset
+    of 2 .. max_total_count single swaps (the system's integer
+    "semigrand_canonical_multiswaps" input, System_json_io.cc:974-983),
+    each a list of parts (unit, from, to, count).  keep(type) -> bool, if
+    given, filters the types (for example charge neutrality); a kept type's
+    reverse must be kept too.  Multisets that change one site twice
+    (a part and its reverse: a unit's two different occupants exchanged) are
+    kept: they are swaps."""
+    import itertools
+    singles = single_swaps if single_swaps is not None else semigrand_canonical_swaps(flat)
+    types = []
+    for n in range(2, max_total_count + 1):
+        for combo in itertools.combinations_with_replacement(range(len(singles)), n):
+            cnt: Dict[int, int] = {}
+            for i in combo:
+                cnt[i] = cnt.get(i, 0) + 1
+            if len(cnt) > 4 or n > 8:
+                continue
+            t = [(singles[i][0], singles[i][1], singles[i][2], c) for i, c in sorted(cnt.items())]
+            if keep is None or keep(t):
+                types.append(t)
+    return types
 
 
 def save_flat(flat: FlatCE, path: str) -> None:

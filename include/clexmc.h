@@ -269,6 +269,30 @@ clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs);
  * supercell ergodic, including ones too small for non-interacting pairs. */
 clexmc_status clexmc_set_global_swaps(clexmc_handle *h, int per_pass);
 
+/* Semi-grand canonical multiswaps: the reference proposes only multiswaps
+ * when its system's single semi-grand canonical swaps are empty
+ * (SemiGrandCanonicalCalculator.cc:104-115; the system's
+ * "semigrand_canonical_multiswaps", System_json_io.cc:955-1006), for
+ * example charge-neutral combinations of single-site changes.
+ * n_types types (at most 256); type t has the parts
+ * parts[type_begin[t] .. type_begin[t+1])[4] = (asymmetric unit, occupant
+ * from, occupant to, count): 1..4 parts and at most 8 sites per type, no
+ * part repeated; the list must hold, for every type, the type that undoes
+ * it (the same parts with from and to exchanged).  per_pass: multiswap
+ * attempts per pass (0: one per mutable site, the reference's pass).
+ * With n_types > 0, semi-grand canonical runs (clexmc_run_passes kind 1,
+ * graph captures) change composition only through multiswaps: each pass is
+ * the fixed-composition canonical sweep (matchings, pair phases, random-pair
+ * swaps) followed by per_pass serial attempts.  An attempt draws a type
+ * uniformly and, per part, `count` distinct sites uniformly among the sites
+ * of the unit holding `from`; it is accepted with
+ * min(1, exp(-beta dOmega) q_reverse / q_forward), q the proposal
+ * probability of the move and of the move that undoes it (exact
+ * Metropolis-Hastings; dOmega with sequential site changes).  n_types == 0
+ * restores single-site changes.  The N-fold method refuses multiswaps. */
+clexmc_status clexmc_set_multiswaps(clexmc_handle *h, int n_types, const int *type_begin, const int *parts,
+                                    int per_pass);
+
 /* Row pitch of the padded LDS occupation image of supercells added after
  * this call: -1 (default) chooses, per supercell, the pitch L1 + 2h + x
  * (x <= max(4, (L1 + 2h)/4), image growth <= 25 %, images up to 64 KiB)

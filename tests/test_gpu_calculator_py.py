@@ -206,6 +206,30 @@ def test_python_corr_sampler(system, method):
         assert float(eci @ mc) == pytest.approx(e, rel=1e-9, abs=1e-12)
 
 
+@pytest.mark.gpu
+def test_python_sparse_corr_sampler():
+    """clex.formation_energy.sparse_corr (sampling_functions.cc:185-219): the
+    correlations of the non-zero ECIs only.  The ordering CE has zero empty
+    and point ECIs, so the sampler holds the 6 pair and triplet correlations,
+    named by their indices, and the non-zero ECIs dotted with their means
+    equal the mean formation energy (1e-9)."""
+    print("[case] python calculator sparse_corr sampler", flush=True)
+    from casmcode_clexmonte_amd.workloads import fcc_ordering_ce
+    sysm = System(fcc_ordering_ce())
+    calc = MonteCalculator("canonical", sysm)
+    st = MonteCarloState((6, 6, 6), random_occupation(216, 0.25, 3), {"temperature": 600.0})
+    p = calc.make_default_sampling_fixture_params(n_equilibration_passes=20, n_passes=40, sample_period=2)
+    p.sampler_names = p.sampler_names + ["clex.formation_energy.sparse_corr"]
+    o = calc.run_fixture(st, p)
+    eci = np.asarray(sysm.flat.eci)
+    nz = np.flatnonzero(eci)
+    d = o["statistics"]["clex.formation_energy.sparse_corr"]
+    assert d["component_names"] == [str(i) for i in nz] and d["shape"] == [len(nz)] and len(nz) == 6
+    mc = np.array([d[k]["mean"][0] for k in d["component_names"]])
+    e = o["statistics"]["clex.formation_energy"]["value"]["mean"][0]
+    assert float(eci[nz] @ mc) == pytest.approx(e, rel=1e-9, abs=1e-12)
+
+
 def test_nfold_refuses_corr_sampler(system):
     """The N-fold method's block sums hold energies and compositions only: a
     fixture asking for correlations is refused before any device work."""

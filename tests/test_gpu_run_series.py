@@ -12,7 +12,16 @@ confidence, checked every 10 passes) -> summary.json and completed_runs.json.
   entries; a third call runs nothing;
 * completion: every run stops at a check point (a multiple of 10 passes) and
   at most at the 100-pass cutoff; a run reported converged has a calculated
-  precision of at most 0.001.
+  precision of at most 0.001;
+* results_io (write_observations, write_trajectory) with the
+  clex.formation_energy.sparse_corr sampler: <label>/run.<i>/observations.json
+  and trajectory.json per run (the files the reference's full-run tests
+  expect, tests/unit/clexmonte/canonical_fullrun_test.cpp:348-357), one row
+  per sample of every sampler; the sparse correlations are named by the ECI
+  file's linear function indices of the non-zero ECIs, and the file ECIs
+  dotted with them give each sample's formation energy (1e-9); every
+  trajectory configuration's oracle energy equals its sample's formation
+  energy within the FP32 accumulation bound of the table path.
 """
 import copy
 import json
@@ -21,6 +30,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle as O
 from casmcode_clexmonte_amd.calculator import MonteCalculator, System, run_series
 from casmcode_clexmonte_amd.casm_io import clex_from_casm
 
@@ -126,3 +136,39 @@ def test_canonical_series_files(zro, tmp_path):
     np.testing.assert_allclose(d["conditions"]["mol_composition"]["O"], [22 / 216] * 11)
     np.testing.assert_allclose(d["conditions"]["temperature"]["value"], 300.0 + 10 * np.arange(11))
     _check_completion(d)
+
+
+def test_series_observations_trajectory_and_sparse_corr(zro, tmp_path):
+    _print_case("ZrO series: observations.json, trajectory.json, sparse_corr")
+    rp = copy.deepcopy(ZRO["run_params_sgc_complete"])
+    rp["state_generation"]["kwargs"]["n_states"] = 3
+    fx = rp["sampling_fixtures"]["thermo"]
+    fx["sampling"]["quantities"] = fx["sampling"]["quantities"] + ["clex.formation_energy.sparse_corr"]
+    fx["results_io"]["kwargs"].update({"write_observations": True, "write_trajectory": True})
+    out = str(tmp_path)
+    run_series(MonteCalculator("semigrand_canonical", zro, params={"seed": 5}), rp, out)
+    summ = json.load(open(os.path.join(out, "thermo", "summary.json")))
+    n_samples = summ["completion_check_results"]["N_samples"]
+    eci = {cf["linear_function_index"]: cf.get("eci", 0.0)
+           for orb in ZRO["formation_energy_eci"]["orbits"] for cf in orb["cluster_functions"]}
+    nonzero = sorted(k for k, v in eci.items() if v != 0.0)
+    sp = summ["statistics"]["clex.formation_energy.sparse_corr"]
+    assert sp["component_names"] == [str(k) for k in nonzero]
+    osys = O.OracleSystem(zro.flat, (6, 6, 6))
+    for i in range(3):
+        run_dir = os.path.join(out, "thermo", f"run.{i + 1}")
+        assert sorted(os.listdir(run_dir)) == ["observations.json", "trajectory.json"]
+        obs = json.load(open(os.path.join(run_dir, "observations.json")))
+        traj = json.load(open(os.path.join(run_dir, "trajectory.json")))
+        ns = n_samples[i]
+        assert len(obs["count"]) == ns and len(traj) == ns and obs["count"] == [t["count"] for t in traj]
+        for k in ("clex.formation_energy", "potential_energy", "mol_composition", "param_composition",
+                  "temperature", "param_chem_pot", "corr.formation_energy", "clex.formation_energy.sparse_corr"):
+            assert len(obs[k]["value"]) == ns, k
+        e = np.array(obs["clex.formation_energy"]["value"])[:, 0]
+        spc = np.array(obs["clex.formation_energy.sparse_corr"]["value"])
+        assert obs["clex.formation_energy.sparse_corr"]["component_names"] == [str(k) for k in nonzero]
+        np.testing.assert_allclose(spc @ np.array([eci[k] for k in nonzero]), e, rtol=1e-9, atol=1e-12)
+        for t, ei in zip(traj[::10], e[::10]):
+            occ = np.array(t["dof"]["occ"], dtype=np.int8)
+            assert abs(osys.energy(occ) / 216 - ei) <= 1e-5, (osys.energy(occ) / 216, ei)

@@ -376,6 +376,56 @@ def test_l12_correlations_closed_form():
         assert abs(corr[f] - v) <= 1e-12, (f, corr[f], v)
 
 
+def _b2_occupation(sc):
+    """B2 on BCC (prim vectors (-1,1,1)/2, (1,-1,1)/2, (1,1,-1)/2): the cube
+    corners and the body centres, i.e. the parity of i + j + k."""
+    c = np.arange(sc.n_cells)
+    L1, L2 = sc.dims[0], sc.dims[1]
+    i, j, k = c % L1, (c // L1) % L2, c // (L1 * L2)
+    return ((i + j + k) % 2).astype(np.int8)
+
+
+@pytest.mark.parametrize("case", ["fcc_ordering_16", "fcc_ordering_32", "bcc_quaternary_64", "hcp_ternary_16x16x10"])
+def test_periodic_colourings_independent_and_mixing(case):
+    """The periodic colourings: every class is an independent set (no
+    interaction offset joins two of its sites), the classes cover the
+    variable sites, and on FCC 16^3 / 32^3 (powers of two: no odd residue
+    colouring) every class holds a quarter B in perfect L1_2 and on BCC 64^3 every
+    class holds half of each B2 sublattice, so canonical swaps inside a class
+    can create antisites in these ordered states."""
+    from casmcode_clexmonte_amd.system import periodic_colouring
+    from casmcode_clexmonte_amd.workloads import (bcc_quaternary_ce, fcc_ordering_ce, hcp_ternary_ce,
+                                                  l12_occupation)
+    make, dims = {"fcc_ordering_16": (fcc_ordering_ce, (16, 16, 16)), "fcc_ordering_32": (fcc_ordering_ce, (32, 32, 32)),
+                  "bcc_quaternary_64": (bcc_quaternary_ce, (64, 64, 64)),
+                  "hcp_ternary_16x16x10": (hcp_ternary_ce, (16, 16, 10))}[case]
+    ce = make()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, dims)
+    classes = color_classes(flat, sc)
+    lab = -np.ones(sc.n_sites, dtype=np.int64)
+    for q, c in enumerate(classes):
+        assert np.all(lab[c] < 0) and np.all(np.diff(c) > 0)
+        lab[c] = q
+    var = np.concatenate([np.arange(b * sc.n_cells, (b + 1) * sc.n_cells) for b in range(ce.prim.n_sublat)
+                          if ce.prim.variable(b)])
+    assert np.all(lab[var] >= 0)
+    L1, L2, L3 = sc.dims
+    cells = np.arange(sc.n_cells)
+    i, j, k = cells % L1, (cells // L1) % L2, cells // (L1 * L2)
+    for b, lst in interaction_offsets(flat).items():
+        for (b2, di, dj, dk) in lst:
+            t = b2 * sc.n_cells + ((i + di) % L1) + L1 * (((j + dj) % L2) + L2 * ((k + dk) % L3))
+            assert not np.any(lab[b * sc.n_cells + cells] == lab[t]), (b, b2, di, dj, dk)
+    kind, _ = periodic_colouring(flat, sc.dims)
+    if case.startswith("fcc"):
+        occ = l12_occupation(sc)
+        assert all(abs(occ[c].mean() - 0.25) < 1e-12 for c in classes)
+    elif case.startswith("bcc"):
+        occ = _b2_occupation(sc)
+        assert all(abs(occ[c].mean() - 0.5) < 1e-12 for c in classes)
+
+
 def test_oracle_canonical_conserves_composition(fcc):
     osys = O.OracleSystem(fcc, (6, 6, 6))
     occ = random_occupation(osys.n_sites, 0.25, 3)

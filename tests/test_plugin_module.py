@@ -215,6 +215,17 @@ def test_plugin_corr_sampler(flat, tmp_path, method):
     assert corr.shape == (20, flat.n_corr) and np.allclose(corr[:, 0], 1.0, atol=1e-12)
     np.testing.assert_allclose(corr @ np.asarray(flat.eci), e, rtol=1e-9, atol=1e-12)
     assert "corr.formation_energy(1)" in res.statistics
+    assert "clex.formation_energy.sparse_corr" in P._FACTORIES[method]().standard_sampling_function_names()
+    # the sparse sampler: the correlations of the non-zero ECIs (all of them here)
+    params.sampler_names = params.sampler_names + ["clex.formation_energy.sparse_corr"]
+    rm.params = params
+    calc.run(state, P.OccLocation(), rm)
+    res = rm.results
+    nz = np.flatnonzero(np.asarray(flat.eci))
+    sp = np.array(res.samplers["clex.formation_energy.sparse_corr"])
+    assert sp.shape == (20, len(nz))
+    np.testing.assert_allclose(sp @ np.asarray(flat.eci)[nz], np.array(res.samplers["clex.formation_energy"])[:, 0],
+                               rtol=1e-9, atol=1e-12)
     if method == "semigrand_canonical":  # the N-fold method refuses the correlation sampler
         nf = P.MonteCalculator("nfold", system)
         with pytest.raises(RuntimeError, match="energies and compositions only"):
