@@ -257,14 +257,18 @@ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777
 // normal float (1.2e-38, beta dE ~ 87); a single 24-bit uniform would accept
 // any change with probability at least 2^-24.  p >= 1 accepts; p == 0 or NaN
 // rejects.
+// The common path is branch-free (compares and selects); only the tie takes
+// a branch.
 template <class MORE>
 __host__ __device__ inline bool accept_prob(float p, uint32_t x, MORE more) {
   const float t = p * 4294967296.0f;
-  if (!(t > 0.0f)) return false;
-  if (t >= 4294967296.0f) return true;
-  const uint32_t ti = (uint32_t)t;
-  if (x != ti) return x < ti;
-  return (float)more() < (t - (float)ti) * 4294967296.0f;
+  const bool pos = t > 0.0f;                  // false for 0 and NaN
+  const bool all = t >= 4294967296.0f;        // p >= 1
+  const bool part = pos && !all;
+  const uint32_t ti = part ? (uint32_t)t : 0u;
+  bool acc = all || x < ti;
+  if (part && x == ti) acc = (float)more() < (t - (float)ti) * 4294967296.0f;
+  return acc;
 }
 
 }  // namespace clexmc

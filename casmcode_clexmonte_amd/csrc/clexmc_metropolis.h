@@ -251,8 +251,11 @@ __device__ __forceinline__ float general_dE(cptr<DevSystem> sys, int b, int o, i
   return d;
 }
 
+// dE <= 0 accepts (p = 2 by a select, so no branch; exp(-beta dE) could round
+// below 1 for dE just below 0)
 __device__ __forceinline__ bool metropolis_accept(float dE, float beta, uint32_t x, LaneRng &rng) {
-  return dE <= 0.0f || accept_prob(__expf(-beta * dE), x, [&]() { return rng.next_u32(); });
+  const float p = dE <= 0.0f ? 2.0f : __expf(-beta * dE);
+  return accept_prob(p, x, [&]() { return rng.next_u32(); });
 }
 
 // Binary field policy.  FieldCtx carries what the runtime site programs

@@ -2,6 +2,7 @@
 #include "monte_calculator.hh"
 
 #include <algorithm>
+#include <functional>
 #include <array>
 #include <chrono>
 #include <cstring>
@@ -297,6 +298,12 @@ clexmc_handle *BaseMonteCalculator::handle() {
       throw std::runtime_error("Error in " + calculator_name + ": clexmc_create failed: " + msg);
     }
     m_handle = h;
+    try {
+      _configure_handle(h);
+    } catch (...) {
+      release_handle();
+      throw;
+    }
   }
   return m_handle;
 }
@@ -700,6 +707,64 @@ SamplingFixtureParams GpuSemiGrandCanonicalCalculator::make_default_sampling_fix
   SamplingFixtureParams p;
   p.sampler_names = {"clex.formation_energy", "potential_energy", "mol_composition", "param_composition"};
   return p;
+}
+
+// Multiswap types from every multiset of 2 .. max_total single semi-grand
+// canonical changes (unit, occupant from, occupant to) of the system's
+// mutable asymmetric units, at most 4 distinct changes and 8 sites each.
+static void make_multiswaps(System const &sys, int max_total, std::vector<int> &begin, std::vector<int> &parts) {
+  const clexmc_system_desc d = sys.desc();
+  std::map<int, int> unit_occ;  // unit -> occupants
+  for (int b = 0; b < sys.n_sublat; ++b) {
+    if (sys.n_occ[b] < 2) continue;
+    const int u = d.asym ? d.asym[b] : sys.n_occ[b];
+    unit_occ[u] = std::max(unit_occ[u], sys.n_occ[b]);
+  }
+  std::vector<std::array<int, 3>> singles;
+  for (auto const &kv : unit_occ)
+    for (int a = 0; a < kv.second; ++a)
+      for (int c = 0; c < kv.second; ++c)
+        if (a != c) singles.push_back({kv.first, a, c});
+  begin.assign(1, 0);
+  parts.clear();
+  const int ns = (int)singles.size();
+  std::vector<int> idx;
+  // multisets as non-decreasing index sequences
+  std::function<void(int, int)> rec = [&](int start, int left) {
+    if (left == 0) {
+      if (idx.size() < 2) return;
+      std::map<int, int> cnt;
+      for (int i : idx) ++cnt[i];
+      if (cnt.size() > 4 || idx.size() > 8) return;
+      for (auto const &kv : cnt) {
+        parts.insert(parts.end(), {singles[kv.first][0], singles[kv.first][1], singles[kv.first][2], kv.second});
+      }
+      begin.push_back((int)parts.size() / 4);
+      return;
+    }
+    for (int i = start; i < ns; ++i) {
+      idx.push_back(i);
+      rec(i, left - 1);
+      idx.pop_back();
+    }
+  };
+  for (int n = 2; n <= max_total; ++n) rec(0, n);
+}
+
+void GpuSemiGrandCanonicalCalculator::_configure_handle(clexmc_handle *h) {
+  auto it = params.find("semigrand_canonical_multiswaps");
+  if (it == params.end()) return;
+  if (calculator_name == "GpuNfoldCalculator")
+    throw std::runtime_error("Error in GpuNfoldCalculator: the nfold method takes single-site changes only");
+  const int max_total = std::stoi(it->second);
+  if (max_total < 2 || max_total > 8)
+    throw std::runtime_error("Error in " + calculator_name + ": semigrand_canonical_multiswaps must be 2..8");
+  std::vector<int> begin, parts;
+  make_multiswaps(*system, max_total, begin, parts);
+  auto pp = params.find("multiswaps_per_pass");
+  const int per_pass = pp == params.end() ? 0 : std::stoi(pp->second);
+  check(h, clexmc_set_multiswaps(h, (int)begin.size() - 1, begin.data(), parts.data(), per_pass),
+        calculator_name + " (multiswaps)");
 }
 
 std::vector<std::string> GpuSemiGrandCanonicalCalculator::validate_conditions(State const &state) const {

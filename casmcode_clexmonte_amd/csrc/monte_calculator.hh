@@ -199,6 +199,8 @@ class BaseMonteCalculator {
  protected:
   virtual void _reset() {}
   virtual BaseMonteCalculator *_clone() const = 0;
+  /// method-specific settings of a newly created handle
+  virtual void _configure_handle(clexmc_handle *) {}
 
   /// run n chains through the C ABI; kind 0 canonical, 1 semi-grand
   void _run_chains(int kind, std::vector<State *> const &states, std::vector<std::vector<double>> const &chem,
@@ -254,6 +256,11 @@ class GpuSemiGrandCanonicalCalculator : public BaseMonteCalculator {
 
  protected:
   BaseMonteCalculator *_clone() const override { return new GpuSemiGrandCanonicalCalculator(*this); }
+  /// params "semigrand_canonical_multiswaps" (the largest number of single
+  /// semi-grand canonical changes to combine, as the system's integer input,
+  /// System_json_io.cc:974-983) and "multiswaps_per_pass": composition then
+  /// changes through multiswaps only (clexmc_set_multiswaps)
+  void _configure_handle(clexmc_handle *h) override;
 };
 
 /// N-fold way (rejection-free) semi-grand canonical runs on the GPU (the

@@ -232,6 +232,41 @@ def test_plugin_corr_sampler(flat, tmp_path, method):
             nf.run(P.make_state((6, 6, 6), random_occupation(216, 0.5, 5), cond), P.OccLocation(), rm)
 
 
+@pytest.mark.gpu
+def test_plugin_multiswaps(flat, tmp_path):
+    """The C++ semi-grand canonical calculator with the parameter
+    semigrand_canonical_multiswaps = 2 (every pair of single changes of the
+    binary expansion: {A->B x2}, {A->B, B->A}, {B->A x2}): composition changes
+    only by multiswaps, so the number of B sites keeps its parity while it
+    moves toward the chemical potential's side; the running energy equals the
+    oracle's energy of the final state (exact counts)."""
+    print("[case] plugin multiswaps", flush=True)
+    system = P.system_from_flat(flat, str(tmp_path / "system.flat"))
+    calc = P.MonteCalculator("semigrand_canonical", system,
+                             {"semigrand_canonical_multiswaps": 2, "multiswaps_per_pass": 64})
+    occ = random_occupation(216, 0.5, 8)
+    n0 = int(occ.sum())
+    state = P.make_state((6, 6, 6), occ, {"temperature": 1000.0, "param_chem_pot": [0.3]})
+    rm = P.RunManager()
+    rm.seed = 3
+    params = P.SamplingFixtureParams()
+    params.n_equilibration_passes = 0
+    params.n_passes = 200
+    rm.params = params
+    calc.run(state, P.OccLocation(), rm)
+    mol = np.array(rm.results.samplers["mol_composition"])
+    nB = np.rint(mol[:, 1] * 216).astype(int)
+    assert np.all(nB % 2 == n0 % 2) and nB.max() > n0 + 20, (n0, nB.min(), nB.max())
+    final = np.array(state.configuration.occupation)
+    assert int(final.sum()) % 2 == n0 % 2
+    osys = O.OracleSystem(flat, (6, 6, 6))
+    E = osys.energy(final)
+    assert abs(calc.potential.per_supercell() - (E - 0.3 * float(final.sum()))) <= 1e-9 * max(1.0, abs(E))
+    with pytest.raises(RuntimeError, match="single-site"):
+        nf = P.MonteCalculator("nfold", system, {"semigrand_canonical_multiswaps": 2})
+        nf.run(P.make_state((6, 6, 6), occ, {"temperature": 1000.0, "param_chem_pot": [0.3]}), P.OccLocation(), rm)
+
+
 def test_no_device_fails_loudly(flat, tmp_path):
     """Without a GPU the C++ calculators raise (no CPU fallback); with one,
     this test has nothing to check."""
