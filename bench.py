@@ -427,10 +427,11 @@ def main():
     ap.add_argument("--c4-rounds", type=int, default=32)
     ap.add_argument("--c4-train-samples", type=int, default=512)
     ap.add_argument("--c4-train-steps", type=int, default=200)
-    ap.add_argument("--hw-queues", type=int, default=8,
+    ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process, at most 8 (set before the first HIP call; "
-                         "0 keeps the environment's value): the uneven-shape batch runs one stream per "
-                         "shape, up to 8, which share the hardware queues")
+                         "0, the default, keeps the environment's value, the runtime's 4 unless set): the "
+                         "uneven-shape batch runs one stream per hardware queue, its shape groups balanced "
+                         "over them")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="copy N MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()

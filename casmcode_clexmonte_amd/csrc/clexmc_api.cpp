@@ -36,10 +36,10 @@ struct HostSupercell {
   std::vector<int> gen_prog;
   // specialised kernels per (kind, packed, lanes, launch bound 512 / 1024):
   // 0 not tried, 1 loaded, -1 unavailable
-  int jit_state[2][2][2][2] = {};
+  int jit_state[3][2][2][2] = {};
   size_t jit_lds_fail = 0;  // smallest dynamic LDS size at which a specialised launch was refused
   std::vector<hipModule_t> jit_mods;
-  hipFunction_t jit_fn[2][2][2][2] = {};
+  hipFunction_t jit_fn[3][2][2][2] = {};  // [kind][packed][lanes][block above 512]
   hipFunction_t jit_nfold = nullptr;  // N-fold kernel of the SGC (unpacked, one-lane) module
 };
 
@@ -401,10 +401,12 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
   // periodic colouring of the cells: diagonal residues (i mod m1, j mod m2,
   // k mod m3) or a linear form (i + a j + c k) mod M, conflict-free (no
   // same-sublattice interaction offset inside a class) with at most
-  // kMaxClassSites cells per class.  Preferred: the most parities (mod 2) of
-  // nearest-neighbour offsets among the class translations (so period-2
-  // orderings driven by nearest neighbours, such as L1_2 and L1_0 on FCC or
-  // B2 on BCC, leave no class pure), then the fewest classes.
+  // kMaxClassSites cells per class.  Score: the parities (mod 2) of
+  // nearest-neighbour offsets among the class translations (with none,
+  // period-2 orderings driven by nearest neighbours such as L1_2, L1_0 or B2
+  // can leave every class pure).  Choice: among the colourings with a
+  // positive score (all, if none has one), the fewest classes, except that
+  // one holding every parity is kept when it needs at most 25 % more classes.
   std::set<std::array<int, 3>> par;  // parities of same-sublattice offsets in the shortest pair orbit
   for (int a = 0; a < h->n_orbits && par.empty(); ++a) {
     if (h->orbit_k[a] != 2) continue;
@@ -422,25 +424,21 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
     for (auto const &o : offs[b])
       if (o[0] == b) same.push_back({o[1], o[2], o[3]});
   constexpr long kMaxClassSites = 4096;
-  long best_rank_score = -1, best_n = -1;
-  int best_kind = -1, bm[3] = {0, 0, 0};  // kind 0: residues bm; kind 1: linear (M, a, c) = bm
-  auto consider = [&](int kind, int p0, int p1, int p2, long n_classes, int score) {
-    if (nc / n_classes > kMaxClassSites) return;
-    if (best_kind < 0 || score > best_rank_score || (score == best_rank_score && n_classes < best_n)) {
-      best_kind = kind;
-      best_rank_score = score;
-      best_n = n_classes;
-      bm[0] = p0;
-      bm[1] = p1;
-      bm[2] = p2;
-    }
+  struct Cand {
+    int score;
+    long n;
+    int kind;  // 0: residues (p0, p1, p2); 1: linear (M, a, c) = (p0, p1, p2)
+    int p0, p1, p2;
   };
+  std::vector<Cand> cands;
   for (int m1 = 1; m1 <= L[0]; ++m1) {
     if (L[0] % m1) continue;
     for (int m2 = 1; m2 <= L[1]; ++m2) {
       if (L[1] % m2) continue;
       for (int m3 = 1; m3 <= L[2]; ++m3) {
         if (L[2] % m3) continue;
+        const long n = (long)m1 * m2 * m3;
+        if (nc / n > kMaxClassSites) continue;
         bool ok = true;
         for (auto const &d : same)
           if (d[0] % m1 == 0 && d[1] % m2 == 0 && d[2] % m3 == 0) {
@@ -451,12 +449,12 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
         int score = 0;
         for (auto const &v : par)
           score += ((m1 % 2 || v[0] == 0) && (m2 % 2 || v[1] == 0) && (m3 % 2 || v[2] == 0)) ? 1 : 0;
-        consider(0, m1, m2, m3, (long)m1 * m2 * m3, score);
+        cands.push_back({score, n, 0, m1, m2, m3});
       }
     }
   }
   for (int M = 2; M <= L[0]; ++M) {
-    if (L[0] % M) continue;
+    if (L[0] % M || nc / M > kMaxClassSites) continue;
     for (int a = 0; a < M; ++a) {
       if (((long)a * L[1]) % M) continue;
       for (int c = 0; c < M; ++c) {
@@ -470,8 +468,35 @@ static clexmc_status auto_partition(clexmc_handle *h, const int L[3], int n_pair
         if (!ok) continue;
         int score = 0;
         for (auto const &v : par) score += (M % 2 || (v[0] + a * v[1] + c * v[2]) % 2 == 0) ? 1 : 0;
-        consider(1, M, a, c, M, score);
+        cands.push_back({score, (long)M, 1, M, a, c});
       }
+    }
+  }
+  int best_kind = -1, bm[3] = {0, 0, 0};
+  {
+    bool any_mix = false;
+    for (auto const &c : cands) any_mix = any_mix || c.score > 0;
+    long n_min = -1;
+    for (auto const &c : cands)
+      if ((c.score > 0 || !any_mix) && (n_min < 0 || c.n < n_min)) n_min = c.n;
+    const int full = (int)par.size();
+    const Cand *best = nullptr;
+    bool best_full = false;
+    for (auto const &c : cands) {
+      if (!(c.score > 0 || !any_mix)) continue;
+      const bool is_full = c.score == full && c.n * 4 <= n_min * 5;
+      // full-score colourings within 25 % of the fewest classes come first
+      if (!best || (is_full && !best_full) ||
+          (is_full == best_full && (c.n < best->n || (c.n == best->n && c.score > best->score)))) {
+        best = &c;
+        best_full = is_full;
+      }
+    }
+    if (best) {
+      best_kind = best->kind;
+      bm[0] = best->p0;
+      bm[1] = best->p1;
+      bm[2] = best->p2;
     }
   }
   cb.assign(1, 0);
@@ -1519,7 +1544,8 @@ struct RunPlan {
 // is off or unavailable.
 static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool packed, int lanes, int threads) {
   HostSupercell &hs = h->scs[sc_id];
-  const int k = kind ? 1 : 0, p = packed ? 1 : 0, l = lanes == 2 ? 1 : 0, w = lanes * threads > kAotMaxThreads;
+  const int k = kind < 0 ? 0 : (kind > 2 ? 2 : kind), p = packed ? 1 : 0, l = lanes == 2 ? 1 : 0,
+            w = lanes * threads > kAotMaxThreads;
   int &state = hs.jit_state[k][p][l][w];
   hipFunction_t &fn = hs.jit_fn[k][p][l][w];
   if (state == 0) {
@@ -1629,11 +1655,11 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     h->err = "multiswaps run with the plain proposal only";
     return CLEXMC_ERR_UNSUPPORTED;
   }
-  kind = mswap_run ? 0 : kind;
+  kind = mswap_run ? 2 : kind;
   P.kind = kind;
   P.lanes = 1;
   const int lanes_req = lanes_override ? lanes_override : h->lanes_per_site;
-  if (h->binary && !P.learned && lanes_req == 2 && 2 * threads <= 1024) P.lanes = 2;
+  if (h->binary && !P.learned && !mswap_run && lanes_req == 2 && 2 * threads <= 1024) P.lanes = 2;
   P.threads = threads;
   P.packed = packed;
   P.lds = lds;
@@ -1831,9 +1857,30 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
     P.pass_start = pass_start;
     max_slabs = std::max(max_slabs, P.slabs);
   }
-  // one stream per group (up to 8): groups of whole chains run one persistent
-  // launch, slab groups their sequence of launches
-  const int ns = std::min(ng, 8);
+  // streams: as many as the runtime's hardware queues (GPU_MAX_HW_QUEUES,
+  // 4 unless the process sets it; at most 8), so that streams do not share a
+  // queue; groups of whole chains run one persistent launch, slab groups their
+  // sequence of launches.  Groups go to streams longest first, each to the
+  // stream with the least work so far (work = sites x passes of the group).
+  int n_queues = 4;
+  if (const char *q = std::getenv("GPU_MAX_HW_QUEUES")) n_queues = std::max(1, std::min(8, std::atoi(q)));
+  const int ns = std::min(ng, n_queues);
+  std::vector<int> stream_of(ng, 0);
+  {
+    std::vector<std::pair<long long, int>> work(ng);
+    for (int g = 0; g < ng; ++g) {
+      long long w = 0;
+      for (int c : h->groups[g]) w += h->scs[h->chain_sc[c]].dev.n_sites;
+      work[g] = {w, g};
+    }
+    std::sort(work.begin(), work.end(), [](auto const &a, auto const &b) { return a.first > b.first; });
+    std::vector<long long> load(ns, 0);
+    for (auto const &wg : work) {
+      const int s_min = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      stream_of[wg.second] = s_min;
+      load[s_min] += wg.first;
+    }
+  }
   while ((int)h->side.size() < ns) {
     hipStream_t s;
     hipEvent_t e;
@@ -1847,7 +1894,7 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
   bool all_spec = true, any_packed = false;
   for (int g = 0; g < ng; ++g) {
     RunPlan &P = plans[g];
-    hipStream_t st = h->side[g % ns];
+    hipStream_t st = h->side[stream_of[g]];
     hipError_t le = plan_launch(P, h, st);
     if (le != hipSuccess && P.jit) {
       drop_jit(h, P, P.sc_id);

@@ -203,9 +203,11 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
 }
 
 const char *jit_kernel_name(int kind, bool packed, int lanes_per_site) {
-  static const char *names[2][2][2] = {{{"clexmc_jit_c_u8", "clexmc_jit_c_u8_x2"}, {"clexmc_jit_c_p2", "clexmc_jit_c_p2_x2"}},
-                                       {{"clexmc_jit_s_u8", "clexmc_jit_s_u8_x2"}, {"clexmc_jit_s_p2", "clexmc_jit_s_p2_x2"}}};
-  return names[kind ? 1 : 0][packed ? 1 : 0][lanes_per_site == 2 ? 1 : 0];
+  // kind 0 canonical (c), 1 semi-grand canonical (s), 2 with multiswaps (m)
+  static const char *names[3][2][2] = {{{"clexmc_jit_c_u8", "clexmc_jit_c_u8_x2"}, {"clexmc_jit_c_p2", "clexmc_jit_c_p2_x2"}},
+                                       {{"clexmc_jit_s_u8", "clexmc_jit_s_u8_x2"}, {"clexmc_jit_s_p2", "clexmc_jit_s_p2_x2"}},
+                                       {{"clexmc_jit_m_u8", "clexmc_jit_m_u8_x2"}, {"clexmc_jit_m_p2", "clexmc_jit_m_p2_x2"}}};
+  return names[kind < 0 ? 0 : (kind > 2 ? 2 : kind)][packed ? 1 : 0][lanes_per_site == 2 ? 1 : 0];
 }
 
 // code objects of the sources compiled so far in this process (a handle or

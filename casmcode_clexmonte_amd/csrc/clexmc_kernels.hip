@@ -182,18 +182,23 @@ hipError_t launch_metropolis_split(const RunArgs &args, int kind, bool packed, i
 
 hipError_t launch_metropolis(const RunArgs &args, bool binary, int kind, bool packed, int n_chains, int threads,
                              size_t lds, hipStream_t st, bool launch) {
+  // kind 0 canonical, 1 semi-grand canonical, 2 semi-grand canonical with multiswaps
   if (packed) {
     if (binary)
-      return kind == 0 ? launch_metro<true, 0, true>(args, n_chains, threads, lds, st, launch)
-                       : launch_metro<true, 1, true>(args, n_chains, threads, lds, st, launch);
-    return kind == 0 ? launch_metro<false, 0, true>(args, n_chains, threads, lds, st, launch)
-                     : launch_metro<false, 1, true>(args, n_chains, threads, lds, st, launch);
+      return kind == 0   ? launch_metro<true, 0, true>(args, n_chains, threads, lds, st, launch)
+             : kind == 1 ? launch_metro<true, 1, true>(args, n_chains, threads, lds, st, launch)
+                         : launch_metro<true, 2, true>(args, n_chains, threads, lds, st, launch);
+    return kind == 0   ? launch_metro<false, 0, true>(args, n_chains, threads, lds, st, launch)
+           : kind == 1 ? launch_metro<false, 1, true>(args, n_chains, threads, lds, st, launch)
+                       : launch_metro<false, 2, true>(args, n_chains, threads, lds, st, launch);
   }
   if (binary)
-    return kind == 0 ? launch_metro<true, 0, false>(args, n_chains, threads, lds, st, launch)
-                     : launch_metro<true, 1, false>(args, n_chains, threads, lds, st, launch);
-  return kind == 0 ? launch_metro<false, 0, false>(args, n_chains, threads, lds, st, launch)
-                   : launch_metro<false, 1, false>(args, n_chains, threads, lds, st, launch);
+    return kind == 0   ? launch_metro<true, 0, false>(args, n_chains, threads, lds, st, launch)
+           : kind == 1 ? launch_metro<true, 1, false>(args, n_chains, threads, lds, st, launch)
+                       : launch_metro<true, 2, false>(args, n_chains, threads, lds, st, launch);
+  return kind == 0   ? launch_metro<false, 0, false>(args, n_chains, threads, lds, st, launch)
+         : kind == 1 ? launch_metro<false, 1, false>(args, n_chains, threads, lds, st, launch)
+                     : launch_metro<false, 2, false>(args, n_chains, threads, lds, st, launch);
 }
 
 // ---------------------------------------------------------------------------

@@ -296,6 +296,9 @@ struct GenericField {
 // SPLIT == 2 (binary path): the two lanes of a pair share one site's field in
 // the class sweeps (each sums half of every orbit's clusters, combined with a
 // DPP lane swap), which doubles the wavefronts per chain.
+// KIND: 0 canonical, 1 semi-grand canonical, 2 semi-grand canonical with
+// multiswaps (the canonical sweep followed by the multiswap attempts; a kind
+// of its own so that the other kernels carry none of its registers).
 template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1>
 __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -385,8 +388,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) slabI[q] = 0;
     *slabE = 0.0;
   }
-  // (multiswap runs use the fixed-composition sweep: canonical kernels only)
-  const bool mswaps = KIND == 0 && args.n_mswaps > 0 && n_slabs == 1;
+  // (multiswap runs: the fixed-composition sweep, kind 2)
+  const bool mswaps = KIND == 2 && args.n_mswaps > 0 && n_slabs == 1;
   if (mswaps) {
     if (tid < kMaxSublat * kMaxOcc) poolN[tid] = 0;
     __syncthreads();
@@ -516,7 +519,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
       n_mut = 0;
       for (int c = 0; c < n_classes; ++c) n_mut += ranges(c).n();
     }
-    if (KIND == 0 && auto_rounds)
+    if (KIND != 1 && auto_rounds)
       for (int c = 0; c < n_classes; ++c) m_total += class_mixed(ranges(c));
     for (int c = 0; c < n_classes; ++c) {
       const int b = sc->class_sublat[c];
@@ -697,10 +700,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     // ---- canonical cross-class mixing: randomly chosen pair phases (one per
     // pass, or with automatic rounds about n_mutable / 4 pair attempts)
     const int n_pair_rounds =
-        (KIND == 0 && auto_rounds && sc->n_pair_phases > 0)
+        (KIND != 1 && auto_rounds && sc->n_pair_phases > 0)
             ? auto_pair_rounds(sc->class_begin[n_classes], sc->pair_begin[sc->n_pair_phases], sc->n_pair_phases)
             : 1;
-    for (int pr = 0; KIND == 0 && sc->n_pair_phases > 0 && pr < n_pair_rounds; ++pr) {
+    for (int pr = 0; KIND != 1 && sc->n_pair_phases > 0 && pr < n_pair_rounds; ++pr) {
       const uint32_t ph = uniform_int((uint32_t)sc->n_pair_phases, gpass, 0xFFFFFFF0u - (uint32_t)pr,
                                       (uint32_t)chain, cp->key0, cp->key1);
       const int pb = sc->pair_begin[ph], pe = sc->pair_begin[ph + 1];
@@ -754,7 +757,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     // first site is updated before the second site's field is taken).  These
     // are the reference's unrestricted pair moves; they make every supercell
     // ergodic, however small, for the canonical ensemble.
-    if (KIND == 0 && args.global_swaps > 0 && n_slabs == 1) {
+    if (KIND != 1 && args.global_swaps > 0 && n_slabs == 1) {
       if (tid == 0) {
         const int n_var = sc->class_begin[n_classes];
         for (int g = 0; g < args.global_swaps; ++g) {
@@ -819,7 +822,7 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args) {
     // sites earlier parts took from the same pool) in the state the move
     // starts from and the reverse type's parts for q_rev.  A part whose pool
     // is too small makes the attempt a rejection.
-    if (KIND == 0 && mswaps) {
+    if (KIND == 2 && mswaps) {
       const int n_ty = args.n_mswaps;
       auto unit_sites = [&](int u) -> int {
         int m = 0;

@@ -854,38 +854,37 @@ def periodic_colouring(flat: FlatCE, dims, offs=None):
     ("residue", (m1, m2, m3)), class = (i mod m1, j mod m2, k mod m3), or
     ("linear", (M, a, c)), class = (i + a j + c k) mod M; None if neither is
     conflict-free (no interaction offset within one sublattice joins two
-    cells of a class).  Among the conflict-free ones with at most
-    MAX_CLASS_SITES cells per class it prefers the colouring whose class
-    translations hold the most parities of nearest-neighbour offsets (so a
-    period-2 ordering driven by nearest-neighbour pairs, such as L1_2 or L1_0
-    on FCC or B2 on BCC, does not make every class pure), then the fewest
-    classes.  Residue colourings with odd moduli hold every parity; for
-    example FCC 24^3 keeps 3 x 3 x 3 residues, FCC 32^3 takes
-    (i + 3 j + 7 k) mod 16, whose classes hold every L1_2 sublattice."""
+    cells of a class).  Candidates hold at most MAX_CLASS_SITES cells per
+    class.  A colouring's score is the number of parities of nearest-
+    neighbour offsets (nn_parities) among its class translations: with none,
+    a period-2 ordering driven by nearest-neighbour pairs (L1_2 or L1_0 on
+    FCC, B2 on BCC) can make every class pure, and swaps inside a class then
+    never create an antisite.  Choice: among the colourings with a positive
+    score (all, if none has one), the fewest classes, except that one holding
+    every parity is kept when it needs at most 25 % more classes (a pass
+    costs about one barrier-separated phase per class).  FCC 24^3 keeps
+    3 x 3 x 3 residues (27 classes, every parity); FCC 32^3 takes
+    (i + 3 j + 7 k) mod 16, whose classes hold every L1_2 sublattice; HCP
+    40 x 40 x 24 takes 10 linear classes per sublattice instead of 50
+    residue classes."""
     if offs is None:
         offs = interaction_offsets(flat)
     same = [o[1:] for b, lst in offs.items() for o in lst if o[0] == b]
     L1, L2, L3 = (int(x) for x in dims)
     n_cells = L1 * L2 * L3
     P = nn_parities(flat)
-    best = None
-
-    def consider(key, n_classes, score):
-        nonlocal best
-        if n_cells // n_classes > MAX_CLASS_SITES:
-            return
-        rank = (-score, n_classes)
-        if best is None or rank < best[0]:
-            best = (rank, key)
-
+    cands = []  # (score, n_classes, key)
     divs = [[m for m in range(1, d + 1) if d % m == 0] for d in (L1, L2, L3)]
     for m in itertools.product(*divs):
+        n = m[0] * m[1] * m[2]
+        if n_cells // n > MAX_CLASS_SITES:
+            continue
         if all(d[0] % m[0] or d[1] % m[1] or d[2] % m[2] for d in same):
             odd = [x % 2 == 1 for x in m]
             score = sum(1 for v in P if all(odd[t] or v[t] == 0 for t in range(3)))
-            consider(("residue", tuple(m)), m[0] * m[1] * m[2], score)
+            cands.append((score, n, ("residue", tuple(m))))
     for M in range(2, L1 + 1):
-        if L1 % M:
+        if L1 % M or n_cells // M > MAX_CLASS_SITES:
             continue
         for a in range(M):
             if (a * L2) % M:
@@ -895,8 +894,14 @@ def periodic_colouring(flat: FlatCE, dims, offs=None):
                     continue
                 if all((d[0] + a * d[1] + c * d[2]) % M for d in same):
                     score = sum(1 for v in P if M % 2 == 1 or (v[0] + a * v[1] + c * v[2]) % 2 == 0)
-                    consider(("linear", (M, a, c)), M, score)
-    return best[1] if best else None
+                    cands.append((score, M, ("linear", (M, a, c))))
+    if not cands:
+        return None
+    mixing = [c for c in cands if c[0] > 0] or cands
+    n_min = min(c[1] for c in mixing)
+    full = [c for c in mixing if c[0] == len(P) and c[1] * 4 <= n_min * 5]
+    pool = full or mixing
+    return min(pool, key=lambda c: (c[1], -c[0]))[2]
 
 
 def color_classes(flat: FlatCE, sc: Supercell) -> List[np.ndarray]:
