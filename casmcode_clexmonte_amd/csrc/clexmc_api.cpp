@@ -73,6 +73,11 @@ struct clexmc_handle {
   std::vector<std::vector<int>> groups;
   std::vector<int> group_off;
   int *d_group_map = nullptr;
+  // fused launches (chains on several supercell shapes, one specialised
+  // kernel): per key (kind, launch bound, supercell and image format of each
+  // group) the kernel, nullptr when it could not be built or launched
+  std::map<std::string, hipFunction_t> fused_fn;
+  std::vector<hipModule_t> fused_mods;
   std::vector<hipStream_t> side;    // side streams of the grouped launches (default priority)
   std::vector<hipEvent_t> side_ev;
   long long occ_total = 0;
@@ -92,7 +97,8 @@ struct clexmc_handle {
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
-  int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1;
+  int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1, last_fused = 0;
+  int fuse = 1;  // chains on several supercell shapes: one fused launch (1) or one launch per shape (0)
   int slabs = 0;  // slabs per chain: 0 automatic, 1 off, >= 2 requested
   SlabRecord *d_slab_rec = nullptr;  // [n_chains][kSlabMax][kSlabRecords] (slab launches)
   int slab_rec_chains = 0;
@@ -1299,6 +1305,19 @@ clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs) {
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_set_fused_launch(clexmc_handle *h, int mode) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(mode == 0 || mode == 1, "fused launch mode must be 0 (one launch per shape) or 1 (fused)");
+  h->fuse = mode;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_last_run_fused(clexmc_handle *h, int *fused) {
+  if (!h || !fused) return CLEXMC_ERR_ARG;
+  *fused = h->last_fused;
+  return CLEXMC_OK;
+}
+
 static void free_mswaps(clexmc_handle *h) {
   hipFree(h->d_mswap_parts);
   hipFree(h->d_mswap_begin);
@@ -1605,9 +1624,12 @@ static clexmc_status grow_samples(clexmc_handle *h, long long n_new_samples) {
 
 // threads_override / packing_override: 0 = the handle's setting
 // subset: chain indices of one supercell group (nullptr: all chains)
+// fused_spec: the caller supplies a fused specialised kernel for several
+// plans (no per-plan kernel is built; blocks above 512 threads are kept)
 static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
                               int total_passes, RunPlan &P, int threads_override = 0, int packing_override = 0,
-                              const std::vector<int> *subset = nullptr, int lanes_override = 0) {
+                              const std::vector<int> *subset = nullptr, int lanes_override = 0,
+                              bool fused_spec = false) {
   ARG_CHECK(canonical_rounds >= 0 && canonical_rounds <= 4096, "canonical_rounds out of range (0..4096)");
   int max_pad = 0, max_class = 1;
   const int n_sel = subset ? (int)subset->size() : h->n_chains;
@@ -1701,10 +1723,10 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
     P.sc_id = h->chain_sc[c0];
     P.one_shape = one_shape;
     const size_t fail = h->scs[h->chain_sc[c0]].jit_lds_fail;
-    if (h->specialize && !P.learned && one_shape && (fail == 0 || lds < fail))
+    if (h->specialize && !P.learned && one_shape && !fused_spec && (fail == 0 || lds < fail))
       P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes, P.threads);
   }
-  if (!P.jit) P.threads = std::min(P.threads, kAotMaxThreads);
+  if (!P.jit && !fused_spec) P.threads = std::min(P.threads, kAotMaxThreads);
   return CLEXMC_OK;
 }
 
@@ -1827,11 +1849,159 @@ static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &p
   return hipSuccess;
 }
 
+// The fused kernel for these group plans (one specialised field per supercell
+// shape), compiled on first use; nullptr when it cannot be built.
+static hipFunction_t fused_kernel(clexmc_handle *h, const std::vector<RunPlan> &plans, int kind, int threads) {
+  std::string key = std::to_string(kind) + (threads > kAotMaxThreads ? "w" : "n");
+  for (const RunPlan &P : plans) key += ":" + std::to_string(P.sc_id) + (P.packed ? "p" : "u");
+  auto it = h->fused_fn.find(key);
+  if (it != h->fused_fn.end()) return it->second;
+  hipFunction_t fn = nullptr;
+  std::vector<int> variable(h->n_sublat);
+  for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
+  std::vector<JitShape> shapes;
+  for (const RunPlan &P : plans) {
+    const HostSupercell &hs = h->scs[P.sc_id];
+    shapes.push_back(JitShape{&hs.bin_sub, &hs.bin_off, &hs.gen_sub, &hs.gen_prog, P.packed});
+  }
+  const std::string src = jit_source_fused(h->n_sublat, variable, h->binary != 0, h->n_slots, shapes, kind, threads);
+  hipModule_t mod = nullptr;
+  const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
+  if (je != hipSuccess) {
+    h->jit_log += std::string("\n[fused compile/load: ") + hipGetErrorString(je) + "]";
+  } else if (hipModuleGetFunction(&fn, mod, kJitFusedName) == hipSuccess) {
+    h->fused_mods.push_back(mod);
+  } else {
+    fn = nullptr;
+    h->jit_log += "\n[fused kernel lookup failed]";
+    hipModuleUnload(mod);
+  }
+  h->fused_fn[key] = fn;
+  return fn;
+}
+
+// Chains on different supercells in ONE launch of a fused specialised kernel:
+// each workgroup dispatches on its group (the field of its supercell, its
+// image format, chain map and slabs), so every group runs at once and the
+// hardware spreads all of them over the compute units; slab groups get their
+// slab_post_kernel after each launch.  Returns false (nothing launched) when
+// the fused path does not apply or its kernel is unavailable.
+static bool run_fused(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds,
+                      clexmc_status *status) {
+  const int ng = (int)h->groups.size();
+  *status = CLEXMC_OK;
+  if (ng < 2 || ng > kMaxFusedGroups || h->proposal != 0 || h->lanes_per_site == 2 || h->block_size != 0)
+    return false;
+  std::vector<RunPlan> plans(ng);
+  for (int g = 0; g < ng; ++g) {
+    const clexmc_status st = plan_run(h, kind, n_passes, sample_period, canonical_rounds, n_passes, plans[g], 0, 0,
+                                      &h->groups[g], 0, true);
+    if (st != CLEXMC_OK) {
+      *status = st;
+      return true;
+    }
+    if (plans[g].lanes != 1) return false;
+    plans[g].a.chain_map = h->d_group_map + h->group_off[g];
+  }
+  std::vector<RunPlan *> pp;
+  for (auto &P : plans) pp.push_back(&P);
+  const hipError_t se = choose_slabs(h, pp);
+  if (se != hipSuccess) {
+    h->err = std::string("HIP: ") + hipGetErrorString(se);
+    *status = CLEXMC_ERR_HIP;
+    return true;
+  }
+  int threads = 64, max_slabs = 1;
+  size_t lds = 0;
+  bool any_packed = false;
+  FusedGroups t{};
+  int nb = 0;
+  for (int g = 0; g < ng; ++g) {
+    const RunPlan &P = plans[g];
+    threads = std::max(threads, P.threads);
+    lds = std::max(lds, P.lds);
+    max_slabs = std::max(max_slabs, P.slabs);
+    any_packed = any_packed || P.packed;
+    t.block_begin[g] = nb;
+    t.slabs[g] = std::max(1, P.slabs);
+    t.chain_map[g] = P.a.chain_map;
+    nb += P.n_blocks * std::max(1, P.slabs);
+  }
+  t.block_begin[ng] = nb;
+  if (threads > kMaxJitThreads || (long long)lds > (long long)h->max_lds - 64) return false;
+  const hipFunction_t fn = fused_kernel(h, plans, plans[0].kind, threads);
+  if (!fn) return false;
+  // every plan shares the final sample buffers and the slab records
+  RunArgs a = plans[0].a;
+  a.sampE = h->d_sampE;
+  a.sampN = h->d_sampN;
+  a.sample_cap = h->sample_cap;
+  a.slab_rec = h->d_slab_rec;
+  a.chain_map = nullptr;
+  a.slabs = 1;
+  std::vector<int> lens;
+  if (max_slabs > 1)
+    lens = slab_epochs(n_passes);
+  else
+    lens.push_back(n_passes);
+#define FUSED_TRY(x)                                          \
+  do {                                                        \
+    const hipError_t e_ = (x);                                \
+    if (e_ != hipSuccess) {                                   \
+      h->err = std::string("HIP: ") + hipGetErrorString(e_); \
+      *status = CLEXMC_ERR_HIP;                               \
+      return true;                                            \
+    }                                                         \
+  } while (0)
+  FUSED_TRY(hipEventRecord(h->ev0, h->stream));
+  bool first = true;
+  for (int len : lens) {
+    a.n_passes = len;
+    void *params[] = {(void *)&a, (void *)&t};
+    const hipError_t le = hipModuleLaunchKernel(fn, (unsigned)nb, 1, 1, (unsigned)threads, 1, 1, (unsigned)lds,
+                                                h->stream, params, nullptr);
+    if (le != hipSuccess && first) {
+      // refused before anything ran (for example its LDS size): not again
+      (void)hipGetLastError();
+      for (auto &kv : h->fused_fn)
+        if (kv.second == fn) kv.second = nullptr;
+      return false;
+    }
+    FUSED_TRY(le);
+    first = false;
+    for (int g = 0; g < ng; ++g) {
+      if (plans[g].slabs <= 1) continue;
+      RunArgs ag = a;
+      ag.chain_map = t.chain_map[g];
+      ag.slabs = plans[g].slabs;
+      FUSED_TRY(launch_slab_post(ag, plans[g].n_blocks, h->stream));
+    }
+  }
+  FUSED_TRY(hipEventRecord(h->ev1, h->stream));
+  FUSED_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  FUSED_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+#undef FUSED_TRY
+  h->last_ms = ms;
+  h->last_specialized = 1;
+  h->last_packed = any_packed;
+  h->last_threads = threads;
+  h->last_lanes = 1;
+  h->last_slabs = max_slabs;
+  h->last_fused = 1;
+  return true;
+}
+
 // Chains on different supercells: one launch per supercell group, each with
 // the kernel specialised to its shape and its own block size and LDS image,
 // on side streams so that the groups run concurrently; the handle's stream
 // waits for all of them.
 static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int sample_period, int canonical_rounds) {
+  {
+    clexmc_status st = CLEXMC_OK;
+    if (h->fuse && run_fused(h, kind, n_passes, sample_period, canonical_rounds, &st)) return st;
+  }
+  h->last_fused = 0;
   const int ng = (int)h->groups.size();
   std::vector<RunPlan> plans(ng);
   for (int g = 0; g < ng; ++g) {
@@ -1969,6 +2139,7 @@ static clexmc_status run_passes_impl(clexmc_handle *h, int kind, int n_passes, i
   h->last_threads = P.threads;
   h->last_lanes = P.lanes;
   h->last_slabs = P.slabs;
+  h->last_fused = 0;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   hipError_t le = plan_launch(P, h);
@@ -2573,6 +2744,7 @@ void clexmc_destroy(clexmc_handle *h) {
     free_all(s.allocs);
     for (auto m : s.jit_mods) hipModuleUnload(m);
   }
+  for (auto m : h->fused_mods) hipModuleUnload(m);
   free_all(h->sys_allocs);
   free_mswaps(h);
   if (h->d_scs) hipFree(h->d_scs);

@@ -58,14 +58,12 @@ static std::string hexf(float v) {
   return buf;
 }
 
-std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
-                       const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
-  std::ostringstream o;
-  // the semi-grand canonical, unpacked, one-lane variant also carries the N-fold kernel
-  const bool with_nfold = kind == 1 && !packed && lanes == 1;
-  o << "#include \"clexmc_metropolis.h\"\n" << (with_nfold ? "#include \"clexmc_nfold.h\"\n" : "")
-    << "namespace clexmc {\nstruct JitField {\n"
+// The field policy struct `name` for one supercell's site programs (inside
+// namespace clexmc).
+static void emit_field(std::ostringstream &o, const std::string &name, int n_sublat, const std::vector<int> &variable,
+                       bool binary, const std::vector<BinSub> &subs, const std::vector<int> &bin_off, int n_slots,
+                       const std::vector<GenSub> &gsubs, const std::vector<int> &gen_prog, bool packed) {
+  o << "struct " << name << " {\n"
     << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static float bin(const FieldCtx &, const IMG &img, int b, int pa, int o,\n"
     << "                                              int (&K)[kMaxSlots]) {\n"
@@ -177,28 +175,89 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
       }
     }
     o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    if (packed) {
+      // 2-bit image, 16 sites per word: a read at base + d takes word
+      // (base >> 4) + (d >> 4) + carry and bit field ((base & 15) + (d & 15)) & 15,
+      // where carry and field depend only on d & 15 -- so one word pointer and
+      // one shift per residue used, and each read is a load with a constant
+      // offset and one bit-field extract
+      bool used[16] = {};
+      int q = S.begin;
+      for (int e = 0; e < S.n_entries; ++e) {
+        const int k = gen_prog[q];
+        q += 2;
+        for (int i = 0; i < k; ++i, q += 2) used[(gen_prog[q] - minoff) & 15] = true;
+      }
+      o << "      const uint32_t *wb_ = img.w + (base >> 4);\n      const int rb_ = base & 15;\n";
+      for (int r = 0; r < 16; ++r)
+        if (used[r])
+          o << "      const uint32_t *w" << r << "_ = wb_ + ((rb_ + " << r << ") >> 4);\n"
+            << "      const uint32_t s" << r << "_ = (uint32_t)((rb_ + " << r << ") & 15) << 1;\n";
+    }
     int q = S.begin;
     for (int e = 0; e < S.n_entries; ++e) {
       const int k = gen_prog[q], toff = gen_prog[q + 1];
       q += 2;
       o << "      {\n        const int idx = 0";
-      for (int i = 0; i < k; ++i, q += 2)
-        o << " + img.get(base + " << (gen_prog[q] - minoff) << ") * " << gen_prog[q + 1];
+      for (int i = 0; i < k; ++i, q += 2) {
+        const int d = gen_prog[q] - minoff;
+        if (packed)
+          o << " + (int)__builtin_amdgcn_ubfe(w" << (d & 15) << "_[" << (d >> 4) << "], s" << (d & 15) << "_, 2u) * "
+            << gen_prog[q + 1];
+        else
+          o << " + img.get(base + " << d << ") * " << gen_prog[q + 1];
+      }
       o << ";\n        const float *V = tab + " << toff << " + idx * " << S.nf << ";\n";
       for (int f = 0; f < S.nf && f < kMaxFunc; ++f) o << "        G[" << f << "] += V[" << f << "];\n";
       o << "      }\n";
     }
     o << "      return;\n    }\n";
   }
-  o << "  }\n};\n}  // namespace clexmc\n";
+  o << "  }\n};\n";
+}
+
+std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
+                       const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
+  std::ostringstream o;
+  // the semi-grand canonical, unpacked, one-lane variant also carries the N-fold kernel
+  const bool with_nfold = kind == 1 && !packed && lanes == 1;
+  o << "#include \"clexmc_metropolis.h\"\n" << (with_nfold ? "#include \"clexmc_nfold.h\"\n" : "")
+    << "namespace clexmc {\n";
+  emit_field(o, "JitField", n_sublat, variable, binary, subs, bin_off, n_slots, gsubs, gen_prog, packed);
+  o << "}  // namespace clexmc\n";
   o << "extern \"C\" __global__ __launch_bounds__(" << (lanes == 2 || max_threads > 512 ? 1024 : 512) << ") void "
     << jit_kernel_name(kind, packed, lanes) << "(clexmc::RunArgs a) { clexmc::metropolis_body<"
     << (binary ? "true" : "false") << ", " << kind << ", " << (packed ? "true" : "false") << ", clexmc::JitField, "
-    << lanes << ">(a); }\n";
+    << lanes << ">(a, clexmc::BlockRole{(int)blockIdx.x, a.chain_map, a.slabs}); }\n";
   if (with_nfold)
     o << "extern \"C\" __global__ __launch_bounds__(256) void " << kJitNfoldName
       << "(clexmc::NfoldArgs na) { clexmc::nfold_body<" << (binary ? "true" : "false")
       << ", clexmc::JitField>(na); }\n";
+  return o.str();
+}
+
+std::string jit_source_fused(int n_sublat, const std::vector<int> &variable, bool binary, int n_slots,
+                             const std::vector<JitShape> &shapes, int kind, int max_threads) {
+  std::ostringstream o;
+  o << "#include \"clexmc_metropolis.h\"\nnamespace clexmc {\n";
+  for (size_t g = 0; g < shapes.size(); ++g) {
+    const JitShape &J = shapes[g];
+    emit_field(o, "JitField" + std::to_string(g), n_sublat, variable, binary, *J.subs, *J.bin_off, n_slots, *J.gsubs,
+               *J.gen_prog, J.packed);
+  }
+  o << "}  // namespace clexmc\n";
+  // the group of a block: the last group whose first block is at or below it
+  o << "extern \"C\" __global__ __launch_bounds__(" << (max_threads > 512 ? 1024 : 512) << ") void "
+    << kJitFusedName << "(clexmc::RunArgs a, clexmc::FusedGroups t) {\n"
+    << "  const int blk = (int)blockIdx.x;\n  int g = 0;\n"
+    << "  for (int q = 1; q < " << shapes.size() << "; ++q) g += blk >= t.block_begin[q] ? 1 : 0;\n"
+    << "  const clexmc::BlockRole role{blk - t.block_begin[g], t.chain_map[g], t.slabs[g]};\n"
+    << "  switch (g) {\n";
+  for (size_t g = 0; g < shapes.size(); ++g)
+    o << "    case " << g << ": clexmc::metropolis_body<" << (binary ? "true" : "false") << ", " << kind << ", "
+      << (shapes[g].packed ? "true" : "false") << ", clexmc::JitField" << g << ", 1>(a, role); break;\n";
+  o << "    default: break;\n  }\n}\n";
   return o.str();
 }
 

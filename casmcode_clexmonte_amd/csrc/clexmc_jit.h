@@ -23,6 +23,23 @@ std::string jit_source(int n_sublat, const std::vector<int> &variable, bool bina
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
                        const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads);
 
+// One supercell shape of a fused launch: its site programs (as for
+// jit_source) and image format.
+struct JitShape {
+  const std::vector<BinSub> *subs;
+  const std::vector<int> *bin_off;
+  const std::vector<GenSub> *gsubs;
+  const std::vector<int> *gen_prog;
+  bool packed;
+};
+
+// Source of one kernel for chains on several supercell shapes (one launch):
+// a field struct per shape and a block -> group dispatch (FusedGroups), one
+// lane per site; `max_threads` sets the launch bound as for jit_source.
+std::string jit_source_fused(int n_sublat, const std::vector<int> &variable, bool binary, int n_slots,
+                             const std::vector<JitShape> &shapes, int kind, int max_threads);
+constexpr const char *kJitFusedName = "clexmc_jit_fused";
+
 // Kernel name for (kind, packed, lanes per site) in a module built from
 // jit_source (two lanes per site: binary expansions only).
 const char *jit_kernel_name(int kind, bool packed, int lanes_per_site);

@@ -47,6 +47,18 @@ struct RunArgs {
   const int *mswap_rev;
 };
 
+// A fused launch of chains on several supercell shapes (one specialised
+// kernel for all of them): group g owns blocks [block_begin[g],
+// block_begin[g + 1]), maps its workgroups to chains through chain_map[g]
+// and runs slabs[g] workgroups per chain.  The RunArgs fields chain_map and
+// slabs are not used by such a launch.
+constexpr int kMaxFusedGroups = 16;
+struct FusedGroups {
+  int block_begin[kMaxFusedGroups + 1];
+  int slabs[kMaxFusedGroups];
+  const int *chain_map[kMaxFusedGroups];
+};
+
 struct DeltaArgs {
   const DevSystem *sys;
   const DevSupercell *scs;

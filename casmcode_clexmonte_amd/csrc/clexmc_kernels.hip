@@ -35,13 +35,13 @@ namespace clexmc {
 // ---------------------------------------------------------------------------
 template <bool BIN, int KIND, bool PACK>
 __global__ __launch_bounds__(512) void metropolis_kernel(RunArgs args) {
-  metropolis_body<BIN, KIND, PACK, GenericField>(args);
+  metropolis_body<BIN, KIND, PACK, GenericField>(args, BlockRole{(int)blockIdx.x, args.chain_map, args.slabs});
 }
 
 // binary path, two lanes per site in the class sweeps (up to 1024 threads)
 template <int KIND, bool PACK>
 __global__ __launch_bounds__(1024) void metropolis_split_kernel(RunArgs args) {
-  metropolis_body<true, KIND, PACK, GenericField, 2>(args);
+  metropolis_body<true, KIND, PACK, GenericField, 2>(args, BlockRole{(int)blockIdx.x, args.chain_map, args.slabs});
 }
 
 // ---------------------------------------------------------------------------

@@ -299,13 +299,24 @@ struct GenericField {
 // KIND: 0 canonical, 1 semi-grand canonical, 2 semi-grand canonical with
 // multiswaps (the canonical sweep followed by the multiswap attempts; a kind
 // of its own so that the other kernels carry none of its registers).
+//
+// `role`: this workgroup's block index within its launch group, the group's
+// chain map (nullptr: block -> chain directly) and its slabs per chain.  A
+// launch of one group passes (blockIdx.x, args.chain_map, args.slabs); a
+// fused launch of several supercell groups passes each group's own.
+struct BlockRole {
+  int block;
+  const int *chain_map;
+  int slabs;
+};
+
 template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1>
-__device__ __forceinline__ void metropolis_body(const RunArgs &args) {
+__device__ __forceinline__ void metropolis_body(const RunArgs &args, const BlockRole role) {
   extern __shared__ __align__(16) unsigned char smem[];
   // slabs > 1: this workgroup runs slab `slab` of its chain
-  const int n_slabs = args.slabs > 1 ? args.slabs : 1;
-  const int wg = (int)blockIdx.x / n_slabs, slab = (int)blockIdx.x - wg * n_slabs;
-  const int chain = args.chain_map ? args.chain_map[wg] : wg;
+  const int n_slabs = role.slabs > 1 ? role.slabs : 1;
+  const int wg = role.block / n_slabs, slab = role.block - wg * n_slabs;
+  const int chain = role.chain_map ? role.chain_map[wg] : wg;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int lane = tid & 63;

@@ -77,7 +77,8 @@ def lib():
                      "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
-                     "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs"]:
+                     "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
+                     "clexmc_set_fused_launch", "clexmc_last_run_fused"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -101,7 +102,8 @@ def exported_symbols() -> List[str]:
             "clexmc_reanchor_energies", "clexmc_graph_capture", "clexmc_graph_replay",
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
-            "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs"]
+            "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
+            "clexmc_set_fused_launch", "clexmc_last_run_fused"]
 
 
 def _p(a, t):
@@ -496,6 +498,16 @@ class Engine:
         v = C.c_int()
         self._check(lib().clexmc_last_run_slabs(self.h, C.byref(v)), "clexmc_last_run_slabs")
         return v.value
+
+    def set_fused_launch(self, mode: int = 1):
+        """Chains on several supercell shapes: 1 one fused launch (default),
+        0 one launch per shape on concurrent streams."""
+        self._check(lib().clexmc_set_fused_launch(self.h, C.c_int(mode)), "clexmc_set_fused_launch")
+
+    def last_run_fused(self) -> bool:
+        v = C.c_int()
+        self._check(lib().clexmc_last_run_fused(self.h, C.byref(v)), "clexmc_last_run_fused")
+        return bool(v.value)
 
     def set_multiswaps(self, types, per_pass: int = 0):
         """Semi-grand canonical multiswaps: `types` is a list of multiswap

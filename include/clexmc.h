@@ -262,6 +262,20 @@ clexmc_status clexmc_set_slabs(clexmc_handle *h, int slabs);
  * launches; 1 = whole chains). */
 clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs);
 
+/* Chains on several supercell shapes with run-time specialisation on
+ * (plain proposal, one lane per site, automatic block size, at most 16
+ * shapes): mode 1 (default) runs all of them in ONE launch of a kernel that
+ * holds the specialised field of every shape, each workgroup dispatching on
+ * its shape (image format, chain map, slabs), so all shapes run at once;
+ * mode 0 runs one specialised launch per shape on concurrent streams, which
+ * share the process's hardware queues.  Results depend on the block size
+ * only: the fused launch uses the largest block of the shapes.  No
+ * counterpart in the reference (its runs hold one configuration); this is
+ * the batching of chains on uneven supercells. */
+clexmc_status clexmc_set_fused_launch(clexmc_handle *h, int mode);
+/* 1 when the last clexmc_run_passes ran as one fused launch, else 0. */
+clexmc_status clexmc_last_run_fused(clexmc_handle *h, int *fused);
+
 /* Canonical runs: serial swaps of uniformly random site pairs per pass
  * (default 1), beside the parallel independent-set matchings and pair
  * phases.  The pair may share clusters (the reference's unrestricted

@@ -2,6 +2,7 @@
 // source the library generates for it (test helper).
 //   binary:  "bin" n_slots, then per slot: k_others n_clusters weight offsets...
 //   general: "gen" nf n_entries, then per entry: k table_offset (offset radix)*k
+// A second argument "1" asks for the 2-bit packed image variant.
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -50,6 +51,9 @@ int main(int argc, char **argv) {
       }
     }
   }
-  std::cout << clexmc::jit_source(1, {1}, mode == "bin", subs, off, ns, gsubs, prog, 1, false, mode == "bin" ? 2 : 1, 512);
+  // optional second argument: 1 = 2-bit packed image (table path)
+  const bool packed = argc > 2 && std::string(argv[2]) == "1";
+  std::cout << clexmc::jit_source(1, {1}, mode == "bin", subs, off, ns, gsubs, prog, 1, packed, mode == "bin" ? 2 : 1,
+                                  512);
   return 0;
 }

@@ -168,9 +168,10 @@ def test_specialized_general_kernel_bit_identical_to_generic(kind, packing):
 
 
 def test_grouped_shape_launches_bit_identical_to_one_launch():
-    """Chains on uneven supercells: the grouped launches (one specialised
-    kernel per supercell shape, concurrent streams) equal the single generic
-    launch over all chains bit for bit."""
+    """Chains on uneven supercells: the fused specialised launch (one kernel
+    holding every shape's field) and the per-shape specialised launches on
+    concurrent streams both equal the single generic launch over all chains
+    bit for bit (every shape's blocks have 64 threads here)."""
     _print_case("grouped specialised launches vs one generic launch")
     ce = hcp_ternary_ce()
     flat = FlatCE(ce)
@@ -182,19 +183,22 @@ def test_grouped_shape_launches_bit_identical_to_one_launch():
         occs.append(random_occupation_counts(composition_counts(2 * d[0] * d[1] * d[2], (0.4, 0.3, 0.3)), 80 + c))
     temps = np.linspace(700, 1700, 6)
     runs = []
-    for spec in (False, True):
+    for spec, fuse in ((False, 1), (True, 1), (True, 0)):
         eng = Engine(flat)
         eng.set_specialization(spec)
+        eng.set_fused_launch(fuse)
         sids = [eng.add_supercell(Supercell(ce.prim, d), n_pair_phases=4, seed=i) for i, d in enumerate(shapes)]
         eng.load_chains([sids[c % 3] for c in range(6)], occs, temps, 300 + np.arange(6))
         eng.run_passes("canonical", 10, 1, canonical_rounds=2)
         assert eng.last_run_info()["specialized"] == spec, eng.specialization_log()[-2000:]
+        assert eng.last_run_fused() == (spec and fuse == 1)
         E, cnt = eng.read_samples()
         runs.append((_states(eng, 6), E, cnt))
         eng.close()
-    (s0, E0, n0), (s1, E1, n1) = runs
-    _same(s0, s1)
-    assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
+    (s0, E0, n0) = runs[0]
+    for (s1, E1, n1) in runs[1:]:
+        _same(s0, s1)
+        assert np.array_equal(E0, E1) and np.array_equal(n0, n1)
 
 
 @pytest.mark.parametrize("which", ["binary_sgc", "ternary_canonical"])

@@ -184,3 +184,45 @@ def test_slabs_through_cpp_calculator(tmp_path):
     tol = 4e-6 * np.sqrt(res.n_accept) + 1e-9 * abs(Eref)
     print(f"last sample {e_last:.9f} oracle {Eref:.9f} accepted {res.n_accept}")
     assert abs(e_last - Eref) <= tol, (e_last, Eref, tol)
+
+
+def test_fused_launch_equals_per_shape_launches():
+    """Chains on two HCP ternary shapes (24x24x16 and 32x32x20, two chains
+    each), both run as automatic slabs with 512-thread blocks: the one fused
+    launch (a specialised field per shape, workgroups dispatching on their
+    shape) gives bit-identical states and samples to one specialised launch
+    per shape on concurrent streams; the final energies equal the oracle's."""
+    _print_case("fused launch vs per-shape launches: hcp 24x24x16 + 32x32x20 slabs")
+    ce = hcp_ternary_ce()
+    flat = FlatCE(ce)
+    shapes = [(24, 24, 16), (32, 32, 20)]
+    occs, dims = [], []
+    for c in range(4):
+        d = shapes[c % 2]
+        dims.append(d)
+        occs.append(random_occupation_counts(composition_counts(2 * d[0] * d[1] * d[2], (0.4, 0.3, 0.3)), 60 + c))
+    temps = np.linspace(700.0, 1600.0, 4)
+    runs = []
+    for fuse in (1, 0):
+        eng = Engine(flat, device=0)
+        eng.set_fused_launch(fuse)
+        sids = [eng.add_supercell(Supercell(ce.prim, d), n_pair_phases=4, seed=i) for i, d in enumerate(shapes)]
+        eng.load_chains([sids[c % 2] for c in range(4)], occs, temps, 500 + np.arange(4))
+        eng.run_passes("canonical", 10, 1)
+        info = eng.last_run_info()
+        assert eng.last_run_fused() == bool(fuse), eng.specialization_log()[-2000:]
+        assert info["specialized"] and info["threads"] == 512 and eng.last_run_slabs() > 1, info
+        E, cnt = eng.read_samples()
+        states = [eng.read_state(c) for c in range(4)]
+        runs.append((states, E, cnt))
+        if fuse:
+            for c in range(4):
+                osys = O.OracleSystem(flat, dims[c])
+                Eref = osys.energy(states[c]["occ"])
+                tol = 4e-6 * np.sqrt(max(1, states[c]["n_accept"])) + 1e-9 * abs(Eref)
+                assert abs(states[c]["E"] - Eref) <= tol, (c, states[c]["E"], Eref)
+        eng.close()
+    (s1, E1, n1), (s0, E0, n0) = runs
+    for a, b in zip(s1, s0):
+        assert np.array_equal(a["occ"], b["occ"]) and a["E"] == b["E"] and a["n_accept"] == b["n_accept"]
+    assert np.array_equal(E1, E0) and np.array_equal(n1, n0)

@@ -176,7 +176,19 @@ def _gen_program(flat, L):
     return P, h, nf, tab_max, f"gen {nf} {n}\n" + "\n".join(lines) + "\n"
 
 
-def test_generated_general_field_matches_interpreted_program(tmp_path):
+HOST_GEN_P2 = HOST_GEN.replace(
+    "struct HostImg { const uint8_t *p; int get(int a) const { return p[a]; } };",
+    "#define __builtin_amdgcn_ubfe(x, off, w) ((uint32_t)(x) >> (off) & ((1u << (w)) - 1u))\n"
+    "struct HostImg { const uint8_t *p; const uint32_t *w; int get(int a) const { return p[a]; } };").replace(
+    "HostImg im{img.data()};", "std::vector<uint32_t> words((img.size() + 15) / 16, 0u);\n"
+    "    for (size_t s = 0; s < img.size(); ++s) words[s >> 4] |= (uint32_t)img[s] << (2 * (s & 15));\n"
+    "    HostImg im{img.data(), words.data()};")
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_generated_general_field_matches_interpreted_program(tmp_path, packed):
+    """Table path, int8 image (one load per read) and 2-bit packed image
+    (per-residue word pointers and shifts, one bit-field extract per read)."""
     if not os.path.exists(DUMP):
         pytest.skip("tests/cpp/jit_source_dump not built (run __graft_entry__.build())")
     from casmcode_clexmonte_amd.system import BasisSet, ClusterExpansion, fcc_prim
@@ -186,10 +198,12 @@ def test_generated_general_field_matches_interpreted_program(tmp_path):
     P, h, nf, tab_max, prog = _gen_program(flat, 9)
     pf = tmp_path / "prog.txt"
     pf.write_text(prog)
-    src = subprocess.run([DUMP, str(pf)], capture_output=True, text=True, check=True).stdout
+    src = subprocess.run([DUMP, str(pf), "1" if packed else "0"], capture_output=True, text=True,
+                         check=True).stdout
+    assert ("__builtin_amdgcn_ubfe" in src) == packed
     i = src.index("struct JitField {")
     j = src.index("};\n}  // namespace clexmc")
-    host = (HOST_GEN.replace("JITFIELD", src[i:j + 2]).replace("PADDIM", str(P)).replace("HALO", str(h))
+    host = ((HOST_GEN_P2 if packed else HOST_GEN).replace("JITFIELD", src[i:j + 2]).replace("PADDIM", str(P)).replace("HALO", str(h))
             .replace("NOCCUPANTS", "3").replace("TABSIZE", str(tab_max + 8)))
     cpp = tmp_path / "check.cpp"
     cpp.write_text(host)
