@@ -467,10 +467,11 @@ def main():
         eng.set_block_size(args.block)
     sid = eng.add_supercell(sc, n_pair_phases=0)
     eng.load_chains([sid] * n_chains, batch.occ, batch.temperatures, batch.seeds, chem=batch.chem)
+    # attempt counters before the warm-up, so that the timed launch follows
+    # the warm-up launch directly (the device does not idle in between)
+    att0 = sum(eng.read_state(c)["n_attempt"] for c in range(n_chains))
     if args.warmup > 0:
         eng.run_passes("semigrand_canonical", args.warmup, 1)
-    eng.clear_samples()
-    att0 = sum(eng.read_state(c)["n_attempt"] for c in range(n_chains))
     # clexmc_run_passes returns after the launch has completed on the device
     # (event synchronisation on the handle's stream), so the wall clock below
     # brackets exactly the K timed passes.
@@ -478,7 +479,7 @@ def main():
     kernel_ms = eng.run_passes("semigrand_canonical", args.steps, 1)
     wall = time.perf_counter() - t0
     att1 = sum(eng.read_state(c)["n_attempt"] for c in range(n_chains))
-    attempts = att1 - att0
+    attempts = att1 - att0 - n_chains * sc.n_sites * max(0, args.warmup)
     expected = n_chains * sc.n_sites * args.steps
     assert attempts == expected, (attempts, expected)
     value = attempts / wall
@@ -487,6 +488,7 @@ def main():
     ess_total, tau_med = None, None
     if args.steps >= 500:
         E, cnt = eng.read_samples()
+        E = E[:, E.shape[1] - args.steps:]  # the timed passes (the warm-up's samples come first)
         taus = np.array([tau_int(E[c]) for c in range(n_chains)])
         ess_total = float(np.sum(E.shape[1] / (2.0 * taus)))
         tau_med = float(np.median(taus))

@@ -97,6 +97,50 @@ struct ImgP2 {
   }
 };
 
+// Mixed-radix position (b, pk, pj, pi) of an index into a P1 x P2 x P3 box
+// per sublattice, advanced by a fixed stride with carries instead of
+// divisions (image loads and stores visit indices t, t + s, t + 2s, ...).
+struct PadWalk {
+  int pi, pj, pk, b;
+  int si, sj, sk, sb;
+  int P1, P2, P3;
+  __device__ __forceinline__ PadWalk(int p, int stride, int P1_, int P2_, int P3_) : P1(P1_), P2(P2_), P3(P3_) {
+    const int P12 = P1 * P2, S = P12 * P3;
+    b = p / S;
+    int r = p - b * S;
+    pk = r / P12;
+    r -= pk * P12;
+    pj = r / P1;
+    pi = r - pj * P1;
+    sb = stride / S;
+    r = stride - sb * S;
+    sk = r / P12;
+    r -= sk * P12;
+    sj = r / P1;
+    si = r - sj * P1;
+  }
+  __device__ __forceinline__ void step() {
+    pi += si;
+    int c = pi >= P1 ? 1 : 0;
+    pi -= c * P1;
+    pj += sj + c;
+    c = pj >= P2 ? 1 : 0;
+    pj -= c * P2;
+    pk += sk + c;
+    c = pk >= P3 ? 1 : 0;
+    pk -= c * P3;
+    b += sb + c;
+  }
+};
+
+// the cell coordinate of padded coordinate q (halo h, period L): q - h in [0, L)
+__device__ __forceinline__ int pad_wrap(int q, int h, int L) {
+  int x = q - h;
+  while (x < 0) x += L;
+  while (x >= L) x -= L;
+  return x;
+}
+
 // bytes of the LDS image for pad_sites padded sites, a multiple of 16 so that
 // the arrays placed after it keep their natural (up to 16-byte) alignment
 __host__ __device__ inline int image_bytes(int pad_sites, bool packed) {
@@ -364,31 +408,28 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
 
   // ---- load the padded image (and general tables) ----
   const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
-  auto padded_occ = [&](int p) -> int {
-    int b = p / S;
-    int r = p - b * S;
-    int pk = r / P12;
-    int r2 = r - pk * P12;
-    int pj = r2 / P1;
-    int pi = r2 - pj * P1;
-    int i = ((pi - h) % L1 + L1) % L1;
-    int j = ((pj - h) % L2 + L2) % L2;
-    int k = ((pk - h) % L3 + L3) % L3;
-    return occG[b * n_cells + i + L1 * (j + L2 * k)];
+  const int P2 = P12 / P1, P3 = S / P12;
+  auto walk_occ = [&](const PadWalk &w) -> int {
+    return occG[w.b * n_cells + pad_wrap(w.pi, h, L1) + L1 * (pad_wrap(w.pj, h, L2) + L2 * pad_wrap(w.pk, h, L3))];
   };
   if constexpr (PACK) {
     // each thread assembles whole words (16 padded sites)
     const int n_words = (pad_bytes + 15) / 16;
-    for (int wd = tid; wd < n_words; wd += nthr) {
+    PadWalk ws(16 * tid, 16 * nthr, P1, P2, P3);
+    for (int wd = tid; wd < n_words; wd += nthr, ws.step()) {
+      PadWalk w = ws;
+      w.si = 1;
+      w.sj = w.sk = w.sb = 0;
       uint32_t word = 0u;
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < 16; ++q, w.step()) {
         const int p = wd * 16 + q;
-        if (p < pad_bytes) word |= (uint32_t)(padded_occ(p) & 3) << (2 * q);
+        if (p < pad_bytes) word |= (uint32_t)(walk_occ(w) & 3) << (2 * q);
       }
       img.w[wd] = word;
     }
   } else {
-    for (int p = tid; p < pad_bytes; p += nthr) img.p[p] = (uint8_t)padded_occ(p);
+    PadWalk w(tid, nthr, P1, P2, P3);
+    for (int p = tid; p < pad_bytes; p += nthr, w.step()) img.p[p] = (uint8_t)walk_occ(w);
   }
   if (!BIN) {
     const float *__restrict__ tg = sys->flower_tables;
@@ -475,7 +516,6 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     const int b = a / S;
     const int r = a - b * S;
     const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
-    const int P2 = P12 / P1, P3 = S / P12;
     const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
     for (int dk = -m3; dk <= m3; ++dk) {
       const int qk = pk + dk * L3;
@@ -1162,14 +1202,9 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
   {
     int8_t *occW = args.occ + cp->occ_offset;
     if (n_slabs == 1) {
-      for (int l = tid; l < n_sites; l += nthr) {
-        int b = l / n_cells;
-        int cc = l - b * n_cells;
-        int i = cc % L1;
-        int j = (cc / L1) % L2;
-        int k = cc / (L1 * L2);
-        occW[l] = (int8_t)img.get(b * S + (k + h) * P12 + (j + h) * P1 + (i + h));
-      }
+      PadWalk w(tid, nthr, L1, L2, L3);
+      for (int l = tid; l < n_sites; l += nthr, w.step())
+        occW[l] = (int8_t)img.get(w.b * S + (w.pk + h) * P12 + (w.pj + h) * P1 + (w.pi + h));
     } else {
       // the sites of the updated planes only (one or two plane ranges per sublattice)
       const int plane = L1 * L2, n_sub = n_cells > 0 ? n_sites / n_cells : 0;
