@@ -21,7 +21,7 @@ import numpy as np
 
 from . import analysis as A
 from .completion import CompletionCheckParams, check as completion_check
-from .engine import Engine
+from .engine import NFOLD_PAIRS, Engine
 from .system import KB, ClusterExpansion, CompositionAxes, FlatCE, Supercell
 
 _METHODS = ("canonical", "semigrand_canonical", "nfold")
@@ -302,12 +302,10 @@ class MonteCalculator:
         p.n_passes passes of mol_size rejection-free events each.  Every visited
         state is weighted by its expected residence time, so the statistics are
         residence-weighted averages; one block of events per pass gives the
-        series the precisions are estimated from.  Binary expansions (the
-        events' record holds the count of one occupant); all states on one
-        supercell shape; fixed pass counts (no completion checks)."""
+        series the precisions are estimated from.  Any number of occupants (the
+        events' records hold the moments of every occupant count); all states
+        on one supercell shape; fixed pass counts (no completion checks)."""
         flat = self.system.flat
-        if not flat.binary:
-            raise ValueError("Error in MonteCalculator: the nfold method reports binary cluster expansions only")
         if p.wants_correlations():
             raise ValueError("Error in MonteCalculator: the nfold method samples energies and compositions only "
                              "(no corr.formation_energy or clex.formation_energy.sparse_corr)")
@@ -323,21 +321,23 @@ class MonteCalculator:
         n_cells = int(np.prod(next(iter(shapes))))
         n_var = n_cells * sum(1 for b in range(flat.n_sublat) if flat.n_occ[b] >= 2)
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
-        blocks = self.engine.nfold_run(p.n_passes * n_var, n_var)
+        blocks = self.engine.nfold_run_moments(p.n_passes * n_var, n_var)
         elapsed = time.perf_counter() - t0
         out = []
         for c, s in enumerate(states):
             st = self.engine.read_state(c)
             s.occupation = st["occ"]
-            out.append(self._nfold_summary(s, blocks[c], n_var, st["n_attempt"] - att0[c], elapsed, p))
+            out.append(self._nfold_summary(s, blocks[c], st["n_attempt"] - att0[c], elapsed, p))
         return out
 
-    def _nfold_summary(self, state, blk, n_var, n_events, elapsed, p) -> dict:
-        """Summary in the reference layout from residence-weighted block sums
-        (W, WE, WN, WE2, WN2, WEN per block).  Every sampled quantity q is
-        affine in (E, N1) per supercell, q = a + b E + c N1, so its weighted
-        mean and the covariances of the analysis functions follow exactly from
-        the weighted moments; precisions use the per-block means."""
+    def _nfold_summary(self, state, blk, n_events, elapsed, p) -> dict:
+        """Summary in the reference layout from residence-weighted block
+        moments (clexmc_nfold_run_moments: sum w, sum w E, sum w E^2,
+        sum w N_o, sum w N_o N_q, sum w E N_o per block).  Every sampled
+        quantity q is affine in (E, N_0..N_3) per supercell,
+        q = a + b E + sum_o c_o N_o, so its weighted mean and the covariances
+        of the analysis functions follow exactly from the weighted moments;
+        precisions use the per-block means."""
         axes = self.system.composition_axes
         n = int(np.prod(state.supercell))
         T = float(state.conditions["temperature"])
@@ -345,38 +345,46 @@ class MonteCalculator:
         comps = axes.components
         var_names = [chr(ord("a") + i) for i in range(axes.n_independent)]
         Wt = blk[:, 0].sum()
-        E1, N1 = blk[:, 1].sum() / Wt, blk[:, 2].sum() / Wt
-        E2, N2, EN = blk[:, 3].sum() / Wt, blk[:, 4].sum() / Wt, blk[:, 5].sum() / Wt
-        vEE, vNN, vEN = E2 - E1 * E1, N2 - N1 * N1, EN - E1 * N1
-        # composition as affine functions of N1 (occupant-1 count on the variable sites)
-        ser0 = self._series(state, np.zeros(2), np.array([[n_var, 0, 0, 0], [n_var - 1, 1, 0, 0]]))
-        aff = {}  # name -> (a, b, c): q = a + b E + c N1
-        for i, k in enumerate(comps):
+        E1 = blk[:, 1].sum() / Wt
+        N1 = blk[:, 3:7].sum(axis=0) / Wt
+        NN = np.zeros((4, 4))
+        for t, (o, q) in enumerate(NFOLD_PAIRS):
+            NN[o, q] = NN[q, o] = blk[:, 7 + t].sum() / Wt
+        vEE = blk[:, 2].sum() / Wt - E1 * E1
+        vEN = blk[:, 17:21].sum(axis=0) / Wt - E1 * N1
+        vNN = NN - np.outer(N1, N1)
+        # compositions as affine functions of the occupant counts (sites over
+        # the variable sublattices holding each occupant)
+        basis = np.vstack([np.zeros(4), np.eye(4)])
+        ser0 = self._series(state, np.zeros(5), basis)
+        aff = {}  # name -> (a, b, c[4]): q = a + b E + c . N
+        for k in comps:
             m = ser0[f"mol_composition.{k}"]
-            aff[f"mol_composition.{k}"] = (m[0], 0.0, m[1] - m[0])
-        xa = np.array([ser0[f"param_composition.{k}"][0] for k in var_names])
-        xb = np.array([ser0[f"param_composition.{k}"][1] for k in var_names]) - xa
-        for j, k in enumerate(var_names):
-            aff[f"param_composition.{k}"] = (xa[j], 0.0, xb[j])
-        aff["clex.formation_energy"] = (0.0, 1.0 / n, 0.0)
-        aff["potential_energy"] = (-float(mu @ xa), 1.0 / n, -float(mu @ xb))
+            aff[f"mol_composition.{k}"] = (m[0], 0.0, m[1:] - m[0])
+        for k in var_names:
+            x = ser0[f"param_composition.{k}"]
+            aff[f"param_composition.{k}"] = (x[0], 0.0, x[1:] - x[0])
+        xa = np.array([aff[f"param_composition.{k}"][0] for k in var_names])
+        xc = np.array([aff[f"param_composition.{k}"][2] for k in var_names]).reshape(len(var_names), 4)
+        aff["clex.formation_energy"] = (0.0, 1.0 / n, np.zeros(4))
+        aff["potential_energy"] = (-float(mu @ xa), 1.0 / n, -(mu @ xc) if len(var_names) else np.zeros(4))
 
         def mean(q):
             a, b, c = aff[q]
-            return a + b * E1 + c * N1
+            return a + b * E1 + c @ N1
 
         def cov(q, r):
             _, bq, cq = aff[q]
             _, br, cr = aff[r]
-            return bq * br * vEE + (bq * cr + cq * br) * vEN + cq * cr * vNN
+            return bq * br * vEE + bq * (cr @ vEN) + br * (cq @ vEN) + cq @ vNN @ cr
 
         z = 2.0
         Wb = blk[:, 0]
-        Eb, Nb = blk[:, 1] / Wb, blk[:, 2] / Wb
+        Eb, Nb = blk[:, 1] / Wb, blk[:, 3:7] / Wb[:, None]
 
         def stat(q):
             a, b, c = aff[q]
-            _, err = A.mean_and_error(a + b * Eb + c * Nb)
+            _, err = A.mean_and_error(a + b * Eb + Nb @ c)
             return {"mean": [float(mean(q))], "calculated_precision": [z * err]}
 
         stats = {"clex.formation_energy": {"shape": [], "value": stat("clex.formation_energy")},
@@ -393,7 +401,7 @@ class MonteCalculator:
             d = {"component_names": [f"{i},{j}" for j in bn for i in an], "shape": [len(an), len(bn)]}
             for j in bn:
                 for i in an:
-                    d[f"{i},{j}"] = [cov(f"{pa}{i}", f"{pb}{j}") * n / kT]
+                    d[f"{i},{j}"] = [float(cov(f"{pa}{i}", f"{pb}{j}")) * n / kT]
             analysis[name] = d
         cov_block(comps, comps, "mol_composition.", "mol_composition.", "mol_susc")
         cov_block(var_names, var_names, "param_composition.", "param_composition.", "param_susc")

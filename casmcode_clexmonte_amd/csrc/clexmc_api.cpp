@@ -1429,8 +1429,8 @@ static clexmc_status nfold_check_stalls(clexmc_handle *h) {
   return CLEXMC_ERR_STATE;
 }
 
-clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
-  if (!h) return CLEXMC_ERR_ARG;
+// The N-fold run; blocks_out receives kNfoldBlockSums moments per block.
+static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
   ARG_CHECK(n_events >= 0 && events_per_block >= 1 && n_events % events_per_block == 0,
             "n_events must be a non-negative multiple of events_per_block");
@@ -1495,6 +1495,34 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
   hipFree(d_blocks);
   HIP_TRY(e);
   return nfold_check_stalls(h);
+}
+
+clexmc_status clexmc_nfold_run_moments(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  return nfold_run_impl(h, n_events, events_per_block, blocks_out);
+}
+
+clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_events >= 0 && events_per_block >= 1 && n_events % events_per_block == 0,
+            "n_events must be a non-negative multiple of events_per_block");
+  ARG_CHECK(h->n_chains > 0, "no chains loaded");
+  const size_t n_rec = (size_t)h->n_chains * (size_t)(n_events / events_per_block);
+  std::vector<double> m(n_rec * kNfoldBlockSums);
+  const clexmc_status st = nfold_run_impl(h, n_events, events_per_block, m.data());
+  if (st != CLEXMC_OK) return st;
+  // (W, sum w E, sum w N1, sum w E^2, sum w N1^2, sum w E N1)
+  for (size_t r = 0; r < n_rec; ++r) {
+    const double *q = m.data() + r * kNfoldBlockSums;
+    double *o = blocks_out + r * 6;
+    o[0] = q[0];
+    o[1] = q[1];
+    o[2] = q[3 + 1];
+    o[3] = q[2];
+    o[4] = q[7 + 4];
+    o[5] = q[17 + 1];
+  }
+  return CLEXMC_OK;
 }
 
 clexmc_status clexmc_set_specialization(clexmc_handle *h, int mode) {

@@ -124,14 +124,18 @@ struct AdamArgs {
 };
 
 // N-fold way (rejection-free) semi-grand canonical runs.
-constexpr int kNfoldBlockSums = 6;
+// Per block: [0] sum w, [1] sum w E, [2] sum w E^2, [3 + o] sum w N_o,
+// [7 + t] sum w N_o N_q (o <= q in row order: t = 0 for (0,0), 1 (0,1), 2
+// (0,2), 3 (0,3), 4 (1,1), 5 (1,2), 6 (1,3), 7 (2,2), 8 (2,3), 9 (3,3)),
+// [17 + o] sum w E N_o; E the energy per supercell, N_o the sites holding
+// occupant o, w the residence weight of the state.
+constexpr int kNfoldBlockSums = 21;
 struct NfoldArgs {
   RunArgs a;               // system, supercells, chain params / state, occupations, exact counts
   int n_events;            // events per chain in this launch
   int events_per_block;    // events per output record
   int n_blocks;            // n_events / events_per_block
-  double *blocks;          // [n_chains][n_blocks][kNfoldBlockSums]: W, sum w E, sum w N1,
-                           // sum w E^2, sum w N1^2, sum w E N1
+  double *blocks;          // [n_chains][n_blocks][kNfoldBlockSums]
 };
 
 // clexmc_delta_energy_batch_noalloc: proposals per call, and the layout of

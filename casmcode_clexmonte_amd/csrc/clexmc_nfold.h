@@ -178,7 +178,10 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   long long cnt[kMaxOcc];
   for (int q = 0; q < kMaxOcc; ++q) cnt[q] = cs->counts[q];
   double E_run = cs->E;  // general: running energy
-  double W = 0.0, WE = 0.0, WN = 0.0, WE2 = 0.0, WN2 = 0.0, WEN = 0.0;
+  // residence-weighted moments of the current block (kNfoldBlockSums layout)
+  double mom[kNfoldBlockSums];
+#pragma unroll
+  for (int q = 0; q < kNfoldBlockSums; ++q) mom[q] = 0.0;
   long long n_done = 0;
   double n_mutable = 0.0;
   for (int b = 0; b < sys->n_sublat; ++b)
@@ -236,13 +239,18 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
           else
             E = E_run;
           const double w = n_mutable / (double)R;
-          const double x = (double)cnt[1];
-          W += w;
-          WE += w * E;
-          WN += w * x;
-          WE2 += w * E * E;
-          WN2 += w * x * x;
-          WEN += w * E * x;
+          mom[0] += w;
+          mom[1] += w * E;
+          mom[2] += w * E * E;
+          int t = 0;
+#pragma unroll
+          for (int o = 0; o < kMaxOcc; ++o) {
+            const double wn = w * (double)cnt[o];
+            mom[3 + o] += wn;
+            mom[17 + o] += wn * E;
+#pragma unroll
+            for (int q = o; q < kMaxOcc; ++q, ++t) mom[7 + t] += wn * (double)cnt[q];
+          }
         }
       }
     }
@@ -346,14 +354,11 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       const int blk = (ev + 1) / na.events_per_block - 1;
       if (blk < na.n_blocks) {
         double *out = na.blocks + ((long long)chain * na.n_blocks + blk) * kNfoldBlockSums;
-        out[0] = W;
-        out[1] = WE;
-        out[2] = WN;
-        out[3] = WE2;
-        out[4] = WN2;
-        out[5] = WEN;
+#pragma unroll
+        for (int q = 0; q < kNfoldBlockSums; ++q) out[q] = mom[q];
       }
-      W = WE = WN = WE2 = WN2 = WEN = 0.0;
+#pragma unroll
+      for (int q = 0; q < kNfoldBlockSums; ++q) mom[q] = 0.0;
     }
   }
 

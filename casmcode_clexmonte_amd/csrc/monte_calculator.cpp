@@ -855,8 +855,7 @@ void BaseMonteCalculator::_run_nfold(std::vector<State *> const &states, std::ve
   if (n == 0) throw std::runtime_error("Error in " + where + ": no states");
   int n_var_sub = 0, bvar = -1;
   for (int b = 0; b < sys.n_sublat; ++b) {
-    if (sys.n_occ[b] > 2) throw std::runtime_error("Error in " + where + ": nfold runs take binary expansions only");
-    if (sys.n_occ[b] == 2) {
+    if (sys.n_occ[b] >= 2) {
       ++n_var_sub;
       if (bvar < 0) bvar = b;
     }
@@ -892,86 +891,114 @@ void BaseMonteCalculator::_run_nfold(std::vector<State *> const &states, std::ve
     att0[c] = na;
   }
   const int nb = (int)P.n_passes;
-  std::vector<double> blocks((size_t)n * nb * 6);
-  check(h, clexmc_nfold_run(h, (int)n_events, (int)n_var, blocks.data()), where + " (nfold)");
+  constexpr int M = 21;  // clexmc_nfold_run_moments record: W, wE, wE^2, wN_o, wN_oN_q (o <= q), wEN_o
+  std::vector<double> blocks((size_t)n * nb * M);
+  check(h, clexmc_nfold_run_moments(h, (int)n_events, (int)n_var, blocks.data()), where + " (nfold)");
   double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  // composition as an affine function of N1 (sites holding occupant 1)
+  // compositions as affine functions of the occupant counts N_o (sites of the
+  // variable sublattices holding occupant o)
   std::vector<double> fixed(sys.n_components, 0.0);
   for (int b = 0; b < sys.n_sublat; ++b)
     if (sys.n_occ[b] < 2) fixed[sys.occ_component[b * sys.n_occ_max]] += 1.0;
-  auto mol_of = [&](double N1) {
+  auto mol_of = [&](const double (&N)[4]) {
     std::vector<double> mol(fixed);
-    const double cnt[2] = {(double)n_var - N1, N1};
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < sys.n_occ[bvar]; ++q) {
       int comp = sys.occ_component[bvar * sys.n_occ_max + q];
-      if (comp >= 0) mol[comp] += cnt[q] / (double)nu;
+      if (comp >= 0) mol[comp] += N[q] / (double)nu;
     }
     return mol;
   };
-  const std::vector<double> m0 = mol_of(0.0), m1 = mol_of(1.0);
-  const std::vector<double> x0 = sys.param_composition(m0), x1 = sys.param_composition(m1);
+  const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+  const std::vector<double> m0 = mol_of(zero), x0 = sys.param_composition(m0);
+  std::vector<std::vector<double>> dm(4), dx(4);  // change per unit of N_o
+  for (int o = 0; o < 4; ++o) {
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    e[o] = 1.0;
+    const std::vector<double> mo = mol_of(e), xo = sys.param_composition(mo);
+    for (size_t i = 0; i < m0.size(); ++i) dm[o].push_back(mo[i] - m0[i]);
+    for (size_t i = 0; i < x0.size(); ++i) dx[o].push_back(xo[i] - x0[i]);
+  }
   results.assign(n, Results());
   for (int c = 0; c < n; ++c) {
     State &st = *states[c];
     Results &R = results[c];
     const double T = temps[c];
     const auto mu = required_vector(st, "param_chem_pot");
-    // q = a + b E + g N1 for every sampled component (E per supercell)
+    // q = a + b E + g . N for every sampled component (E per supercell)
     struct Aff {
-      double a, b, g;
+      double a, b, g[4];
     };
     std::map<std::string, std::vector<Aff>> aff;
-    double pe_a = 0.0, pe_g = 0.0;
+    Aff pe{0.0, 1.0 / (double)nu, {0.0, 0.0, 0.0, 0.0}};
     for (size_t i = 0; i < x0.size(); ++i) {
-      pe_a -= mu[i] * x0[i];
-      pe_g -= mu[i] * (x1[i] - x0[i]);
+      pe.a -= mu[i] * x0[i];
+      for (int o = 0; o < 4; ++o) pe.g[o] -= mu[i] * dx[o][i];
     }
-    aff["clex.formation_energy"] = {{0.0, 1.0 / (double)nu, 0.0}};
-    aff["potential_energy"] = {{pe_a, 1.0 / (double)nu, pe_g}};
-    for (size_t i = 0; i < m0.size(); ++i) aff["mol_composition"].push_back({m0[i], 0.0, m1[i] - m0[i]});
-    for (size_t i = 0; i < x0.size(); ++i) aff["param_composition"].push_back({x0[i], 0.0, x1[i] - x0[i]});
-    const double *bk = blocks.data() + (size_t)c * nb * 6;
-    double S[6] = {0, 0, 0, 0, 0, 0};
-    std::vector<double> Eb(nb), Nb(nb);
+    aff["clex.formation_energy"] = {Aff{0.0, 1.0 / (double)nu, {0.0, 0.0, 0.0, 0.0}}};
+    aff["potential_energy"] = {pe};
+    for (size_t i = 0; i < m0.size(); ++i)
+      aff["mol_composition"].push_back(Aff{m0[i], 0.0, {dm[0][i], dm[1][i], dm[2][i], dm[3][i]}});
+    for (size_t i = 0; i < x0.size(); ++i)
+      aff["param_composition"].push_back(Aff{x0[i], 0.0, {dx[0][i], dx[1][i], dx[2][i], dx[3][i]}});
+    const double *bk = blocks.data() + (size_t)c * nb * M;
+    double S[M] = {};
+    std::vector<double> Eb(nb);
+    std::vector<std::array<double, 4>> Nb(nb);
     for (int k = 0; k < nb; ++k) {
-      for (int j = 0; j < 6; ++j) S[j] += bk[k * 6 + j];
-      Eb[k] = bk[k * 6 + 1] / bk[k * 6];
-      Nb[k] = bk[k * 6 + 2] / bk[k * 6];
+      for (int j = 0; j < M; ++j) S[j] += bk[k * M + j];
+      Eb[k] = bk[k * M + 1] / bk[k * M];
+      for (int o = 0; o < 4; ++o) Nb[k][o] = bk[k * M + 3 + o] / bk[k * M];
     }
-    const double E1 = S[1] / S[0], N1 = S[2] / S[0];
-    const double vEE = S[3] / S[0] - E1 * E1, vNN = S[4] / S[0] - N1 * N1, vEN = S[5] / S[0] - E1 * N1;
+    const double E1 = S[1] / S[0];
+    double N1[4], vEN[4], vNN[4][4];
+    for (int o = 0; o < 4; ++o) N1[o] = S[3 + o] / S[0];
+    const double vEE = S[2] / S[0] - E1 * E1;
+    for (int o = 0; o < 4; ++o) vEN[o] = S[17 + o] / S[0] - E1 * N1[o];
+    for (int o = 0, t = 0; o < 4; ++o)
+      for (int q = o; q < 4; ++q, ++t) vNN[o][q] = vNN[q][o] = S[7 + t] / S[0] - N1[o] * N1[q];
+    auto mean = [&](Aff const &p) {
+      double v = p.a + p.b * E1;
+      for (int o = 0; o < 4; ++o) v += p.g[o] * N1[o];
+      return v;
+    };
     auto cov = [&](Aff const &p, Aff const &q) {
-      return p.b * q.b * vEE + (p.b * q.g + p.g * q.b) * vEN + p.g * q.g * vNN;
+      double v = p.b * q.b * vEE;
+      for (int o = 0; o < 4; ++o) v += (p.b * q.g[o] + p.g[o] * q.b) * vEN[o];
+      for (int o = 0; o < 4; ++o)
+        for (int r = 0; r < 4; ++r) v += p.g[o] * q.g[r] * vNN[o][r];
+      return v;
     };
     // one sample per pass: the pass's residence-weighted means
     for (auto const &kv : aff) {
       auto &ser = R.samplers[kv.first];
       ser.assign(nb, std::vector<double>(kv.second.size()));
       for (int k = 0; k < nb; ++k)
-        for (size_t i = 0; i < kv.second.size(); ++i)
-          ser[k][i] = kv.second[i].a + kv.second[i].b * Eb[k] + kv.second[i].g * Nb[k];
+        for (size_t i = 0; i < kv.second.size(); ++i) {
+          const Aff &f = kv.second[i];
+          double v = f.a + f.b * Eb[k];
+          for (int o = 0; o < 4; ++o) v += f.g[o] * Nb[k][o];
+          ser[k][i] = v;
+        }
       for (size_t i = 0; i < kv.second.size(); ++i) {
         auto col = column(ser, i);
         double v = cov_of(col, col);
         double err = std::sqrt(std::max(0.0, v) * 2.0 * tau_int(col) / col.size());
-        R.statistics[kv.first + "(" + std::to_string(i) + ")"] = {
-            kv.second[i].a + kv.second[i].b * E1 + kv.second[i].g * N1, err};
+        R.statistics[kv.first + "(" + std::to_string(i) + ")"] = {mean(kv.second[i]), err};
       }
     }
     R.samplers["temperature"].assign(nb, {T});
     R.samplers["param_chem_pot"].assign(nb, mu);
     // analysis functions (analysis_functions.cc:17-173) on the weighted moments
-    const Aff pe = aff["potential_energy"][0];
-    const auto &M = aff["mol_composition"];
+    const auto &Mc = aff["mol_composition"];
     const auto &X = aff["param_composition"];
     const double kT = KB * T;
     R.analysis["heat_capacity"] = {cov(pe, pe) * (double)nu / (kT * T)};
     std::vector<double> mol_susc, param_susc, mol_tc, param_tc;
-    for (size_t j = 0; j < M.size(); ++j)
-      for (size_t i = 0; i < M.size(); ++i) mol_susc.push_back(cov(M[i], M[j]) * (double)nu / kT);
+    for (size_t j = 0; j < Mc.size(); ++j)
+      for (size_t i = 0; i < Mc.size(); ++i) mol_susc.push_back(cov(Mc[i], Mc[j]) * (double)nu / kT);
     for (size_t j = 0; j < X.size(); ++j)
       for (size_t i = 0; i < X.size(); ++i) param_susc.push_back(cov(X[i], X[j]) * (double)nu / kT);
-    for (size_t j = 0; j < M.size(); ++j) mol_tc.push_back(cov(pe, M[j]) * (double)nu / kT);
+    for (size_t j = 0; j < Mc.size(); ++j) mol_tc.push_back(cov(pe, Mc[j]) * (double)nu / kT);
     for (size_t j = 0; j < X.size(); ++j) param_tc.push_back(cov(pe, X[j]) * (double)nu / kT);
     R.analysis["mol_susc"] = mol_susc;
     R.analysis["param_susc"] = param_susc;

@@ -78,7 +78,7 @@ def lib():
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
                      "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
-                     "clexmc_set_fused_launch", "clexmc_last_run_fused"]:
+                     "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -103,7 +103,12 @@ def exported_symbols() -> List[str]:
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
             "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
-            "clexmc_set_fused_launch", "clexmc_last_run_fused"]
+            "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments"]
+
+
+# layout of clexmc_nfold_run_moments' records
+NFOLD_MOMENTS = 21
+NFOLD_PAIRS = [(o, q) for o in range(4) for q in range(o, 4)]  # index t of sum w N_o N_q is 7 + t
 
 
 def _p(a, t):
@@ -483,6 +488,19 @@ class Engine:
         out = np.zeros((self.n_chains, n_blocks, 6))
         self._check(lib().clexmc_nfold_run(self.h, C.c_int(n_events), C.c_int(events_per_block), _p(out, f64p)),
                     "clexmc_nfold_run")
+        return out
+
+    def nfold_run_moments(self, n_events: int, events_per_block: int) -> np.ndarray:
+        """The same run with the moments of every occupant count: returns
+        [n_chains, n_events // events_per_block, 21] with per block
+        [0] sum w, [1] sum w E, [2] sum w E^2, [3 + o] sum w N_o,
+        [7 + t] sum w N_o N_q (o <= q in row order), [17 + o] sum w E N_o."""
+        _check_cint("n_events", n_events)
+        _check_cint("events_per_block", events_per_block)
+        n_blocks = n_events // events_per_block
+        out = np.zeros((self.n_chains, n_blocks, NFOLD_MOMENTS))
+        self._check(lib().clexmc_nfold_run_moments(self.h, C.c_int(n_events), C.c_int(events_per_block),
+                                                   _p(out, f64p)), "clexmc_nfold_run_moments")
         return out
 
     def set_lds_row_pitch(self, extra: int = -1):

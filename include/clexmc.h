@@ -336,6 +336,15 @@ clexmc_status clexmc_set_lds_row_pitch(clexmc_handle *h, int extra);
  * per supercell and N1 the number of sites holding occupant 1.  Chain state (occupation,
  * exact counts, energy, counters) advances as in clexmc_run_passes. */
 clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out);
+/* The same run with the moments of every occupant count (any number of
+ * occupants, the calculator-level nfold method for ternary and larger
+ * expansions): blocks_out[n_chains][n_events / events_per_block][21] receives
+ * per block [0] sum w, [1] sum w E, [2] sum w E^2, [3 + o] sum w N_o
+ * (o = 0..3), [7 + t] sum w N_o N_q for o <= q in row order (t = 0 for
+ * (0,0), 1 (0,1), 2 (0,2), 3 (0,3), 4 (1,1), 5 (1,2), 6 (1,3), 7 (2,2),
+ * 8 (2,3), 9 (3,3)), [17 + o] sum w E N_o; N_o the sites holding occupant
+ * o (0 for occupants the system does not have). */
+clexmc_status clexmc_nfold_run_moments(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out);
 
 /* Run-time specialisation (hipRTC, gfx950) of the binary Metropolis kernel
  * to the loaded supercell: constant site offsets instead of interpreted site

@@ -100,47 +100,81 @@ def test_calculator_rejects_bad_conditions(system):
         MonteCalculator("grand_canonical_typo", system)
 
 
+def _moment_blocks(w, E, N, n_blocks):
+    """Residence-weighted moment records (clexmc_nfold_run_moments layout)
+    of states (w, E, N[4]) cut into n_blocks consecutive blocks."""
+    from casmcode_clexmonte_amd.engine import NFOLD_MOMENTS, NFOLD_PAIRS
+    blk = np.zeros((n_blocks, NFOLD_MOMENTS))
+    for b, s in enumerate(np.array_split(np.arange(len(w)), n_blocks)):
+        ww, ee, nn = w[s].astype(float), E[s], N[s].astype(float)
+        blk[b, 0], blk[b, 1], blk[b, 2] = ww.sum(), (ww * ee).sum(), (ww * ee * ee).sum()
+        blk[b, 3:7] = (ww[:, None] * nn).sum(0)
+        for t, (o, q) in enumerate(NFOLD_PAIRS):
+            blk[b, 7 + t] = (ww * nn[:, o] * nn[:, q]).sum()
+        blk[b, 17:21] = (ww[:, None] * ee[:, None] * nn).sum(0)
+    return blk
+
+
+def _compare_nfold_summary(system, st, w, E, N, comps, axes_names):
+    nf = object.__new__(MonteCalculator)
+    nf.method, nf.system = "nfold", system
+    sg = object.__new__(MonteCalculator)
+    sg.method, sg.system = "semigrand_canonical", system
+    got = nf._nfold_summary(st, _moment_blocks(w, E, N, 30), 3000, 0.0, None)
+    Er, Nr = np.repeat(E, w), np.repeat(N, w, axis=0)
+    ref = sg._summary(st, Er, Nr, 1, 1, 0.0)
+    for q in ("clex.formation_energy", "potential_energy"):
+        assert got["statistics"][q]["value"]["mean"][0] == pytest.approx(ref["statistics"][q]["value"]["mean"][0],
+                                                                         rel=1e-12, abs=1e-12)
+    for k in comps:
+        assert got["statistics"]["mol_composition"][k]["mean"][0] == pytest.approx(
+            ref["statistics"]["mol_composition"][k]["mean"][0], rel=1e-12, abs=1e-14)
+    for k in axes_names:
+        assert got["statistics"]["param_composition"][k]["mean"][0] == pytest.approx(
+            ref["statistics"]["param_composition"][k]["mean"][0], rel=1e-12, abs=1e-14)
+    assert got["analysis"]["heat_capacity"]["value"][0] == pytest.approx(ref["analysis"]["heat_capacity"]["value"][0],
+                                                                        rel=1e-8)
+    for key, names in (("mol_susc", [f"{i},{j}" for i in comps for j in comps]),
+                       ("param_susc", [f"{i},{j}" for i in axes_names for j in axes_names]),
+                       ("mol_thermochem_susc", [f"S,{j}" for j in comps]),
+                       ("param_thermochem_susc", [f"S,{j}" for j in axes_names])):
+        for c in names:
+            g, r = got["analysis"][key][c][0], ref["analysis"][key][c][0]
+            assert g == pytest.approx(r, rel=1e-8, abs=1e-10), (key, c, g, r)
+    return got
+
+
 def test_nfold_summary_equals_replicated_samples(system):
     """The N-fold summary is formed from residence-weighted moments (sums of
-    w, wE, wN1, wE^2, wN1^2, wEN1 per block).  With integer weights, the same
-    states repeated w times are ordinary samples: their plain summary (the
-    semi-grand canonical one) must give the same means, heat capacity and
-    susceptibilities.  No device work: the calculators are built without an
+    w, wE, wE^2, wN_o, wN_oN_q, wEN_o per block).  With integer weights, the
+    same states repeated w times are ordinary samples: their plain summary
+    (the semi-grand canonical one) must give the same means, heat capacity
+    and susceptibilities.  Binary FCC and ternary HCP (two independent
+    composition axes).  No device work: the calculators are built without an
     engine."""
     rng = np.random.default_rng(5)
     n = 512
     E = rng.normal(-20.0, 3.0, 300)
     N1 = rng.integers(150, 360, 300)
+    N = np.zeros((300, 4), dtype=np.int64)
+    N[:, 0], N[:, 1] = n - N1, N1
     w = rng.integers(1, 6, 300)
-    cond = {"temperature": 700.0, "param_chem_pot": [0.07]}
-    st = MonteCarloState((8, 8, 8), np.zeros(n, np.int8), cond)
-    nf = object.__new__(MonteCalculator)
-    nf.method, nf.system = "nfold", system
-    sg = object.__new__(MonteCalculator)
-    sg.method, sg.system = "semigrand_canonical", system
-    blk = np.zeros((30, 6))
-    for b in range(30):
-        s = slice(10 * b, 10 * b + 10)
-        ww, ee, xx = w[s].astype(float), E[s], N1[s].astype(float)
-        blk[b] = [ww.sum(), (ww * ee).sum(), (ww * xx).sum(), (ww * ee * ee).sum(), (ww * xx * xx).sum(),
-                  (ww * ee * xx).sum()]
-    got = nf._nfold_summary(st, blk, n, 3000, 0.0, None)
-    Er, Nr = np.repeat(E, w), np.repeat(N1, w)
-    ref = sg._summary(st, Er, np.stack([n - Nr, Nr], axis=1), 1, 1, 0.0)
-    for q in ("clex.formation_energy", "potential_energy"):
-        assert got["statistics"][q]["value"]["mean"][0] == pytest.approx(ref["statistics"][q]["value"]["mean"][0],
-                                                                         rel=1e-12, abs=1e-12)
-    for k in ("A", "B"):
-        assert got["statistics"]["mol_composition"][k]["mean"][0] == pytest.approx(
-            ref["statistics"]["mol_composition"][k]["mean"][0], rel=1e-12)
-    assert got["statistics"]["param_composition"]["a"]["mean"][0] == pytest.approx(
-        ref["statistics"]["param_composition"]["a"]["mean"][0], rel=1e-12)
-    for key, comp in (("heat_capacity", None), ("mol_susc", "B,B"), ("param_susc", "a,a"),
-                      ("mol_thermochem_susc", "S,B"), ("param_thermochem_susc", "S,a")):
-        g = got["analysis"][key]["value" if comp is None else comp][0]
-        r = ref["analysis"][key]["value" if comp is None else comp][0]
-        assert g == pytest.approx(r, rel=1e-8), key
+    st = MonteCarloState((8, 8, 8), np.zeros(n, np.int8), {"temperature": 700.0, "param_chem_pot": [0.07]})
+    got = _compare_nfold_summary(system, st, w, E, N, ("A", "B"), ("a",))
     assert got["conditions"]["param_chem_pot"]["a"] == [0.07]
+    # ternary HCP: 2 sublattices x 4^3 cells, occupants A, B, C on both
+    from casmcode_clexmonte_amd.workloads import hcp_ternary_ce
+    sys3 = System(hcp_ternary_ce())
+    n3 = 2 * 64
+    N3 = np.zeros((300, 4), dtype=np.int64)
+    N3[:, 1] = rng.integers(20, 60, 300)
+    N3[:, 2] = rng.integers(10, 50, 300)
+    N3[:, 0] = n3 - N3[:, 1] - N3[:, 2]
+    E3 = rng.normal(-5.0, 1.5, 300)
+    st3 = MonteCarloState((4, 4, 4), np.zeros(n3, np.int8), {"temperature": 900.0, "param_chem_pot": [0.05, -0.03]})
+    axes = sys3.composition_axes
+    _compare_nfold_summary(sys3, st3, w, E3, N3, tuple(axes.components),
+                           tuple(chr(ord("a") + i) for i in range(axes.n_independent)))
 
 
 @pytest.mark.gpu
@@ -168,6 +202,45 @@ def test_python_nfold_matches_sgc(system):
             se = np.hypot(a["calculated_precision"][0], b["calculated_precision"][0]) / 2.0
             assert abs(a["mean"][0] - b["mean"][0]) < 4 * se + 1e-6, (sec, a, b)
         assert g["completion_check_results"]["count"][0] == 1500 * 512
+
+
+@pytest.mark.gpu
+def test_python_nfold_ternary_matches_sgc():
+    """The nfold method for a ternary expansion (HCP, 2 sublattices, A/B/C;
+    the records carry the moments of every occupant count) against the
+    semi-grand canonical calculator: formation energy and both composition
+    axes agree within 4 combined standard errors at two chemical
+    potentials; heat capacity and the a,a / b,b susceptibilities agree
+    within 25 % (their errors are not estimated)."""
+    from casmcode_clexmonte_amd.workloads import hcp_ternary_ce
+    print("[case] python calculator nfold vs sgc, hcp ternary 6x6x4, T 1500 K", flush=True)
+    sys3 = System(hcp_ternary_ce())
+    mus = [[0.0, 0.0], [0.04, -0.03]]
+    n = 2 * 6 * 6 * 4
+
+    def states():
+        rng = np.random.default_rng(3)
+        return [MonteCarloState((6, 6, 4), rng.integers(0, 3, n).astype(np.int8),
+                                {"temperature": 1500.0, "param_chem_pot": list(mu)}) for mu in mus]
+    sg = MonteCalculator("semigrand_canonical", sys3)
+    ref = sg.run_multistate(states(), sg.make_default_sampling_fixture_params(n_equilibration_passes=300,
+                                                                              n_passes=4000))
+    nf = MonteCalculator("nfold", sys3)
+    got = nf.run_multistate(states(), nf.make_default_sampling_fixture_params(n_equilibration_passes=300,
+                                                                               n_passes=2000))
+    validate_summary(merge_summaries(got), len(mus), False)
+    for r, g in zip(ref, got):
+        for sec, key in (("clex.formation_energy", "value"), ("param_composition", "a"), ("param_composition", "b")):
+            a, b = r["statistics"][sec][key], g["statistics"][sec][key]
+            se = np.hypot(a["calculated_precision"][0], b["calculated_precision"][0]) / 2.0
+            print(sec, key, a["mean"][0], b["mean"][0], se)
+            assert abs(a["mean"][0] - b["mean"][0]) < 4 * se + 1e-6, (sec, key, a, b)
+        for key, comp in (("heat_capacity", None), ("param_susc", "a,a"), ("param_susc", "b,b")):
+            a = r["analysis"][key]["value" if comp is None else comp][0]
+            b = g["analysis"][key]["value" if comp is None else comp][0]
+            print(key, comp, a, b)
+            assert abs(a - b) <= 0.25 * abs(a), (key, comp, a, b)
+        assert g["completion_check_results"]["count"][0] == 2000 * n
 
 
 def test_fixture_legacy_quantity_names():

@@ -15,14 +15,16 @@
   states) and quaternary BCC 2x2x2 (4^8 states), T = 1000 K: weighted
   averages of E, E^2, N_1, N_1^2, E N_1 over 128 chains within 3 sigma of exact
   enumeration; final energies equal the oracle's within the FP32
-  accumulation bound (4e-6 eV x sqrt(events)); occupant counts follow the
-  occupations.
+  accumulation bound (4e-6 eV x sqrt(events)); then the per-occupant
+  moments (clexmc_nfold_run_moments: <N_o>, <N_o N_q>, <E N_o> for every
+  occupant) within 3 sigma of the exact Boltzmann moments over all states;
+  occupant counts follow the occupations.
 """
 import numpy as np
 import pytest
 
 import oracle as O
-from casmcode_clexmonte_amd.engine import Engine
+from casmcode_clexmonte_amd.engine import NFOLD_MOMENTS, NFOLD_PAIRS, Engine
 from casmcode_clexmonte_amd.system import (BasisSet, ClusterExpansion, FlatCE, Supercell, bcc_prim, fcc_prim,
                                            hcp_prim)
 from casmcode_clexmonte_amd.workloads import fcc_binary_ce, random_occupation
@@ -169,6 +171,29 @@ def test_nfold_general_exact_enumeration(system):
         st = eng.read_state(c)
         Eref = osys.energy(st["occ"])
         assert abs(st["E"] - Eref) <= 4e-6 * np.sqrt(n_events) + 1e-9 * abs(Eref), (st["E"], Eref)
+    # the moments of every occupant count (clexmc_nfold_run_moments) against
+    # the exact Boltzmann moments over all states
+    mom = eng.nfold_run_moments(n_events, 500)
+    assert mom.shape == (nch, 40, NFOLD_MOMENTS) and np.all(mom[:, :, 0] > 0)
+    ex = osys.enumerate("sgc", T, chem=chem, want_probs=True)
+    pr, Es = ex["probs"], osys.state_energies()
+    st_i = np.arange(len(pr))
+    occ_s = (st_i[:, None] // (nocc ** np.arange(sc.n_sites))[None, :]) % nocc
+    Ns = np.stack([(occ_s == o).sum(1) for o in range(4)], axis=1).astype(float)
+    b = mom.sum(axis=1)
+    W = b[:, 0]
+    checks = [("E", b[:, 1] / W, pr @ Es), ("E2", b[:, 2] / W, pr @ (Es * Es))]
+    for o in range(nocc):
+        checks.append((f"N{o}", b[:, 3 + o] / W, pr @ Ns[:, o]))
+        checks.append((f"E N{o}", b[:, 17 + o] / W, pr @ (Es * Ns[:, o])))
+    for t, (o, q) in enumerate(NFOLD_PAIRS):
+        if q < nocc:
+            checks.append((f"N{o} N{q}", b[:, 7 + t] / W, pr @ (Ns[:, o] * Ns[:, q])))
+    for name, per, ref in checks:
+        m, se = per.mean(), per.std(ddof=1) / np.sqrt(nch)
+        print(f"{name}: nfold {m:.6f} +- {se:.6f} exact {ref:.6f}")
+        assert abs(m - ref) <= NSIG * se + 1e-12, (name, m, ref, se)
+    assert np.all(b[:, 3 + nocc:7] == 0.0)
     eng.clear_samples()
     eng.run_passes("semigrand_canonical", 1, 1)
     _, cnt = eng.read_samples()

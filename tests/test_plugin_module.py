@@ -184,6 +184,54 @@ def test_plugin_nfold_equals_python_nfold(flat, tmp_path):
     assert np.array_equal(np.array(state.configuration.occupation), pc.engine.read_state(0)["occ"])
 
 
+def test_plugin_nfold_ternary_equals_python_nfold(tmp_path):
+    """As above for a ternary expansion (HCP, two composition axes): the C++
+    and the Python N-fold calculators, fed the moments of every occupant
+    count, agree on every mean and analysis value to rounding."""
+    print("[case] plugin nfold vs python nfold, hcp ternary, same seed", flush=True)
+    from casmcode_clexmonte_amd.calculator import MonteCalculator as PyCalc, MonteCarloState, System
+    from casmcode_clexmonte_amd.system import FlatCE
+    from casmcode_clexmonte_amd.workloads import hcp_ternary_ce
+    ce = hcp_ternary_ce()
+    flat3 = FlatCE(ce)
+    dims = (6, 6, 4)
+    occ = np.random.default_rng(8).integers(0, 3, 2 * 6 * 6 * 4).astype(np.int8)
+    cond = {"temperature": 1400.0, "param_chem_pot": [0.03, -0.02]}
+    pc = PyCalc("nfold", System(ce))
+    py = pc.run_multistate([MonteCarloState(dims, occ.copy(), dict(cond))],
+                           pc.make_default_sampling_fixture_params(n_equilibration_passes=0, n_passes=300))[0]
+    system = P.system_from_flat(flat3, str(tmp_path / "system3.flat"))
+    calc = P.MonteCalculator("nfold", system)
+    state = P.make_state(dims, occ.copy(), cond)
+    rm = P.RunManager()
+    rm.seed = 0
+    params = P.SamplingFixtureParams()
+    params.n_equilibration_passes = 0
+    params.n_passes = 300
+    rm.params = params
+    calc.run(state, P.OccLocation(), rm)
+    res = rm.results
+    st, an = py["statistics"], py["analysis"]
+    comps = st["mol_composition"]["component_names"]
+    axes = st["param_composition"]["component_names"]
+    assert len(axes) == 2 and len(comps) == 3
+    pairs = [(res.statistics["clex.formation_energy(0)"][0], st["clex.formation_energy"]["value"]["mean"][0]),
+             (res.statistics["potential_energy(0)"][0], st["potential_energy"]["value"]["mean"][0]),
+             (res.analysis["heat_capacity"][0], an["heat_capacity"]["value"][0])]
+    for i, k in enumerate(axes):
+        pairs.append((res.statistics[f"param_composition({i})"][0], st["param_composition"][k]["mean"][0]))
+        pairs.append((res.analysis["param_thermochem_susc"][i], an["param_thermochem_susc"][f"S,{k}"][0]))
+        for j, q in enumerate(axes):
+            pairs.append((res.analysis["param_susc"][j * 2 + i], an["param_susc"][f"{k},{q}"][0]))
+    for i, k in enumerate(comps):
+        pairs.append((res.statistics[f"mol_composition({i})"][0], st["mol_composition"][k]["mean"][0]))
+        for j, q in enumerate(comps):
+            pairs.append((res.analysis["mol_susc"][j * 3 + i], an["mol_susc"][f"{k},{q}"][0]))
+    for c, p in pairs:
+        assert c == pytest.approx(p, rel=1e-9, abs=1e-12), pairs
+    assert np.array_equal(np.array(state.configuration.occupation), pc.engine.read_state(0)["occ"])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", ["canonical", "semigrand_canonical"])
 def test_plugin_corr_sampler(flat, tmp_path, method):
