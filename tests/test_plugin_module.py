@@ -184,6 +184,7 @@ def test_plugin_nfold_equals_python_nfold(flat, tmp_path):
     assert np.array_equal(np.array(state.configuration.occupation), pc.engine.read_state(0)["occ"])
 
 
+@pytest.mark.gpu
 def test_plugin_nfold_ternary_equals_python_nfold(tmp_path):
     """As above for a ternary expansion (HCP, two composition axes): the C++
     and the Python N-fold calculators, fed the moments of every occupant
