@@ -30,9 +30,8 @@ struct LaneRng {
       ++draw;
       left = 4;
     }
-    uint32_t x = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
     --left;
-    return u01(x);
+    return u01(word(left));
   }
   // one 32-bit word of the stream (acceptance tests)
   __device__ __forceinline__ uint32_t next_u32() {
@@ -41,9 +40,16 @@ struct LaneRng {
       ++draw;
       left = 4;
     }
-    uint32_t x = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
     --left;
-    return x;
+    return word(left);
+  }
+  // word 3 - r of the block (r = words left after this one): selects, not
+  // branches, so lanes whose counters differ stay converged
+  __device__ __forceinline__ uint32_t word(int r) const {
+    const uint32_t b0 = 0u - (uint32_t)(r & 1), b1 = 0u - (uint32_t)((r >> 1) & 1);
+    const uint32_t lo = (buf.z & b0) | (buf.w & ~b0);  // r = 1 : 0
+    const uint32_t hi = (buf.x & b0) | (buf.y & ~b0);  // r = 3 : 2
+    return (hi & b1) | (lo & ~b1);
   }
 };
 
@@ -428,12 +434,30 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
       img.w[wd] = word;
     }
   } else {
+    // batches of 8 global loads in flight per lane before their LDS stores
+    // (one load at a time left a chain of global-memory latencies per lane)
+    constexpr int kBatch = 8;
     PadWalk w(tid, nthr, P1, P2, P3);
-    for (int p = tid; p < pad_bytes; p += nthr, w.step()) img.p[p] = (uint8_t)walk_occ(w);
+    for (int p0 = tid; p0 < pad_bytes; p0 += kBatch * nthr) {
+      int v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u, w.step()) v[u] = p0 + u * nthr < pad_bytes ? walk_occ(w) : 0;
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u)
+        if (p0 + u * nthr < pad_bytes) img.p[p0 + u * nthr] = (uint8_t)v[u];
+    }
   }
   if (!BIN) {
     const float *__restrict__ tg = sys->flower_tables;
-    for (int p = tid; p < n_tab; p += nthr) tabL[p] = tg[p];
+    constexpr int kBatch = 8;
+    for (int p0 = tid; p0 < n_tab; p0 += kBatch * nthr) {
+      float v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) v[u] = p0 + u * nthr < n_tab ? tg[p0 + u * nthr] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u)
+        if (p0 + u * nthr < n_tab) tabL[p0 + u * nthr] = v[u];
+    }
   }
   if (tid == 0) {
     *maxDraw = 0ull;
