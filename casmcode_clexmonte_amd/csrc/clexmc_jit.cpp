@@ -52,6 +52,14 @@ static std::vector<int> split_slots(const BinSub &B, const std::vector<int> &bin
   return part;
 }
 
+// `base` = pa + minoff with every read at base + d, d >= 0: an empty asm
+// statement makes base opaque, so the compiler cannot fold minoff back into
+// the offsets (negative offsets do not fit the LDS instructions' unsigned
+// immediate and cost one address add per read)
+static std::string base_line(int minoff) {
+  return "      int base = pa + (" + std::to_string(minoff) + ");\n      asm volatile(\"\" : \"+v\"(base));\n";
+}
+
 static std::string hexf(float v) {
   char buf[64];
   std::snprintf(buf, sizeof(buf), "%af", (double)v);
@@ -79,7 +87,7 @@ static void emit_field(std::ostringstream &o, const std::string &name, int n_sub
       for (int e = 0; e < n * ko; ++e) minoff = std::min(minoff, bin_off[B.slot_begin[j] + e]);
     }
     // base + (off - minoff) == pa + off, with every immediate offset >= 0
-    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    o << "    if (b == " << b << ") {\n" << base_line(minoff);
     for (int j = 0; j < n_slots; ++j) {
       const int n = B.slot_n[j], ko = B.slot_k[j];
       if (n <= 0) continue;
@@ -114,7 +122,7 @@ static void emit_field(std::ostringstream &o, const std::string &name, int n_sub
     int minoff = 0;
     for (int j = 0; j < n_slots; ++j)
       for (int e = 0; e < B.slot_n[j] * B.slot_k[j]; ++e) minoff = std::min(minoff, bin_off[B.slot_begin[j] + e]);
-    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    o << "    if (b == " << b << ") {\n" << base_line(minoff);
     const std::vector<int> slot_part = split_slots(B, bin_off, n_slots);
     for (int part = 0; part < 2; ++part) {
       o << "      if (part == " << part << ") {\n";
@@ -174,7 +182,7 @@ static void emit_field(std::ostringstream &o, const std::string &name, int n_sub
         for (int i = 0; i < k; ++i, q += 2) minoff = std::min(minoff, gen_prog[q]);
       }
     }
-    o << "    if (b == " << b << ") {\n      const int base = pa + (" << minoff << ");\n";
+    o << "    if (b == " << b << ") {\n" << base_line(minoff);
     if (packed) {
       // 2-bit image, 16 sites per word: a read at base + d takes word
       // (base >> 4) + (d >> 4) + carry and bit field ((base & 15) + (d & 15)) & 15,
