@@ -47,7 +47,7 @@ LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # VALU lane-ops per attempt of the specialised kernel on this workload
 # (SQ_INSTS_VALU x 64 / attempts, profiles/metropolis_specialized_fcc24_sgc256.md)
-VALU_OPS_PER_ATTEMPT = 267
+VALU_OPS_PER_ATTEMPT = 253
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
