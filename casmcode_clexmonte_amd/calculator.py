@@ -69,6 +69,9 @@ class SamplingFixtureParams:
     # (the configuration at every sample)
     write_observations: bool = False
     write_trajectory: bool = False
+    # "pass", or "time" (the nfold method only): samples every sample_period
+    # units of N-fold time (passes: expected Metropolis steps per mutable site)
+    sample_by: str = "pass"
 
     @classmethod
     def from_casm(cls, label: str, d: dict, **kw) -> "SamplingFixtureParams":
@@ -76,8 +79,9 @@ class SamplingFixtureParams:
         linear ``period``, ``quantities``; ``completion_check``), as in the
         reference's run params (``sampling_fixtures.<label>``)."""
         smp = d.get("sampling", {})
-        if smp.get("sample_by", "pass") != "pass" or smp.get("spacing", "linear") != "linear":
-            raise ValueError("only linear sampling by pass is supported")
+        if smp.get("sample_by", "pass") not in ("pass", "time") or smp.get("spacing", "linear") != "linear":
+            raise ValueError("only linear sampling by pass or by time is supported")
+        kw.setdefault("sample_by", smp.get("sample_by", "pass"))
         names = []
         legacy = {"formation_energy": "clex.formation_energy", "formation_energy_corr": "corr.formation_energy"}
         for q in smp.get("quantities", []):
@@ -87,8 +91,9 @@ class SamplingFixtureParams:
         kw.setdefault("write_observations", bool(rkw.get("write_observations", False)))
         kw.setdefault("write_trajectory", bool(rkw.get("write_trajectory", False)) or
                       bool(smp.get("sample_trajectory", False)))
+        per = smp.get("period", 1)
         return cls(label=label, sampler_names=names or cls().sampler_names,
-                   sample_period=int(smp.get("period", 1)), n_equilibration_passes=0,
+                   sample_period=float(per) if kw["sample_by"] == "time" else int(per), n_equilibration_passes=0,
                    completion=CompletionCheckParams.from_casm(d.get("completion_check", {})), **kw)
 
     def wants_correlations(self) -> bool:
@@ -205,6 +210,8 @@ class MonteCalculator:
         final occupations are written back into the states."""
         if self.method == "nfold":
             return self._run_nfold(states, p)
+        if p.sample_by != "pass":
+            raise ValueError("Error in MonteCalculator: sampling by time needs the nfold method")
         self._load(states, p.n_pair_phases)
         kind = self.method
         t0 = time.perf_counter()
@@ -321,6 +328,26 @@ class MonteCalculator:
         n_cells = int(np.prod(next(iter(shapes))))
         n_var = n_cells * sum(1 for b in range(flat.n_sublat) if flat.n_occ[b] >= 2)
         att0 = [self.engine.read_state(c)["n_attempt"] for c in range(len(states))]
+        if p.sample_by == "time":
+            # samples at times 0, period, ... up to n_passes; every rate is at
+            # most 1, so a state lasts at least 1/n_var passes and
+            # n_var x (n_passes + period + 1) events always suffice
+            n_samples = max(1, int(round(p.n_passes / float(p.sample_period))))
+            max_events = int(n_var * (n_samples * float(p.sample_period) + 1.0)) + 1
+            if max_events >= 2 ** 31:
+                raise ValueError("Error in MonteCalculator: n_passes x mutable sites must lie below 2^31")
+            self.engine.clear_samples()
+            self.engine.nfold_run_time(n_samples, float(p.sample_period), max_events)
+            elapsed = time.perf_counter() - t0
+            E, cnt = self.engine.read_samples()
+            out = []
+            for c, s in enumerate(states):
+                st = self.engine.read_state(c)
+                s.occupation = st["occ"]
+                d = self._summary(s, E[c], cnt[c], st["n_attempt"] - att0[c], st["n_attempt"] - att0[c], elapsed)
+                d["completion_check_results"]["N_samples"] = [int(E.shape[1])]
+                out.append(d)
+            return out
         blocks = self.engine.nfold_run_moments(p.n_passes * n_var, n_var)
         elapsed = time.perf_counter() - t0
         out = []

@@ -1429,8 +1429,11 @@ static clexmc_status nfold_check_stalls(clexmc_handle *h) {
   return CLEXMC_ERR_STATE;
 }
 
+static clexmc_status grow_samples(clexmc_handle *h, long long n_new_samples);
+
 // The N-fold run; blocks_out receives kNfoldBlockSums moments per block.
-static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {
+static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out,
+                                    int n_time_samples = 0, double sample_dt = 0.0) {
   ARG_CHECK(h->n_chains > 0, "no chains loaded");
   ARG_CHECK(n_events >= 0 && events_per_block >= 1 && n_events % events_per_block == 0,
             "n_events must be a non-negative multiple of events_per_block");
@@ -1460,6 +1463,19 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
   na.n_events = n_events;
   na.events_per_block = events_per_block;
   na.n_blocks = n_events / events_per_block;
+  std::vector<int> count0;
+  if (n_time_samples > 0) {
+    const clexmc_status gs = grow_samples(h, n_time_samples);
+    if (gs != CLEXMC_OK) return gs;
+    count0.resize(h->n_chains);
+    HIP_TRY(hipMemcpy(count0.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
+    na.a.sampE = h->d_sampE;
+    na.a.sampN = h->d_sampN;
+    na.a.sample_count = h->d_sample_count;
+    na.a.sample_cap = h->sample_cap;
+    na.n_time_samples = n_time_samples;
+    na.sample_dt = sample_dt;
+  }
   const size_t nb = (size_t)h->n_chains * na.n_blocks * kNfoldBlockSums;
   double *d_blocks = nullptr;
   HIP_TRY(hipMalloc(&d_blocks, nb * sizeof(double)));
@@ -1494,7 +1510,25 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_blocks);
   HIP_TRY(e);
-  return nfold_check_stalls(h);
+  const clexmc_status st = nfold_check_stalls(h);
+  if (st != CLEXMC_OK || n_time_samples <= 0) return st;
+  std::vector<int> count1(h->n_chains);
+  HIP_TRY(hipMemcpy(count1.data(), h->d_sample_count, sizeof(int) * h->n_chains, hipMemcpyDeviceToHost));
+  for (int c = 0; c < h->n_chains; ++c)
+    if (count1[c] - count0[c] != n_time_samples) {
+      h->err = "N-fold time sampling: chain " + std::to_string(c) + " took " + std::to_string(count1[c] - count0[c]) +
+               " of " + std::to_string(n_time_samples) + " samples within " + std::to_string(n_events) + " events";
+      return CLEXMC_ERR_STATE;
+    }
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_nfold_run_time(clexmc_handle *h, int n_samples, double period, int max_events) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(n_samples >= 1 && period > 0.0 && max_events >= 1,
+            "n_samples and max_events must be positive and the period positive");
+  std::vector<double> blk((size_t)h->n_chains * kNfoldBlockSums);
+  return nfold_run_impl(h, max_events, max_events, blk.data(), n_samples, period);
 }
 
 clexmc_status clexmc_nfold_run_moments(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out) {

@@ -136,6 +136,12 @@ struct NfoldArgs {
   int events_per_block;    // events per output record
   int n_blocks;            // n_events / events_per_block
   double *blocks;          // [n_chains][n_blocks][kNfoldBlockSums]
+  // sampling by N-fold time: n_time_samples > 0 samples per chain at times
+  // 0, sample_dt, 2 sample_dt, ... (passes: expected Metropolis steps per
+  // mutable site) into the sample storage of RunArgs; the chain stops after
+  // its last sample (n_events is then an upper bound)
+  int n_time_samples;
+  double sample_dt;
 };
 
 // clexmc_delta_energy_batch_noalloc: proposals per call, and the layout of

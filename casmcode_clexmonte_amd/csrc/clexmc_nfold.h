@@ -192,6 +192,14 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     for (int a = 0; a < args.n_orbits; ++a) E0 += sys->bin_w64[a] * (double)C0[a];
   }
   const unsigned long long ev0 = cs->pass_base;
+  // sampling by N-fold time (na.n_time_samples > 0): samples at times
+  // 0, dt, 2 dt, ... (passes since the launch) of the state occupied then
+  const int n_tsamp = na.n_time_samples;
+  // (the sample storage is only passed in for time sampling)
+  const int s_base = n_tsamp > 0 ? args.sample_count[chain] : 0;
+  double T_now = 0.0;
+  int n_rec = 0;
+  if (tid == 0) sel[3] = 0;
 
   for (int ev = 0; ev < na.n_events; ++ev) {
     // ---- select an event (wavefront 0)
@@ -239,6 +247,20 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
           else
             E = E_run;
           const double w = n_mutable / (double)R;
+          // sampling by N-fold time: every sample time in this state's
+          // residence interval [T, T + w / n_mutable) (in passes) samples it
+          if (n_tsamp > 0) {
+            const double t1 = T_now + w / n_mutable;
+            while (n_rec < n_tsamp && (double)n_rec * na.sample_dt < t1) {
+              const long long so = (long long)chain * args.sample_cap + (s_base + n_rec);
+              args.sampE[so] = E;
+#pragma unroll
+              for (int q = 0; q < kMaxOcc; ++q) args.sampN[so * kMaxOcc + q] = (int)cnt[q];
+              ++n_rec;
+            }
+            T_now = t1;
+            if (n_rec >= n_tsamp) sel[3] = 1;  // every sample taken: stop before this event
+          }
           mom[0] += w;
           mom[1] += w * E;
           mom[2] += w * E * E;
@@ -261,6 +283,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       if (tid == 0) cs->flags |= kChainFlagNfoldStalled;
       break;
     }
+    if (sel[3]) break;  // time sampling complete (uniform across the workgroup)
 
     // ---- apply the change (one lane); binary: its cluster sums K come with
     // its new rate below; general: the new occupant in proportion to its rate
@@ -376,6 +399,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       cs->E = E_run;
     }
     for (int q = 0; q < kMaxOcc; ++q) cs->counts[q] = cnt[q];
+    if (n_tsamp > 0) args.sample_count[chain] = s_base + n_rec;
     cs->n_attempt += n_done;
     cs->n_accept += n_done;
     cs->pass_base = ev0 + (unsigned long long)na.n_events;

@@ -78,7 +78,8 @@ def lib():
                      "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
                      "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
-                     "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments"]:
+                     "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
+                     "clexmc_nfold_run_time"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -103,7 +104,8 @@ def exported_symbols() -> List[str]:
             "clexmc_graph_release", "clexmc_autotune", "clexmc_set_specialization",
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
             "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
-            "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments"]
+            "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
+            "clexmc_nfold_run_time"]
 
 
 # layout of clexmc_nfold_run_moments' records
@@ -502,6 +504,17 @@ class Engine:
         self._check(lib().clexmc_nfold_run_moments(self.h, C.c_int(n_events), C.c_int(events_per_block),
                                                    _p(out, f64p)), "clexmc_nfold_run_moments")
         return out
+
+    def nfold_run_time(self, n_samples: int, period: float, max_events: int):
+        """N-fold way sampled by time: every chain records n_samples samples
+        (energy per supercell and occupant counts, read with read_samples)
+        at times 0, period, 2 period, ... in passes (expected Metropolis steps
+        per mutable site), each the state occupied at that time; at most
+        max_events events per chain."""
+        _check_cint("n_samples", n_samples)
+        _check_cint("max_events", max_events)
+        self._check(lib().clexmc_nfold_run_time(self.h, C.c_int(n_samples), C.c_double(period), C.c_int(max_events)),
+                    "clexmc_nfold_run_time")
 
     def set_lds_row_pitch(self, extra: int = -1):
         """Padded-image row pitch of supercells added later: -1 automatic

@@ -345,6 +345,19 @@ clexmc_status clexmc_nfold_run(clexmc_handle *h, int n_events, int events_per_bl
  * 8 (2,3), 9 (3,3)), [17 + o] sum w E N_o; N_o the sites holding occupant
  * o (0 for occupants the system does not have). */
 clexmc_status clexmc_nfold_run_moments(clexmc_handle *h, int n_events, int events_per_block, double *blocks_out);
+/* N-fold way sampled by time (the reference's nfold method allows
+ * time-based sampling, include/casm/clexmonte/nfold/nfold.hh:22-23): every
+ * chain records n_samples samples into its sample storage (read with
+ * clexmc_read_samples: the energy per supercell and the occupant counts) at
+ * times 0, period, 2 period, ... since the call, each the state the chain
+ * occupies at that time, and stops after its last sample.  Time advances by
+ * each visited state's expected residence time 1/R in passes (expected
+ * Metropolis steps per mutable site; R the total rate), so the samples are
+ * Boltzmann-distributed states with equal weights.  max_events bounds the
+ * events per chain; a chain that takes fewer samples (or whose rates all
+ * vanish) makes the call fail.  Since every rate is at most 1, n_mutable x
+ * (n_samples x period + 1) events always suffice. */
+clexmc_status clexmc_nfold_run_time(clexmc_handle *h, int n_samples, double period, int max_events);
 
 /* Run-time specialisation (hipRTC, gfx950) of the binary Metropolis kernel
  * to the loaded supercell: constant site offsets instead of interpreted site

@@ -243,6 +243,45 @@ def test_python_nfold_ternary_matches_sgc():
         assert g["completion_check_results"]["count"][0] == 2000 * n
 
 
+@pytest.mark.gpu
+def test_python_nfold_time_sampling_matches_sgc(system):
+    """MonteCalculator("nfold") sampled by time (sample_by "time", one sample
+    per 0.5 passes of N-fold time): formation energy and composition means
+    agree with the semi-grand canonical calculator within 4 combined standard
+    errors, and the sample count is n_passes / period."""
+    print("[case] python calculator nfold sampled by time vs sgc, fcc binary 8^3", flush=True)
+    from casmcode_clexmonte_amd.calculator import SamplingFixtureParams
+    mus = [-0.04, 0.05]
+
+    def states():
+        return [MonteCarloState((8, 8, 8), random_occupation(512, 0.5, 60 + i, exact=False),
+                                {"temperature": 900.0, "param_chem_pot": [mu]}) for i, mu in enumerate(mus)]
+    sg = MonteCalculator("semigrand_canonical", system)
+    ref = sg.run_multistate(states(), sg.make_default_sampling_fixture_params(n_equilibration_passes=200,
+                                                                              n_passes=3000))
+    nf = MonteCalculator("nfold", system)
+    p = SamplingFixtureParams(n_equilibration_passes=200, n_passes=1500, sample_period=0.5, sample_by="time")
+    got = nf.run_multistate(states(), p)
+    validate_summary(merge_summaries(got), len(mus), False)
+    for r, g in zip(ref, got):
+        assert g["completion_check_results"]["N_samples"] == [3000]
+        for sec, key in (("clex.formation_energy", "value"), ("param_composition", "a")):
+            a, b = r["statistics"][sec][key], g["statistics"][sec][key]
+            se = np.hypot(a["calculated_precision"][0], b["calculated_precision"][0]) / 2.0
+            print(sec, a["mean"][0], b["mean"][0], se)
+            assert abs(a["mean"][0] - b["mean"][0]) < 4 * se + 1e-6, (sec, a, b)
+
+
+def test_fixture_sample_by_time():
+    """CASM sampling fixtures may sample by time (the nfold method)."""
+    from casmcode_clexmonte_amd.calculator import SamplingFixtureParams
+    p = SamplingFixtureParams.from_casm("thermo", {"sampling": {"sample_by": "time", "period": 0.25,
+                                                                "quantities": ["formation_energy"]}})
+    assert p.sample_by == "time" and p.sample_period == 0.25
+    with pytest.raises(ValueError):
+        SamplingFixtureParams.from_casm("thermo", {"sampling": {"sample_by": "step"}})
+
+
 def test_fixture_legacy_quantity_names():
     """CASM run params name the samplers formation_energy / formation_energy_corr."""
     from casmcode_clexmonte_amd.calculator import SamplingFixtureParams

@@ -232,3 +232,60 @@ def test_nfold_specialized_matches_generic(system):
     assert np.array_equal(out[1][0], out[0][0])
     for a, b in zip(out[1][1], out[0][1]):
         assert np.array_equal(a["occ"], b["occ"]) and a["E"] == b["E"]
+
+
+@pytest.mark.parametrize("system", ["fcc_binary", "hcp_ternary"])
+def test_nfold_time_sampling_exact_enumeration(system):
+    """Sampling by N-fold time (clexmc_nfold_run_time): the states occupied
+    at times 0, dt, 2 dt, ... are Boltzmann-distributed with equal weights,
+    so plain sample means of E, E^2 and every occupant count (and its
+    square) over 128 chains lie within 3 sigma of exact enumeration (FCC
+    binary 2x2x4, T 1200 K; ternary HCP 2x2x1, T 1000 K).  The chain stops
+    at its last sample: that sample's occupant counts are the final state's
+    and its energy the final energy (1e-9: the binary path sums the exact
+    counts in another order at the end)."""
+    _print_case(f"nfold time sampling exact enumeration {system}")
+    if system == "fcc_binary":
+        flat = small_flat()
+        sc = Supercell(flat.ce.prim, (2, 2, 4))
+        T, chem, nocc = 1200.0, [0.0, 0.03], 2
+    else:
+        flat = _hcp_ternary_small()
+        sc = Supercell(flat.ce.prim, (2, 2, 1))
+        T, nocc = 1000.0, 3
+        chem = [0.0, 0.02, -0.03] * flat.n_sublat
+    osys = O.OracleSystem(flat, sc.dims)
+    ex = osys.enumerate("sgc", T, chem=chem, want_probs=True)
+    pr, Es = ex["probs"], osys.state_energies()
+    st_i = np.arange(len(pr))
+    occ_s = (st_i[:, None] // (nocc ** np.arange(sc.n_sites))[None, :]) % nocc
+    Ns = np.stack([(occ_s == o).sum(1) for o in range(nocc)], axis=1).astype(float)
+    nch = 128
+    eng = Engine(flat)
+    sid = eng.add_supercell(sc, n_pair_phases=0)
+    rng = np.random.default_rng(21)
+    occs = [rng.integers(0, nocc, sc.n_sites).astype(np.int8) for _ in range(nch)]
+    eng.load_chains([sid] * nch, occs, [T] * nch, 4000 + np.arange(nch), chem=[chem] * nch)
+    eng.nfold_run(2000, 500)  # equilibration
+    eng.clear_samples()
+    n_samples, dt = 3000, 0.25
+    eng.nfold_run_time(n_samples, dt, int(sc.n_sites * (n_samples * dt + 1)) + 1)
+    E, cnt = eng.read_samples()
+    assert E.shape == (nch, n_samples)
+    checks = [("E", E, pr @ Es), ("E2", E * E, pr @ (Es * Es))]
+    for o in range(nocc):
+        n_o = cnt[:, :, o].astype(float)
+        checks.append((f"N{o}", n_o, pr @ Ns[:, o]))
+        checks.append((f"N{o}^2", n_o * n_o, pr @ (Ns[:, o] ** 2)))
+    for name, x, ref in checks:
+        per = x.mean(axis=1)
+        m, se = per.mean(), per.std(ddof=1) / np.sqrt(nch)
+        print(f"{name}: time-sampled {m:.6f} +- {se:.6f} exact {ref:.6f}")
+        assert abs(m - ref) <= NSIG * se + 1e-12, (name, m, ref, se)
+    for c in range(0, nch, 8):
+        st = eng.read_state(c)
+        assert abs(E[c, -1] - st["E"]) <= 1e-9 * max(1.0, abs(st["E"])), (E[c, -1], st["E"])
+        nb = np.bincount(st["occ"], minlength=4)
+        assert np.array_equal(cnt[c, -1, :4], nb[:4])
+    eng.close()
+
