@@ -41,6 +41,11 @@ struct HostSupercell {
   std::vector<hipModule_t> jit_mods;
   hipFunction_t jit_fn[3][2][2][2] = {};  // [kind][packed][lanes][block above 512]
   hipFunction_t jit_nfold = nullptr;  // N-fold kernel of the SGC (unpacked, one-lane) module
+  hipFunction_t jit_cube[2] = {nullptr, nullptr};  // register-cube kernel of that module [block above 512]
+  // register-cube sweep (binary, one sublattice, classes = the residues of
+  // the cells mod 3 listed in block order): eligible, and its layout
+  bool cube_ok = false;
+  CubeSpec cube{};
 };
 
 }  // namespace
@@ -97,8 +102,9 @@ struct clexmc_handle {
   int block_size = 0;
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
-  int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1, last_fused = 0;
+  int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1, last_fused = 0, last_cube = 0;
   int fuse = 1;  // chains on several supercell shapes: one fused launch (1) or one launch per shape (0)
+  int cube = 1;  // register-cube sweep where the specialised kernel has one (1) or the class loop (0)
   int slabs = 0;  // slabs per chain: 0 automatic, 1 off, >= 2 requested
   SlabRecord *d_slab_rec = nullptr;  // [n_chains][kSlabMax][kSlabRecords] (slab launches)
   int slab_rec_chains = 0;
@@ -993,6 +999,43 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
   HIP_TRY(upload(h, A, pair_sub.data(), pair_sub.size(), (int **)&d.pair_sublat));
   HIP_TRY(upload(h, A, pair_halo_begin.data(), pair_halo_begin.size(), (int **)&d.pair_halo_begin));
   HIP_TRY(upload(h, A, pair_halo.data(), pair_halo.size(), (int2 **)&d.pair_halo));
+  {
+    // register-cube eligibility: every class the cells of one residue
+    // (x, y, z) mod 3, site t of a class in block t = bi + n1 (bj + n2 bk)
+    const int n1 = L1 / 3, n2 = L2 / 3, n3 = L3 / 3;
+    bool ok = h->binary && h->n_sublat == 1 && sc->n_classes == 27 && L1 % 3 == 0 && L2 % 3 == 0 &&
+              L3 % 3 == 0 && n1 * n2 * n3 <= kMaxJitThreads;
+    CubeSpec cs{};
+    bool seen[27] = {};
+    for (int c = 0; c < 27 && ok; ++c) {
+      if (cb[c + 1] - cb[c] != n1 * n2 * n3) {
+        ok = false;
+        break;
+      }
+      const int l0 = sc->class_sites[cb[c]];
+      const int r1 = l0 % L1 % 3, r2 = (l0 / L1) % L2 % 3, r3 = l0 / (L1 * L2) % 3;
+      if (seen[r1 + 3 * (r2 + 3 * r3)]) ok = false;
+      seen[r1 + 3 * (r2 + 3 * r3)] = true;
+      cs.res[c][0] = r1;
+      cs.res[c][1] = r2;
+      cs.res[c][2] = r3;
+      for (int t = 0; t < n1 * n2 * n3 && ok; ++t) {
+        const int bi = t % n1, bj = (t / n1) % n2, bk = t / (n1 * n2);
+        const int want = (3 * bi + r1) + L1 * ((3 * bj + r2) + L2 * (3 * bk + r3));
+        if (sc->class_sites[cb[c] + t] != want) ok = false;
+      }
+    }
+    if (ok) {
+      cs.nb[0] = n1;
+      cs.nb[1] = n2;
+      cs.nb[2] = n3;
+      cs.h = hw;
+      cs.P1 = d.P1;
+      cs.P12 = d.P1 * d.P2;
+      hs.cube = cs;
+    }
+    hs.cube_ok = ok;
+  }
   h->scs.push_back(std::move(hs));
   h->scs_dirty = true;
   *id_out = (int)h->scs.size() - 1;
@@ -1302,6 +1345,19 @@ clexmc_status clexmc_set_slabs(clexmc_handle *h, int slabs) {
 clexmc_status clexmc_last_run_slabs(clexmc_handle *h, int *slabs) {
   if (!h || !slabs) return CLEXMC_ERR_ARG;
   *slabs = h->last_slabs;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_set_register_cube(clexmc_handle *h, int mode) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(mode == 0 || mode == 1, "register cube mode must be 0 (class loop) or 1 (where available)");
+  h->cube = mode;
+  return CLEXMC_OK;
+}
+
+clexmc_status clexmc_last_run_cube(clexmc_handle *h, int *cube) {
+  if (!h || !cube) return CLEXMC_ERR_ARG;
+  *cube = h->last_cube;
   return CLEXMC_OK;
 }
 
@@ -1615,6 +1671,7 @@ struct RunPlan {
   bool one_shape = false;       // every chain of the launch on supercell sc_id
   int slabs = 1;                // workgroups per chain (slab decomposition along k)
   bool threads_auto = false;    // block size chosen by the planner (not requested)
+  bool cube = false;            // jit is the register-cube kernel
   unsigned long long pass_start = 0;  // the chains' pass counter before the run
   RunArgs a;
   LearnArgs la;
@@ -1634,7 +1691,8 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
     std::vector<int> variable(h->n_sublat);
     for (int b = 0; b < h->n_sublat; ++b) variable[b] = h->n_occ[b] >= 2;
     const std::string src = jit_source(h->n_sublat, variable, h->binary != 0, hs.bin_sub, hs.bin_off, h->n_slots,
-                                       hs.gen_sub, hs.gen_prog, kind, packed, lanes, w ? kMaxJitThreads : kAotMaxThreads);
+                                       hs.gen_sub, hs.gen_prog, kind, packed, lanes, w ? kMaxJitThreads : kAotMaxThreads,
+                                       hs.cube_ok ? &hs.cube : nullptr);
     hipModule_t mod = nullptr;
     const hipError_t je = jit_compile_load(src, &mod, &h->jit_log);
     if (je != hipSuccess) h->jit_log += std::string("\n[compile/load: ") + hipGetErrorString(je) + "]";
@@ -1644,6 +1702,9 @@ static hipFunction_t jit_kernel(clexmc_handle *h, int sc_id, int kind, bool pack
         state = 1;
         if (k == 1 && p == 0 && l == 0 && hipModuleGetFunction(&hs.jit_nfold, mod, kJitNfoldName) != hipSuccess)
           hs.jit_nfold = nullptr;
+        if (k == 1 && p == 0 && l == 0 && hs.cube_ok &&
+            hipModuleGetFunction(&hs.jit_cube[w], mod, kJitCubeName) != hipSuccess)
+          hs.jit_cube[w] = nullptr;
       } else {
         h->jit_log += "\n[kernel lookup failed]";
         hipModuleUnload(mod);
@@ -1789,6 +1850,17 @@ static clexmc_status plan_run(clexmc_handle *h, int kind, int n_passes, int samp
       P.jit = jit_kernel(h, h->chain_sc[c0], kind, packed, P.lanes, P.threads);
   }
   if (!P.jit && !fused_spec) P.threads = std::min(P.threads, kAotMaxThreads);
+  // the register-cube sweep: the specialised semi-grand canonical kernel's
+  // module carries it, one lane per 3x3x3 block
+  P.cube = false;
+  if (P.jit && P.one_shape && kind == 1 && !packed && P.lanes == 1 && h->cube) {
+    const HostSupercell &hs = h->scs[P.sc_id];
+    const hipFunction_t cf = hs.jit_cube[P.threads > kAotMaxThreads ? 1 : 0];
+    if (hs.cube_ok && cf && P.threads == hs.cube.nb[0] * hs.cube.nb[1] * hs.cube.nb[2]) {
+      P.jit = cf;
+      P.cube = true;
+    }
+  }
   return CLEXMC_OK;
 }
 
@@ -1799,6 +1871,7 @@ static void drop_jit(clexmc_handle *h, RunPlan &P, int sc_id) {
   (void)hipGetLastError();
   h->scs[sc_id].jit_lds_fail = P.lds;
   P.jit = nullptr;
+  P.cube = false;
   P.threads = std::min(P.threads, kAotMaxThreads);
 }
 
@@ -1890,6 +1963,12 @@ static hipError_t choose_slabs(clexmc_handle *h, const std::vector<RunPlan *> &p
       S = (int)std::min<long long>(cap, (d.n_sites + per_wg / 2) / per_wg);
     }
     if (S < 2) continue;
+    if (P->cube) {
+      // the register-cube kernel runs whole chains only: the class loop's
+      // kernel for slabs
+      P->jit = jit_kernel(h, P->sc_id, P->kind, P->packed, P->lanes, P->threads);
+      P->cube = false;
+    }
     if (!h->d_slab_rec || h->slab_rec_chains < h->n_chains) {
       if (h->d_slab_rec) (void)hipFree(h->d_slab_rec);
       h->d_slab_rec = nullptr;
@@ -2051,6 +2130,7 @@ static bool run_fused(clexmc_handle *h, int kind, int n_passes, int sample_perio
   h->last_lanes = 1;
   h->last_slabs = max_slabs;
   h->last_fused = 1;
+  h->last_cube = 0;
   return true;
 }
 
@@ -2064,6 +2144,7 @@ static clexmc_status run_grouped(clexmc_handle *h, int kind, int n_passes, int s
     if (h->fuse && run_fused(h, kind, n_passes, sample_period, canonical_rounds, &st)) return st;
   }
   h->last_fused = 0;
+  h->last_cube = 0;
   const int ng = (int)h->groups.size();
   std::vector<RunPlan> plans(ng);
   for (int g = 0; g < ng; ++g) {
@@ -2202,6 +2283,7 @@ static clexmc_status run_passes_impl(clexmc_handle *h, int kind, int n_passes, i
   h->last_lanes = P.lanes;
   h->last_slabs = P.slabs;
   h->last_fused = 0;
+  h->last_cube = P.cube && P.slabs <= 1 ? 1 : 0;
   HIP_TRY(plan_attributes(P, h));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   hipError_t le = plan_launch(P, h);

@@ -4,6 +4,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -68,9 +69,13 @@ static std::string hexf(float v) {
 
 // The field policy struct `name` for one supercell's site programs (inside
 // namespace clexmc).
+static void emit_cube(std::ostringstream &o, const BinSub &B, const std::vector<int> &bin_off, int n_slots,
+                      const CubeSpec &cs);
+
 static void emit_field(std::ostringstream &o, const std::string &name, int n_sublat, const std::vector<int> &variable,
                        bool binary, const std::vector<BinSub> &subs, const std::vector<int> &bin_off, int n_slots,
-                       const std::vector<GenSub> &gsubs, const std::vector<int> &gen_prog, bool packed) {
+                       const std::vector<GenSub> &gsubs, const std::vector<int> &gen_prog, bool packed,
+                       const CubeSpec *cube = nullptr) {
   o << "struct " << name << " {\n"
     << "  template <class IMG>\n"
     << "  __device__ __forceinline__ static float bin(const FieldCtx &, const IMG &img, int b, int pa, int o,\n"
@@ -221,23 +226,136 @@ static void emit_field(std::ostringstream &o, const std::string &name, int n_sub
     }
     o << "      return;\n    }\n";
   }
-  o << "  }\n};\n";
+  o << "  }\n";
+  if (cube && binary && n_sublat == 1)
+    emit_cube(o, subs[0], bin_off, n_slots, *cube);
+  else
+    o << "  static constexpr bool kCube = false;\n  struct Cube {};\n";
+  o << "};\n";
+}
+
+// Cube cell q (x + S y + S^2 z, S = 3 + 2h) is bit q & 31 of register q >> 5
+// (binary occupants: one bit per cell, so the cube takes few registers).
+static std::string cube_bit(int q) {
+  return "__builtin_amdgcn_ubfe(cb.r[" + std::to_string(q >> 5) + "], " + std::to_string(q & 31) + "u, 1u)";
+}
+
+static void emit_cube(std::ostringstream &o, const BinSub &B, const std::vector<int> &bin_off, int n_slots,
+                      const CubeSpec &cs) {
+  const int h = cs.h, S = 3 + 2 * h, n_cells = S * S * S, nr = (n_cells + 31) / 32;
+  // cell offsets (dx, dy, dz) of the padded LDS offsets (|d| <= h)
+  auto decode = [&](int off, int &dx, int &dy, int &dz) {
+    dz = (int)std::lround((double)off / cs.P12);
+    const int r = off - dz * cs.P12;
+    dy = (int)std::lround((double)r / cs.P1);
+    dx = r - dy * cs.P1;
+  };
+  o << "  static constexpr bool kCube = true;\n"
+    << "  static constexpr int kCubeBlocks[3] = {" << cs.nb[0] << ", " << cs.nb[1] << ", " << cs.nb[2] << "};\n"
+    << "  struct Cube {\n    uint32_t r[" << nr << "];\n  };\n";
+  // load: every cube cell from the padded image (halos valid at launch start)
+  o << "  __device__ __forceinline__ static void cube_load(Cube &cb, const ImgU8 &img, int base) {\n"
+    << "    asm volatile(\"\" : \"+v\"(base));\n";
+  for (int w = 0; w < nr; ++w) {
+    o << "    cb.r[" << w << "] = 0u";
+    for (int t = 0; t < 32; ++t) {
+      const int q = 32 * w + t;
+      if (q >= n_cells) break;
+      const int x = q % S, y = (q / S) % S, z = q / (S * S);
+      o << " | ((uint32_t)img.p[base + " << (z * cs.P12 + y * cs.P1 + x) << "] << " << t << ")";
+    }
+    o << ";\n";
+  }
+  o << "  }\n";
+  // one pass over the 27 classes: per class the field of the lane's site from
+  // the cube, the proposal (site callback: draws, accepts, accumulates), the
+  // lane's own byte and LDS image updated, a barrier, then the other lanes'
+  // sites of the class read back into the cube.  ax/ay/az: padded
+  // coordinates (times pitch) of the block origins at offsets -1, 0, +1.
+  // Empty asm statements at each class start mark the cube and the block
+  // origins as changed: otherwise the compiler shares bit extractions and
+  // refresh addresses among classes (and hoists them out of the pass loop)
+  // and keeps hundreds of values alive, spilling to scratch memory.
+  o << "  template <class SITE>\n"
+    << "  __device__ __forceinline__ static void cube_pass(Cube &cb, const ImgU8 &img, const int (&ax)[3],\n"
+    << "                                                   const int (&ay)[3], const int (&az)[3], SITE &&site) {\n"
+    << "    int x0 = ax[0], x1 = ax[1], x2 = ax[2], y0 = ay[0], y1 = ay[1], y2 = ay[2], z0 = az[0], z1 = az[1],"
+    << " z2 = az[2];\n";
+  std::string fence = "      asm volatile(\"\" :";
+  for (int w = 0; w < nr; ++w) fence += std::string(w ? "," : "") + " \"+v\"(cb.r[" + std::to_string(w) + "])";
+  fence += ");\n      asm volatile(\"\" : \"+v\"(x0), \"+v\"(x1), \"+v\"(x2), \"+v\"(y0), \"+v\"(y1), "
+           "\"+v\"(y2), \"+v\"(z0), \"+v\"(z1), \"+v\"(z2));\n";
+  const char *xs[3] = {"x0", "x1", "x2"}, *ys[3] = {"y0", "y1", "y2"}, *zs[3] = {"z0", "z1", "z2"};
+  for (int c = 0; c < 27; ++c) {
+    const int r1 = cs.res[c][0], r2 = cs.res[c][1], r3 = cs.res[c][2];
+    const int P = (h + r1) + S * (h + r2) + S * S * (h + r3);
+    o << "    {\n" << fence << "      int K[kMaxSlots];\n";
+    for (int j = 0; j < kMaxSlots; ++j) o << "      K[" << j << "] = 0;\n";
+    o << "      float F = 0.0f;\n";
+    for (int j = 0; j < n_slots; ++j) {
+      const int n = B.slot_n[j], ko = B.slot_k[j];
+      if (n <= 0) continue;
+      o << "      {\n        int X = 0;\n";
+      for (int e = 0; e < n && ko > 0; ++e) {
+        o << "        X += ";
+        for (int i = 0; i < ko; ++i) {
+          int dx, dy, dz;
+          decode(bin_off[B.slot_begin[j] + e * ko + i], dx, dy, dz);
+          const int q = P + dx + S * dy + S * S * dz;
+          if (i) o << " ^ ";
+          o << "(int)" << cube_bit(q);
+        }
+        o << ";\n";
+      }
+      if (ko & 1)
+        o << "        const int Kj = 2 * X - " << n << ";\n";
+      else
+        o << "        const int Kj = " << n << " - 2 * X;\n";
+      o << "        K[" << j << "] = Kj;\n        F = __builtin_fmaf(" << hexf(B.slot_w[j]) << ", (float)Kj, F);\n"
+        << "      }\n";
+    }
+    const int own = r3 * cs.P12 + r2 * cs.P1 + r1;
+    o << "      const int o = (int)" << cube_bit(P) << ";\n"
+      << "      if (site(F, K, o)) {\n"
+      << "        cb.r[" << (P >> 5) << "] ^= " << (1u << (P & 31)) << "u;\n"
+      << "        img.set(z1 + y1 + x1 + " << own << ", o, 1 - o);\n"
+      << "      }\n      __syncthreads();\n";
+    for (int bk = -1; bk <= 1; ++bk)
+      for (int bj = -1; bj <= 1; ++bj)
+        for (int bi = -1; bi <= 1; ++bi) {
+          if (bi == 0 && bj == 0 && bk == 0) continue;
+          const int X = h + 3 * bi + r1, Y = h + 3 * bj + r2, Z = h + 3 * bk + r3;
+          if (X < 0 || X >= S || Y < 0 || Y >= S || Z < 0 || Z >= S) continue;
+          const int q = X + S * Y + S * S * Z;
+          o << "      cb.r[" << (q >> 5) << "] = (cb.r[" << (q >> 5) << "] & " << ~(1u << (q & 31))
+            << "u) | ((uint32_t)img.p[" << zs[bk + 1] << " + " << ys[bj + 1] << " + " << xs[bi + 1] << " + " << own
+            << "] << " << (q & 31) << ");\n";
+        }
+    o << "    }\n";
+  }
+  o << "  }\n";
 }
 
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads) {
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads,
+                       const CubeSpec *cube) {
   std::ostringstream o;
   // the semi-grand canonical, unpacked, one-lane variant also carries the N-fold kernel
   const bool with_nfold = kind == 1 && !packed && lanes == 1;
   o << "#include \"clexmc_metropolis.h\"\n" << (with_nfold ? "#include \"clexmc_nfold.h\"\n" : "")
     << "namespace clexmc {\n";
-  emit_field(o, "JitField", n_sublat, variable, binary, subs, bin_off, n_slots, gsubs, gen_prog, packed);
+  emit_field(o, "JitField", n_sublat, variable, binary, subs, bin_off, n_slots, gsubs, gen_prog, packed,
+             kind == 1 && !packed && lanes == 1 ? cube : nullptr);
   o << "}  // namespace clexmc\n";
   o << "extern \"C\" __global__ __launch_bounds__(" << (lanes == 2 || max_threads > 512 ? 1024 : 512) << ") void "
     << jit_kernel_name(kind, packed, lanes) << "(clexmc::RunArgs a) { clexmc::metropolis_body<"
     << (binary ? "true" : "false") << ", " << kind << ", " << (packed ? "true" : "false") << ", clexmc::JitField, "
     << lanes << ">(a, clexmc::BlockRole{(int)blockIdx.x, a.chain_map, a.slabs}); }\n";
+  if (cube && binary && n_sublat == 1 && kind == 1 && !packed && lanes == 1)
+    o << "extern \"C\" __global__ __launch_bounds__(" << (max_threads > 512 ? 1024 : 512) << ") void "
+      << kJitCubeName << "(clexmc::RunArgs a) { clexmc::metropolis_body<true, 1, false, clexmc::JitField, 1, true>"
+      << "(a, clexmc::BlockRole{(int)blockIdx.x, a.chain_map, a.slabs}); }\n";
   if (with_nfold)
     o << "extern \"C\" __global__ __launch_bounds__(256) void " << kJitNfoldName
       << "(clexmc::NfoldArgs na) { clexmc::nfold_body<" << (binary ? "true" : "false")

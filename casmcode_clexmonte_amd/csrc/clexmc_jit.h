@@ -13,15 +13,31 @@
 
 namespace clexmc {
 
+// Register-cube sweep of the semi-grand canonical binary kernel (one
+// sublattice, classes = the 27 residues of the cells mod 3, one lane per 3x3x3
+// block of cells, the class lists in block order): each lane keeps the
+// (3 + 2h)^3 cells around its block in registers, computes every field from
+// them and refreshes, after each class, the cells of that class the other
+// lanes changed.
+struct CubeSpec {
+  int nb[3];        // blocks per axis (L / 3)
+  int h;            // halo: the largest cell offset of a cluster
+  int P1, P12;      // padded row and plane pitch of the LDS image
+  int res[27][3];   // cell residues mod 3 of each class, in class order
+};
+
 // Source of the specialised kernels for the site programs of one supercell:
 // the binary programs (subs[n_sublat], offsets as laid out by the library)
 // when `binary`, else the general table programs (gsubs[n_sublat], gen_prog:
 // per cluster k, table offset, then k (offset, radix multiplier) pairs).
 // One kernel per (kind, packed, lanes per site) variant; `max_threads` (512 or
 // 1024) is its launch bound.
+// `cube` (semi-grand canonical, unpacked, one-lane variant only; may be
+// nullptr) adds the register-cube sweep.
 std::string jit_source(int n_sublat, const std::vector<int> &variable, bool binary, const std::vector<BinSub> &subs,
                        const std::vector<int> &bin_off, int n_slots, const std::vector<GenSub> &gsubs,
-                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads);
+                       const std::vector<int> &gen_prog, int kind, bool packed, int lanes, int max_threads,
+                       const CubeSpec *cube = nullptr);
 
 // One supercell shape of a fused launch: its site programs (as for
 // jit_source) and image format.
@@ -43,6 +59,10 @@ constexpr const char *kJitFusedName = "clexmc_jit_fused";
 // Kernel name for (kind, packed, lanes per site) in a module built from
 // jit_source (two lanes per site: binary expansions only).
 const char *jit_kernel_name(int kind, bool packed, int lanes_per_site);
+
+// The register-cube kernel in the module of the semi-grand canonical,
+// unpacked, one-lane variant of a supercell with a CubeSpec.
+constexpr const char *kJitCubeName = "clexmc_jit_s_u8_cube";
 
 // The N-fold kernel in the module of the semi-grand canonical, unpacked,
 // one-lane variant.

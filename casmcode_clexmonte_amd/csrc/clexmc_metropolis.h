@@ -319,6 +319,8 @@ struct FieldCtx {
 };
 
 struct GenericField {
+  static constexpr bool kCube = false;
+  struct Cube {};
   template <class IMG>
   __device__ __forceinline__ static float bin(const FieldCtx &c, const IMG &img, int b, int pa, int o,
                                               int (&K)[kMaxSlots]) {
@@ -360,7 +362,9 @@ struct BlockRole {
   int slabs;
 };
 
-template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1>
+// CUBE: the register-cube sweep instead of the class loop (the field policy
+// must provide it; semi-grand canonical binary one-lane unpacked only).
+template <bool BIN, int KIND, bool PACK, class FIELD, int SPLIT = 1, bool CUBE = false>
 __device__ __forceinline__ void metropolis_body(const RunArgs &args, const BlockRole role) {
   extern __shared__ __align__(16) unsigned char smem[];
   // slabs > 1: this workgroup runs slab `slab` of its chain
@@ -561,6 +565,28 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
   cptr<int> gen_prog = cst(sc->gen_prog);
   const FieldCtx fctx{sc, bin_off, gen_prog, n_slots};
 
+  // register-cube sweep: each lane keeps the cells around its 3x3x3 block
+  // (the specialised semi-grand canonical binary kernel of a supercell whose
+  // classes are the residues mod 3, lane t = block t)
+  constexpr bool kCubePath = CUBE && BIN && KIND == 1 && !PACK && SPLIT == 1 && FIELD::kCube;
+  constexpr bool use_cube = kCubePath;  // the host never runs this kernel as slabs
+  [[maybe_unused]] typename FIELD::Cube cube;
+  int cax[3] = {0, 0, 0}, cay[3] = {0, 0, 0}, caz[3] = {0, 0, 0};
+  if constexpr (kCubePath) {
+    if (use_cube) {
+      const int n1 = FIELD::kCubeBlocks[0], n2 = FIELD::kCubeBlocks[1], n3 = FIELD::kCubeBlocks[2];
+      const int bi = tid % n1, bj = (tid / n1) % n2, bk = tid / (n1 * n2);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cax[d] = ((bi + d - 1 + n1) % n1) * 3 + h;
+        cay[d] = (((bj + d - 1 + n2) % n2) * 3 + h) * P1;
+        caz[d] = (((bk + d - 1 + n3) % n3) * 3 + h) * P12;
+      }
+      // cube cell (0,0,0): the block origin minus the halo, in the padded image
+      FIELD::cube_load(cube, img, (bk * 3) * P12 + (bj * 3) * P1 + bi * 3);
+    }
+  }
+
   // n^2 - sum_o n_o^2 over the occupants of one independent set (block-uniform)
   auto class_mixed = [&](const Ranges &R) -> long long {
     const int n = R.n();
@@ -596,7 +622,35 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     }
     if (KIND != 1 && auto_rounds)
       for (int c = 0; c < n_classes; ++c) m_total += class_mixed(ranges(c));
-    for (int c = 0; c < n_classes; ++c) {
+    if constexpr (kCubePath) {
+      if (use_cube) {
+        // register-cube sweep (one sublattice, 27 residue classes, lane =
+        // block): the same proposals, draws and updates as the class loop
+        const float chem0 = cp->chem[0], chem1 = cp->chem[1];
+        auto site = [&](float F, const int (&K)[kMaxSlots], int o) -> bool {
+          const uint32_t u = rng.next_u32();
+          const float dphi = o ? -2.0f : 2.0f;
+          const float dE = dphi * F - (o ? (chem0 - chem1) : (chem1 - chem0));
+          const bool acc = metropolis_accept(dE, beta, u, rng);
+          accAtt += 1;
+          const int sk = acc ? (o ? -2 : 2) : 0;
+#pragma unroll
+          for (int a = 0; a < kMaxSlots; ++a) accC[a] += __mul24(sk, K[a]);
+          accN[1] += sk / 2;
+          accN[0] -= sk / 2;
+          accAcc += acc ? 1 : 0;
+          // keep the running sums in place: without this the compiler defers
+          // the 27 classes' additions to the end of the pass and keeps every
+          // class's cluster sums alive until then
+          asm volatile("" : "+v"(accC[0]), "+v"(accC[1]), "+v"(accC[2]), "+v"(accC[3]), "+v"(accC[4]),
+                       "+v"(accC[5]), "+v"(accC[6]), "+v"(accC[7]), "+v"(accN[0]), "+v"(accN[1]), "+v"(accAtt),
+                       "+v"(accAcc));
+          return acc;
+        };
+        FIELD::cube_pass(cube, img, cax, cay, caz, site);
+      }
+    }
+    for (int c = 0; c < n_classes && !use_cube; ++c) {
       const int b = sc->class_sublat[c];
       const Ranges R = ranges(c);
       const int nR = R.n();

@@ -79,7 +79,7 @@ def lib():
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
                      "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
                      "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
-                     "clexmc_nfold_run_time"]:
+                     "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -105,7 +105,7 @@ def exported_symbols() -> List[str]:
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
             "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
             "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
-            "clexmc_nfold_run_time"]
+            "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube"]
 
 
 # layout of clexmc_nfold_run_moments' records
@@ -529,6 +529,17 @@ class Engine:
         v = C.c_int()
         self._check(lib().clexmc_last_run_slabs(self.h, C.byref(v)), "clexmc_last_run_slabs")
         return v.value
+
+    def set_register_cube(self, mode: int = 1):
+        """Register-cube sweep of the specialised semi-grand canonical binary
+        kernel where the supercell allows it (1, default) or the class loop (0);
+        bit-identical results."""
+        self._check(lib().clexmc_set_register_cube(self.h, C.c_int(mode)), "clexmc_set_register_cube")
+
+    def last_run_cube(self) -> bool:
+        v = C.c_int()
+        self._check(lib().clexmc_last_run_cube(self.h, C.byref(v)), "clexmc_last_run_cube")
+        return bool(v.value)
 
     def set_fused_launch(self, mode: int = 1):
         """Chains on several supercell shapes: 1 one fused launch (default),

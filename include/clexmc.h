@@ -276,6 +276,20 @@ clexmc_status clexmc_set_fused_launch(clexmc_handle *h, int mode);
 /* 1 when the last clexmc_run_passes ran as one fused launch, else 0. */
 clexmc_status clexmc_last_run_fused(clexmc_handle *h, int *fused);
 
+/* Register-cube sweep of the run-time specialised semi-grand canonical
+ * binary kernel (one sublattice, classes = the residues of the cell
+ * coordinates mod 3, one lane per 3x3x3 block of cells; automatic where the
+ * supercell has that layout): each lane keeps the (3 + 2h)^3 cells around
+ * its block in registers (one byte per cell), computes every site's field
+ * from them and, after each class, reads back only the cells of that class
+ * that other lanes may have changed, instead of one LDS read per neighbour
+ * and a halo copy per class.  The proposals, random draws and results are
+ * bit-identical to the class loop.  mode 1 (default): where available; 0:
+ * the class loop.  No counterpart in the reference (a layout of the sweep). */
+clexmc_status clexmc_set_register_cube(clexmc_handle *h, int mode);
+/* 1 when the last clexmc_run_passes used the register-cube sweep, else 0. */
+clexmc_status clexmc_last_run_cube(clexmc_handle *h, int *cube);
+
 /* Canonical runs: serial swaps of uniformly random site pairs per pass
  * (default 1), beside the parallel independent-set matchings and pair
  * phases.  The pair may share clusters (the reference's unrestricted
