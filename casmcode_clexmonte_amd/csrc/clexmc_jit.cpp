@@ -296,7 +296,19 @@ static void emit_cube(std::ostringstream &o, const BinSub &B, const std::vector<
       const int n = B.slot_n[j], ko = B.slot_k[j];
       if (n <= 0) continue;
       o << "      {\n        int X = 0;\n";
-      for (int e = 0; e < n && ko > 0; ++e) {
+      if (ko == 1) {
+        // pairs: the orbit's neighbour bits counted word by word (mask, popcount)
+        std::map<int, uint32_t> masks;
+        for (int e = 0; e < n; ++e) {
+          int dx, dy, dz;
+          decode(bin_off[B.slot_begin[j] + e], dx, dy, dz);
+          const int q = P + dx + S * dy + S * S * dz;
+          masks[q >> 5] |= 1u << (q & 31);
+        }
+        for (auto const &wm : masks)
+          o << "        X += __builtin_popcount(cb.r[" << wm.first << "] & " << wm.second << "u);\n";
+      }
+      for (int e = 0; e < n && ko > 1; ++e) {
         o << "        X += ";
         for (int i = 0; i < ko; ++i) {
           int dx, dy, dz;
