@@ -45,9 +45,10 @@ sys.path.insert(0, ROOT)
 LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
 # VALU issue peak for unpacked 32-bit ops: 256 CUs x 64 lanes x 2.4 GHz
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
-# VALU lane-ops per attempt of the specialised kernel on this workload
-# (SQ_INSTS_VALU x 64 / attempts, profiles/metropolis_specialized_fcc24_sgc256.md)
-VALU_OPS_PER_ATTEMPT = 253
+# VALU lane-ops per attempt of the specialised kernel on this workload (the
+# register-cube sweep; SQ_INSTS_VALU x 64 / attempts,
+# profiles/metropolis_specialized_fcc24_sgc256.md)
+VALU_OPS_PER_ATTEMPT = 291
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
