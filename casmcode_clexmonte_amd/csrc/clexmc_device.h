@@ -213,6 +213,20 @@ struct ChainState {
   unsigned long long flags;    // kChainFlag* conditions raised by a kernel, cleared by the host
 };
 
+// Whole-chain Metropolis launches hold the chain's running totals in LDS for
+// the launch: thread 0 copies them in at the start, updates them at every
+// bookkeeping step (LDS instead of a chain of dependent global
+// read-modify-writes while the workgroup waits at a barrier) and writes them
+// back at the end.
+struct ChainLds {
+  double E, sumE, sumE2, sumN1, sumN12, sumEN1;
+  long long counts[kMaxOcc];
+  long long n_attempt, n_accept, n_samples_acc;
+  int sample_count, pad_;
+  long long C[kMaxOrbits];  // binary: exact correlation counts
+};
+constexpr int kChainLdsBytes = (int)((sizeof(ChainLds) + 15) & ~(size_t)15);
+
 // N-fold way: the chain's total rate fell below the smallest the kernel
 // accepts (no event can be drawn reliably); the run stopped early.
 constexpr unsigned long long kChainFlagNfoldStalled = 1ull;
@@ -224,20 +238,17 @@ constexpr unsigned long long kChainFlagMswapCounts = 2ull;
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
 // easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with the
-// published multipliers and Weyl key increments.
+// published multipliers and Weyl key increments.  Each product is written as
+// one 64-bit multiply: on gfx950 that is a single v_mad_u64_u32 instead of a
+// v_mul_hi_u32 / v_mul_lo_u32 pair.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
-#if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t hi0 = __umulhi(m0, c.x), lo0 = m0 * c.x;
-    const uint32_t hi1 = __umulhi(m1, c.z), lo1 = m1 * c.z;
-#else
     const uint64_t p0 = (uint64_t)m0 * c.x, p1 = (uint64_t)m1 * c.z;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-#endif
     c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;

@@ -371,7 +371,8 @@ size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_ma
   size_t img = (size_t)image_bytes(pad_sites, packed);
   size_t tab = binary ? 0 : (size_t)((n_tab * 4 + 15) & ~15);
   const int n_islot = (binary ? kMaxSlots : 0) + kMaxOcc + 2;
-  size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16 + 128 + kMswapLdsBytes;
+  size_t red = (size_t)n_islot * 16 * sizeof(int) + 16 * sizeof(double) + 16 + 128 + kMswapLdsBytes +
+               kChainLdsBytes;
   const int nfs = binary ? 1 : n_func_max;
   size_t scratch = (size_t)nfs * max_class * sizeof(float) + (binary ? (size_t)kMaxSlots * max_class : 0);
   return img + tab + red + scratch + 64;

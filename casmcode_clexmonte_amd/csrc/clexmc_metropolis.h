@@ -412,6 +412,9 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
   int *poolN = (int *)(smem + off_b);                    // [kMaxSublat][kMaxOcc]
   int *msw = poolN + kMaxSublat * kMaxOcc;               // [8] control + [kMaxMswapSites] picks
   off_b += kMswapLdsBytes;
+  // whole-chain launches: the chain's running totals (thread 0 only)
+  ChainLds *cl = (ChainLds *)(smem + off_b);
+  off_b += kChainLdsBytes;
   const int nfs = BIN ? 1 : sys->n_func_max;              // field values kept per class site
   float *cF = (float *)(smem + off_b);                   // [nfs][max_class]
   int8_t *cK = (int8_t *)(cF + nfs * max_class);         // [kMaxSlots][max_class] (binary)
@@ -467,6 +470,23 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     *maxDraw = 0ull;
     for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) slabI[q] = 0;
     *slabE = 0.0;
+    if (n_slabs == 1) {
+      cl->E = cs->E;
+      cl->sumE = cs->sumE;
+      cl->sumE2 = cs->sumE2;
+      cl->sumN1 = cs->sumN1;
+      cl->sumN12 = cs->sumN12;
+      cl->sumEN1 = cs->sumEN1;
+      for (int o = 0; o < kMaxOcc; ++o) cl->counts[o] = cs->counts[o];
+      cl->n_attempt = cs->n_attempt;
+      cl->n_accept = cs->n_accept;
+      cl->n_samples_acc = cs->n_samples_acc;
+      cl->sample_count = args.sample_count ? args.sample_count[chain] : 0;
+      if (BIN) {
+        const long long *C = args.binC + (long long)chain * args.n_orbits;
+        for (int a = 0; a < args.n_orbits; ++a) cl->C[a] = C[a];
+      }
+    }
   }
   // (multiswap runs: the fixed-composition sweep, kind 2)
   const bool mswaps = KIND == 2 && args.n_mswaps > 0 && n_slabs == 1;
@@ -1240,35 +1260,34 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
           }
         } else {
         if (BIN) {
-          long long *C = args.binC + (long long)chain * args.n_orbits;
-          for (int j = 0; j < n_slots; ++j) C[sys->slot_orbit[j]] += sumI[j];
+          for (int j = 0; j < n_slots; ++j) cl->C[sys->slot_orbit[j]] += sumI[j];
           E = 0.0;
-          for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)C[a];
+          for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)cl->C[a];
           s2 = kMaxSlots;
         } else {
           double dsum = 0.0;
           for (int w = 0; w < n_waves; ++w) dsum += redD[w];
-          E = cs->E + dsum;
+          E = cl->E + dsum;
         }
-        cs->E = E;
-        for (int o = 0; o < kMaxOcc; ++o) cs->counts[o] += sumI[s2 + o];
-        cs->n_attempt += sumI[s2 + kMaxOcc];
-        cs->n_accept += sumI[s2 + kMaxOcc + 1];
+        cl->E = E;
+        for (int o = 0; o < kMaxOcc; ++o) cl->counts[o] += sumI[s2 + o];
+        cl->n_attempt += sumI[s2 + kMaxOcc];
+        cl->n_accept += sumI[s2 + kMaxOcc + 1];
         if (sample_due) {
-          const int idx = args.sample_count[chain];
+          const int idx = cl->sample_count;
           if (idx < args.sample_cap) {
             const long long o = (long long)chain * args.sample_cap + idx;
             args.sampE[o] = E;
-            for (int q = 0; q < kMaxOcc; ++q) args.sampN[o * kMaxOcc + q] = (int)cs->counts[q];
-            args.sample_count[chain] = idx + 1;
+            for (int q = 0; q < kMaxOcc; ++q) args.sampN[o * kMaxOcc + q] = (int)cl->counts[q];
+            cl->sample_count = idx + 1;
           }
-          const double n1 = (double)cs->counts[1];
-          cs->sumE += E;
-          cs->sumE2 += E * E;
-          cs->sumN1 += n1;
-          cs->sumN12 += n1 * n1;
-          cs->sumEN1 += E * n1;
-          cs->n_samples_acc += 1;
+          const double n1 = (double)cl->counts[1];
+          cl->sumE += E;
+          cl->sumE2 += E * E;
+          cl->sumN1 += n1;
+          cl->sumN12 += n1 * n1;
+          cl->sumEN1 += E * n1;
+          cl->n_samples_acc += 1;
         }
         }  // n_slabs == 1
       }
@@ -1309,6 +1328,21 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     } else {
       cs->draw_base = *maxDraw + 1ull;
       cs->pass_base = pass0 + (unsigned long long)args.n_passes;
+      cs->E = cl->E;
+      cs->sumE = cl->sumE;
+      cs->sumE2 = cl->sumE2;
+      cs->sumN1 = cl->sumN1;
+      cs->sumN12 = cl->sumN12;
+      cs->sumEN1 = cl->sumEN1;
+      for (int o = 0; o < kMaxOcc; ++o) cs->counts[o] = cl->counts[o];
+      cs->n_attempt = cl->n_attempt;
+      cs->n_accept = cl->n_accept;
+      cs->n_samples_acc = cl->n_samples_acc;
+      if (args.sample_count) args.sample_count[chain] = cl->sample_count;
+      if (BIN) {
+        long long *C = args.binC + (long long)chain * args.n_orbits;
+        for (int a = 0; a < args.n_orbits; ++a) C[a] = cl->C[a];
+      }
     }
   }
 }
