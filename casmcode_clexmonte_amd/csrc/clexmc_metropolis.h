@@ -61,10 +61,18 @@ __device__ __forceinline__ uint32_t uniform_int(uint32_t n, unsigned long long p
   return n ? (uint32_t)(x % n) : 0u;
 }
 
+// Sum over the 64 lanes of a full wave, returned to every lane.  Four DPP
+// row shifts (row_shr 1, 2, 4, 8; lanes shifted in from outside a row of 16
+// add 0) leave each row's sum in the row's last lane, and the four row sums
+// are read out as scalars: VALU work only, where a butterfly of ds_bpermute
+// shuffles waits on six LDS round trips per value.
 __device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  return __builtin_amdgcn_readlane(v, 15) + __builtin_amdgcn_readlane(v, 31) + __builtin_amdgcn_readlane(v, 47) +
+         __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1235,13 +1243,15 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
         accE = 0.0;
       }
       __syncthreads();
-      if (tid == 0) {
+      if (wave == 0) {
+        // lane q adds up slot q over the waves; thread 0 reads the totals
+        int part = 0;
+        if (lane < n_islot)
+          for (int w = 0; w < n_waves; ++w) part += redI[lane * n_waves + w];
         int sumI[kMaxSlots + kMaxOcc + 2];
-        for (int q = 0; q < n_islot; ++q) {
-          int acc = 0;
-          for (int w = 0; w < n_waves; ++w) acc += redI[q * n_waves + w];
-          sumI[q] = acc;
-        }
+#pragma unroll
+        for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) sumI[q] = q < n_islot ? __builtin_amdgcn_readlane(part, q) : 0;
+      if (lane == 0) {
         int s2 = 0;
         double E;
         if (n_slabs > 1) {
@@ -1290,7 +1300,8 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
           cl->n_samples_acc += 1;
         }
         }  // n_slabs == 1
-      }
+      }  // lane 0
+      }  // wave 0
       __syncthreads();
     }
   }
