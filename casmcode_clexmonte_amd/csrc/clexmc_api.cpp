@@ -46,6 +46,7 @@ struct HostSupercell {
   // the cells mod 3 listed in block order): eligible, and its layout
   bool cube_ok = false;
   CubeSpec cube{};
+  int nb_total = 0;  // N-fold neighbour-list entries over all sublattices
 };
 
 }  // namespace
@@ -943,6 +944,7 @@ clexmc_status clexmc_add_supercell(clexmc_handle *h, const clexmc_supercell_desc
       nb_max = std::max(nb_max, nb_begin[b + 1] - nb_begin[b]);
     }
     d.nb_max = nb_max;
+    hs.nb_total = (int)nb_cell.size();
     // whether the periodic supercell keeps every list's sites distinct
     d.nb_distinct = 1;
     auto md = [](int x, int m) { return ((x % m) + m) % m; };
@@ -1496,16 +1498,18 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
   ARG_CHECK(blocks_out || n_events == 0, "blocks_out is NULL");
   if (n_events == 0) return CLEXMC_OK;
   HIP_TRY(hipSetDevice(h->device));
-  int max_pad = 0, max_sites = 0, nb_max = 1;
+  int max_pad = 0, max_sites = 0, nb_max = 1, nb_total = 1;
   for (int c = 0; c < h->n_chains; ++c) {
-    const DevSupercell &d = h->scs[h->chain_sc[c]].dev;
+    const HostSupercell &hs = h->scs[h->chain_sc[c]];
+    const DevSupercell &d = hs.dev;
     max_pad = std::max(max_pad, d.pad_bytes);
     max_sites = std::max(max_sites, d.n_sites);
     nb_max = std::max(nb_max, d.nb_max);
+    nb_total = std::max(nb_total, hs.nb_total);
   }
   ARG_CHECK(max_sites <= 262144, "the N-fold kernel takes supercells of at most 262,144 sites");
   ARG_CHECK(h->n_mswaps == 0, "the N-fold kernel's events are single-site changes: clear the multiswaps");
-  const size_t lds = nfold_lds_bytes(max_pad, max_sites, nb_max);
+  const size_t lds = nfold_lds_bytes(max_pad, max_sites, nb_max, nb_total);
   ARG_CHECK(lds <= 160 * 1024, "supercell too large for the N-fold kernel's LDS (image + one rate per site)");
   NfoldArgs na;
   memset(&na, 0, sizeof(na));

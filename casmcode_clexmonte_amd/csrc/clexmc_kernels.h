@@ -130,6 +130,8 @@ struct AdamArgs {
 // [17 + o] sum w E N_o; E the energy per supercell, N_o the sites holding
 // occupant o, w the residence weight of the state.
 constexpr int kNfoldBlockSums = 21;
+// LDS bytes of the N-fold kernel's neighbour-list starts ([kMaxSublat + 1] ints)
+constexpr int kNfoldNbBeginBytes = ((kMaxSublat + 1) * 4 + 15) & ~15;
 struct NfoldArgs {
   RunArgs a;               // system, supercells, chain params / state, occupations, exact counts
   int n_events;            // events per chain in this launch
@@ -176,7 +178,8 @@ hipError_t launch_corr(const CorrArgs &a, int n_corr, int n_chains, hipStream_t 
 hipError_t launch_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
 size_t metropolis_lds_bytes(int pad_sites, int n_tab, bool binary, int n_func_max, int max_class, bool packed);
 hipError_t launch_nfold(const NfoldArgs &a, bool binary, int n_chains, int threads, size_t lds, hipStream_t st);
-size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max);
+// nb_total: neighbour-list entries of the supercell, all sublattices
+size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max, int nb_total);
 
 #endif  // __HIPCC_RTC__
 

@@ -41,13 +41,36 @@ namespace clexmc {
 
 __host__ __device__ inline size_t nfold_align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__device__ __forceinline__ double wave_scan_incl_d(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
+// x of the lane that DPP control CTRL names (0 where it names none, and in
+// the rows ROW_MASK leaves out), the two halves moved separately
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Inclusive prefix sum (FP64) over the 64 lanes of a full wave: shifted adds
+// within rows of 16 lanes (DPP row_shr 1, 2, 4, 8), then row 0's total into
+// row 1 and row 2's into row 3 (row_bcast 15), then rows 0-1's total into
+// rows 2-3 (row_bcast 31).  VALU only: no LDS round trips.
+__device__ __forceinline__ double wave_scan_incl_d(double v) {
+  v += dpp_d<0x111, 0xf>(v);
+  v += dpp_d<0x112, 0xf>(v);
+  v += dpp_d<0x114, 0xf>(v);
+  v += dpp_d<0x118, 0xf>(v);
+  v += dpp_d<0x142, 0xa>(v);
+  v += dpp_d<0x143, 0xc>(v);
   return v;
+}
+
+// x of lane l (l the same in every lane)
+__device__ __forceinline__ double lane_d(double x, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // first lane whose inclusive prefix exceeds t; if rounding leaves none, the
@@ -117,8 +140,14 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   int *affp = (int *)(smem + off);   // their padded addresses
   off += nfold_align16((size_t)sc->nb_max * 4);
   int *sel = (int *)(smem + off);
+  off += 16;
+  // the interaction-neighbour offsets of every sublattice, staged once: an
+  // event reads them right after its flip
+  int *nbB = (int *)(smem + off);     // [n_sublat + 1] list starts
+  off += kNfoldNbBeginBytes;
+  int4 *nbL = (int4 *)(smem + off);   // (sublattice, di, dj, dk) per neighbour
 
-  // ---- padded image
+  // ---- padded image, neighbour lists
   const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
   for (int p = tid; p < pad_bytes; p += nthr) {
     const int b = p / S;
@@ -127,7 +156,14 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     const int i = ((pi - h) % L1 + L1) % L1, j = ((pj - h) % L2 + L2) % L2, k = ((pk - h) % L3 + L3) % L3;
     img.p[p] = (uint8_t)occG[b * n_cells + i + L1 * (j + L2 * k)];
   }
+  {
+    const int n_sub = sys->n_sublat;
+    const int nb_tot = sc->nb_begin[n_sub];
+    for (int t = tid; t < nb_tot; t += nthr) nbL[t] = sc->nb_cell[t];
+    if (tid <= n_sub) nbB[tid] = sc->nb_begin[tid];
+  }
   __syncthreads();
+  const int P3 = S / P12;
 
   const float beta = cp->beta;
   cptr<int> bin_off = cst(sc->bin_off);
@@ -208,8 +244,8 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       const int g0 = min(lane * chunk, n_groups), g1 = min(g0 + chunk, n_groups);
       double ls = 0.0;
       for (int g = g0; g < g1; ++g) ls += G[g];
-      const double incl = wave_scan_incl_d(ls, lane);
-      const double R = __shfl(incl, 63, 64);
+      const double incl = wave_scan_incl_d(ls);
+      const double R = lane_d(incl, 63);
       const unsigned long long gev = ev0 + (unsigned long long)ev;
       const uint4 r = philox4x32_10(make_uint4(0x4E464C44u, (uint32_t)gev, (uint32_t)(gev >> 32), (uint32_t)chain),
                                     cp->key0, cp->key1);
@@ -218,17 +254,17 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       const bool live = R >= kNfoldMinTotalRate;
       const int L = live ? pick_lane(incl, ls, t) : -1;
       if (L >= 0) {
-        const double before = __shfl(incl, L, 64) - __shfl(ls, L, 64);
-        const int gl0 = __shfl(g0, L, 64), gl1 = __shfl(g1, L, 64);
+        const double before = lane_d(incl, L) - lane_d(ls, L);
+        const int gl0 = __builtin_amdgcn_readlane(g0, L), gl1 = __builtin_amdgcn_readlane(g1, L);
         const double gv = (gl0 + lane < gl1) ? G[gl0 + lane] : 0.0;
-        const double gi = wave_scan_incl_d(gv, lane);
+        const double gi = wave_scan_incl_d(gv);
         const double t2 = t - before;
         const int lg = pick_lane(gi, gv, t2);
         if (lg >= 0) {
           const int g = gl0 + lg;
-          const double before2 = __shfl(gi, lg, 64) - __shfl(gv, lg, 64);
+          const double before2 = lane_d(gi, lg) - lane_d(gv, lg);
           const double rv = (double)rate[g * 64 + lane];
-          const double ri = wave_scan_incl_d(rv, lane);
+          const double ri = wave_scan_incl_d(rv);
           const int ls2 = pick_lane(ri, rv, t2 - before2);
           if (ls2 >= 0) site = g * 64 + ls2;
         }
@@ -317,10 +353,26 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       }
       cnt[o] -= 1;
       cnt[nw] += 1;
-      // the site and every periodic copy of it (the halo list of its class position)
-      img.p[pa] = (uint8_t)nw;
-      const int tp = sc->site_tpos[site];
-      for (int q = sc->halo_site[tp]; q < sc->halo_site[tp + 1]; ++q) img.p[sc->halo[q].y] = (uint8_t)nw;
+      // the site and every periodic copy of it in the padded image: padded
+      // coordinate (i + h) + d L for every d that stays inside [0, L + 2h)
+      {
+        const int c = site - b * n_cells;
+        const int ci = c % L1, cj = (c / L1) % L2, ck = c / (L1 * L2);
+        const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
+        for (int dk = -m3; dk <= m3; ++dk) {
+          const int qk = ck + h + dk * L3;
+          if (qk < 0 || qk >= P3) continue;
+          for (int dj = -m2; dj <= m2; ++dj) {
+            const int qj = cj + h + dj * L2;
+            if (qj < 0 || qj >= P12 / P1) continue;
+            for (int di = -m1; di <= m1; ++di) {
+              const int qi = ci + h + di * L1;
+              if (qi < 0 || qi >= P1) continue;
+              img.p[b * S + qk * P12 + qj * P1 + qi] = (uint8_t)nw;
+            }
+          }
+        }
+      }
       ++n_done;
     }
     // ---- rates of the site and its interaction neighbours
@@ -328,11 +380,11 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       const int b = site / n_cells;
       const int c = site - b * n_cells;
       const int i = c % L1, j = (c / L1) % L2, k = c / (L1 * L2);
-      const int nb0 = sc->nb_begin[b], nb1 = sc->nb_begin[b + 1];
+      const int nb0 = nbB[b], nb1 = nbB[b + 1];
       const bool distinct = sc->nb_distinct != 0;  // no duplicate sites: no list in LDS, no search
       if (!distinct)
         for (int t = tid; t < nb1 - nb0; t += nthr) {
-          const int4 o = sc->nb_cell[nb0 + t];
+          const int4 o = nbL[nb0 + t];
           const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
           aff[t] = o.x * n_cells + it + L1 * (jt + L2 * kt);
           affp[t] = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
@@ -341,7 +393,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       for (int t = tid; t < nb1 - nb0; t += nthr) {
         int lt, pt;
         if (distinct) {
-          const int4 o = sc->nb_cell[nb0 + t];
+          const int4 o = nbL[nb0 + t];
           const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
           lt = o.x * n_cells + it + L1 * (jt + L2 * kt);
           pt = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
@@ -355,7 +407,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         if (distinct) aff[t] = lt;  // the groups to sum again below
         if (!dup) {
           int K[kMaxSlots];
-          rate[lt] = site_rate_at(sc->nb_cell[nb0 + t].x, pt, K);
+          rate[lt] = site_rate_at(nbL[nb0 + t].x, pt, K);
           if (BIN && t == 0) {
             // the flipped site itself (first in the list, lane 0): exact cluster-count change
             const int sk = o_flip ? -2 : 2;
