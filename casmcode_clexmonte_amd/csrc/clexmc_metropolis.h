@@ -148,11 +148,16 @@ struct PadWalk {
 };
 
 // the cell coordinate of padded coordinate q (halo h, period L): q - h in [0, L)
+// (q in [0, L + 2h), so for h <= L one period either way is enough: two
+// selects; h and L are the same in every lane)
 __device__ __forceinline__ int pad_wrap(int q, int h, int L) {
   int x = q - h;
-  while (x < 0) x += L;
-  while (x >= L) x -= L;
-  return x;
+  if (h <= L) {
+    x = x < 0 ? x + L : x;
+    return x >= L ? x - L : x;
+  }
+  x %= L;
+  return x < 0 ? x + L : x;
 }
 
 // bytes of the LDS image for pad_sites padded sites, a multiple of 16 so that
@@ -1251,8 +1256,23 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
         int sumI[kMaxSlots + kMaxOcc + 2];
 #pragma unroll
         for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q) sumI[q] = q < n_islot ? __builtin_amdgcn_readlane(part, q) : 0;
+        const int s2 = BIN ? kMaxSlots : 0;
+        if (n_slabs == 1) {
+          // lane a adds the slots of orbit a to its exact count, lane o the
+          // change of occupant o's count (one LDS update per lane)
+          if (BIN && lane < args.n_orbits) {
+            long long d = 0;
+            for (int j = 0; j < n_slots; ++j) d += sys->slot_orbit[j] == lane ? (long long)sumI[j] : 0ll;
+            cl->C[lane] += d;
+          }
+          if (lane < kMaxOcc) {
+            int dn = sumI[s2];
+#pragma unroll
+            for (int o = 1; o < kMaxOcc; ++o) dn = lane == o ? sumI[s2 + o] : dn;
+            cl->counts[lane] += dn;
+          }
+        }
       if (lane == 0) {
-        int s2 = 0;
         double E;
         if (n_slabs > 1) {
           // running totals of this workgroup since the launch started; the
@@ -1270,17 +1290,14 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
           }
         } else {
         if (BIN) {
-          for (int j = 0; j < n_slots; ++j) cl->C[sys->slot_orbit[j]] += sumI[j];
           E = 0.0;
           for (int a = 0; a < args.n_orbits; ++a) E += sys->bin_w64[a] * (double)cl->C[a];
-          s2 = kMaxSlots;
         } else {
           double dsum = 0.0;
           for (int w = 0; w < n_waves; ++w) dsum += redD[w];
           E = cl->E + dsum;
         }
         cl->E = E;
-        for (int o = 0; o < kMaxOcc; ++o) cl->counts[o] += sumI[s2 + o];
         cl->n_attempt += sumI[s2 + kMaxOcc];
         cl->n_accept += sumI[s2 + kMaxOcc + 1];
         if (sample_due) {
