@@ -153,7 +153,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     const int b = p / S;
     const int r = p - b * S;
     const int pk = r / P12, pj = (r - pk * P12) / P1, pi = r - pk * P12 - pj * P1;
-    const int i = ((pi - h) % L1 + L1) % L1, j = ((pj - h) % L2 + L2) % L2, k = ((pk - h) % L3 + L3) % L3;
+    const int i = pad_wrap(pi, h, L1), j = pad_wrap(pj, h, L2), k = pad_wrap(pk, h, L3);
     img.p[p] = (uint8_t)occG[b * n_cells + i + L1 * (j + L2 * k)];
   }
   {
@@ -313,7 +313,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       }
     }
     __syncthreads();
-    const int site = sel[0];
+    const int site = __builtin_amdgcn_readfirstlane(sel[0]);  // the same in every lane
     if (site < 0) {
       // uniform across the workgroup: record the stall for the host
       if (tid == 0) cs->flags |= kChainFlagNfoldStalled;
@@ -358,6 +358,20 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       {
         const int c = site - b * n_cells;
         const int ci = c % L1, cj = (c / L1) % L2, ck = c / (L1 * L2);
+        if (h <= L1 && h <= L2 && h <= L3) {
+          // d in {0, +1 (i < h), -1 (i >= L - h)}: at most three per axis
+          const int qi[3] = {ci + h, ci + h + L1, ci + h - L1}, qj[3] = {cj + h, cj + h + L2, cj + h - L2},
+                    qk[3] = {ck + h, ck + h + L3, ck + h - L3};
+          const bool vi[3] = {true, ci < h, ci >= L1 - h}, vj[3] = {true, cj < h, cj >= L2 - h},
+                     vk[3] = {true, ck < h, ck >= L3 - h};
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+#pragma unroll
+              for (int f = 0; f < 3; ++f)
+                if (vk[a] && vj[e] && vi[f]) img.p[b * S + qk[a] * P12 + qj[e] * P1 + qi[f]] = (uint8_t)nw;
+        } else {
         const int m1 = (h + L1 - 1) / L1, m2 = (h + L2 - 1) / L2, m3 = (h + L3 - 1) / L3;
         for (int dk = -m3; dk <= m3; ++dk) {
           const int qk = ck + h + dk * L3;
@@ -372,6 +386,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
             }
           }
         }
+        }
       }
       ++n_done;
     }
@@ -385,7 +400,8 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       if (!distinct)
         for (int t = tid; t < nb1 - nb0; t += nthr) {
           const int4 o = nbL[nb0 + t];
-          const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
+          // (neighbour offsets are within the halo width h)
+          const int it = pad_wrap(i + o.y + h, h, L1), jt = pad_wrap(j + o.z + h, h, L2), kt = pad_wrap(k + o.w + h, h, L3);
           aff[t] = o.x * n_cells + it + L1 * (jt + L2 * kt);
           affp[t] = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
         }
@@ -394,7 +410,8 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         int lt, pt;
         if (distinct) {
           const int4 o = nbL[nb0 + t];
-          const int it = ((i + o.y) % L1 + L1) % L1, jt = ((j + o.z) % L2 + L2) % L2, kt = ((k + o.w) % L3 + L3) % L3;
+          // (neighbour offsets are within the halo width h)
+          const int it = pad_wrap(i + o.y + h, h, L1), jt = pad_wrap(j + o.z + h, h, L2), kt = pad_wrap(k + o.w + h, h, L3);
           lt = o.x * n_cells + it + L1 * (jt + L2 * kt);
           pt = o.x * S + (kt + h) * P12 + (jt + h) * P1 + (it + h);
         } else {
