@@ -48,7 +48,14 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # VALU lane-ops per attempt of the specialised kernel on this workload (the
 # register-cube sweep; SQ_INSTS_VALU x 64 / attempts,
 # profiles/metropolis_specialized_fcc24_sgc256.md)
-VALU_OPS_PER_ATTEMPT = 291
+VALU_OPS_PER_ATTEMPT = 282
+# HBM bytes per launch of that kernel, measured with rocprofv3 on the 20-pass
+# launch of this workload: FETCH_SIZE 2,605 KiB x 2.000 + WRITE_SIZE 3,652 KiB
+# x 1.000, the factors from the same counters on a plain 256 MiB copy
+# (profiles/metropolis_specialized_fcc24_sgc256.md).  About 9 MB whatever the
+# number of passes: each chain's occupation read once into LDS and written
+# back once, plus state, tables and samples.
+HBM_BYTES_PER_LAUNCH = (2605 * 2 + 3652) * 1024
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
@@ -516,7 +523,9 @@ def main():
                    "sites_per_chain": sc.n_sites, "chains": n_chains, "passes": args.steps,
                    "cluster_expansion": "FCC Chebyshev binary, pairs<=4NN, triplets<=2NN, 8 correlations"},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+                     "frac": achieved / LDS_PEAK_GBS,
+                     "traffic": HBM_BYTES_PER_LAUNCH if launch.get("specialized") else None,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
                      "bytes_per_attempt": bytes_per_attempt, "kernel_ms": kernel_ms,
                      "measured_lds_byte_gather_GBs": lds_u8, "frac_of_measured_byte_gather": achieved / lds_u8,
                      "measured_lds_16B_read_GBs": lds_b128,

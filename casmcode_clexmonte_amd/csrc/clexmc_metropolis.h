@@ -534,6 +534,9 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
   int n_records = 0;  // slab launches (thread 0): records written
 
   const unsigned long long pass0 = cs->pass_base;
+  // passes until the next sampled one (pass index, 0 = this launch's first)
+  int to_sample = (int)((unsigned long long)(args.sample_period - 1) -
+                        pass0 % (unsigned long long)args.sample_period);
   // ---- slab of this workgroup: cell planes [ka, ka + U) (mod L3) are
   // updated; the slab's top kreach planes stay frozen, so the updated sites of
   // two slabs are more than kreach planes apart and share no cluster, and
@@ -1211,7 +1214,10 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     }
 
     // ---- reduction of lane accumulators and sampling
-    const bool due = ((gpass + 1) % (unsigned long long)args.sample_period) == 0ull;
+    // a sample is due after pass gpass when (gpass + 1) % sample_period == 0:
+    // a countdown set from one 64-bit remainder at the launch start
+    const bool due = to_sample == 0;
+    to_sample = due ? args.sample_period - 1 : to_sample - 1;
     const bool sample_due = n_slabs == 1 && due;
     const bool record_due = n_slabs > 1 && (due || pass == args.n_passes - 1);
     const bool flush = sample_due || record_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);
