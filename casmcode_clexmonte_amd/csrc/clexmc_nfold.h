@@ -146,6 +146,11 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   int *nbB = (int *)(smem + off);     // [n_sublat + 1] list starts
   off += kNfoldNbBeginBytes;
   int4 *nbL = (int4 *)(smem + off);   // (sublattice, di, dj, dk) per neighbour
+  off += nfold_align16((size_t)(sc->nb_begin[sys->n_sublat]) * 16);
+  // occupant counts and chemical potentials per sublattice: the rate of a
+  // neighbour reads them for its sublattice (a per-lane index)
+  int *noccL = (int *)(smem + off);           // [kMaxSublat]
+  float *chemL = (float *)(noccL + kMaxSublat);  // [kMaxSublat][kMaxOcc]
 
   // ---- padded image, neighbour lists
   const int8_t *__restrict__ occG = args.occ + cp->occ_offset;
@@ -161,6 +166,8 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
     const int nb_tot = sc->nb_begin[n_sub];
     for (int t = tid; t < nb_tot; t += nthr) nbL[t] = sc->nb_cell[t];
     if (tid <= n_sub) nbB[tid] = sc->nb_begin[tid];
+    if (tid < n_sub) noccL[tid] = sys->n_occ[tid];
+    if (tid < n_sub * kMaxOcc) chemL[tid] = cp->chem[tid];
   }
   __syncthreads();
   const int P3 = S / P12;
@@ -178,18 +185,18 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   const FieldCtx fctx{sc, bin_off, gen_prog, n_slots};
   // rate of one change o -> nw of a general site with cluster sums G
   auto change_rate = [&](int b, int o, int nw, const float (&G)[kMaxFunc]) -> float {
-    const float dE = general_dE(sys, b, o, nw, G) - (cp->chem[b * kMaxOcc + nw] - cp->chem[b * kMaxOcc + o]);
+    const float dE = general_dE(sys, b, o, nw, G) - (chemL[b * kMaxOcc + nw] - chemL[b * kMaxOcc + o]);
     return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
   };
   // rate of the site at padded address pa on sublattice b (0 for fixed
   // sites); binary: its cluster sums K (which do not depend on the site itself)
   auto site_rate_at = [&](int b, int pa, int (&K)[kMaxSlots]) -> float {
-    const int no = sys->n_occ[b];
+    const int no = noccL[b];
     if (no < 2) return 0.0f;
     const int o = img.get(pa);
     if (BIN) {
       const float F = FIELD::bin(fctx, img, b, pa, o, K);
-      const float chem0 = cp->chem[b * kMaxOcc + 0], chem1 = cp->chem[b * kMaxOcc + 1];
+      const float chem0 = chemL[b * kMaxOcc + 0], chem1 = chemL[b * kMaxOcc + 1];
       const float dE = (o ? -2.0f : 2.0f) * F - (o ? (chem0 - chem1) : (chem1 - chem0));
       return dE <= 0.0f ? 1.0f : __expf(-beta * dE);
     }
@@ -222,6 +229,11 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
   double n_mutable = 0.0;
   for (int b = 0; b < sys->n_sublat; ++b)
     if (sys->n_occ[b] >= 2) n_mutable += (double)n_cells;
+  // binary: the weight of each slot's orbit, read once (the energy of a
+  // state is formed at every event)
+  double wslot[kMaxSlots];
+#pragma unroll
+  for (int j = 0; j < kMaxSlots; ++j) wslot[j] = (BIN && j < n_slots) ? sys->bin_w64[sys->slot_orbit[j]] : 0.0;
   double E0 = 0.0;  // binary: energy of the exact counts at launch
   if (BIN) {
     const long long *C0 = args.binC + (long long)chain * args.n_orbits;
@@ -278,10 +290,13 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
         if (site >= 0) {
           // weight of the current state: expected Metropolis steps spent in it
           double E = E0;
-          if (BIN)
-            for (int j = 0; j < n_slots; ++j) E += sys->bin_w64[sys->slot_orbit[j]] * (double)C[j];
-          else
+          if (BIN) {
+#pragma unroll
+            for (int j = 0; j < kMaxSlots; ++j)
+              if (j < n_slots) E += wslot[j] * (double)C[j];
+          } else {
             E = E_run;
+          }
           const double w = n_mutable / (double)R;
           // sampling by N-fold time: every sample time in this state's
           // residence interval [T, T + w / n_mutable) (in passes) samples it
@@ -331,7 +346,7 @@ __device__ __forceinline__ void nfold_body(const NfoldArgs &na) {
       o_flip = o;
       int nw = 1 - o;
       if (!BIN) {
-        const int no = sys->n_occ[b];
+        const int no = noccL[b];
         float G[kMaxFunc];
         FIELD::gen(fctx, img, b, pa, tab, G);
         double rq[kMaxOcc] = {0.0, 0.0, 0.0, 0.0}, tot = 0.0;

@@ -11,7 +11,7 @@ size_t nfold_lds_bytes(int pad_sites, int n_sites, int nb_max, int nb_total) {
   const int n_groups = (n_sites + 63) / 64;
   return nfold_align16((size_t)image_bytes(pad_sites, false)) + nfold_align16((size_t)n_groups * 64 * 4) +
          nfold_align16((size_t)n_groups * 8) + 2 * nfold_align16((size_t)nb_max * 4) + 16 + kNfoldNbBeginBytes +
-         (size_t)nb_total * 16;
+         nfold_align16((size_t)nb_total * 16) + (size_t)kMaxSublat * (1 + kMaxOcc) * 4;
 }
 
 template <bool BIN>
