@@ -101,6 +101,7 @@ struct clexmc_handle {
   int max_lds = 0, n_cu = 0;        // device attributes, read at creation
   unsigned char *noalloc = nullptr; // the device's staging array (clexmc_delta_energy_batch_noalloc)
   int block_size = 0;
+  int nfold_threads = 0;  // N-fold kernel block size (0: automatic)
   int packing = 0;                  // LDS image: 0 automatic, 1 int8, 2 packed 2-bit
   bool last_packed = false;
   int last_specialized = 0, last_threads = 0, last_lanes = 1, last_slabs = 1, last_fused = 0, last_cube = 0;
@@ -1337,6 +1338,14 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads) {
   return CLEXMC_OK;
 }
 
+clexmc_status clexmc_set_nfold_block_size(clexmc_handle *h, int threads) {
+  if (!h) return CLEXMC_ERR_ARG;
+  ARG_CHECK(threads == 0 || threads == 64 || threads == 128 || threads == 256,
+            "N-fold block size must be 0 (automatic), 64, 128 or 256");
+  h->nfold_threads = threads;
+  return CLEXMC_OK;
+}
+
 clexmc_status clexmc_set_slabs(clexmc_handle *h, int slabs) {
   if (!h) return CLEXMC_ERR_ARG;
   ARG_CHECK(slabs >= 0 && slabs <= 64, "slabs must be 0 (automatic), 1 (off) or 2..64");
@@ -1510,6 +1519,8 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
   ARG_CHECK(max_sites <= 262144, "the N-fold kernel takes supercells of at most 262,144 sites");
   ARG_CHECK(h->n_mswaps == 0, "the N-fold kernel's events are single-site changes: clear the multiswaps");
   const size_t lds = nfold_lds_bytes(max_pad, max_sites, nb_max, nb_total);
+  // automatic block: the neighbour list's rates one per thread
+  const int nf_threads = h->nfold_threads > 0 ? h->nfold_threads : std::min(256, (nb_max + 63) / 64 * 64);
   ARG_CHECK(lds <= 160 * 1024, "supercell too large for the N-fold kernel's LDS (image + one rate per site)");
   NfoldArgs na;
   memset(&na, 0, sizeof(na));
@@ -1557,7 +1568,8 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
     NfoldArgs args = na;
     void *params[] = {(void *)&args};
     const hipError_t le =
-        hipModuleLaunchKernel(jf, (unsigned)h->n_chains, 1, 1, 256, 1, 1, (unsigned)lds, h->stream, params, nullptr);
+        hipModuleLaunchKernel(jf, (unsigned)h->n_chains, 1, 1, (unsigned)nf_threads, 1, 1, (unsigned)lds,
+                              h->stream, params, nullptr);
     if (le == hipSuccess) {
       h->last_specialized = 1;
     } else {
@@ -1565,7 +1577,7 @@ static clexmc_status nfold_run_impl(clexmc_handle *h, int n_events, int events_p
       jf = nullptr;
     }
   }
-  if (e == hipSuccess && !jf) e = launch_nfold(na, h->binary != 0, h->n_chains, 256, lds, h->stream);
+  if (e == hipSuccess && !jf) e = launch_nfold(na, h->binary != 0, h->n_chains, nf_threads, lds, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(blocks_out, d_blocks, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_blocks);

@@ -79,7 +79,8 @@ def lib():
                      "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
                      "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
                      "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
-                     "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube"]:
+                     "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube",
+                     "clexmc_set_nfold_block_size"]:
             getattr(L, name).restype = C.c_int
         L.clexmc_last_error.argtypes = [C.c_void_p]
         L.clexmc_destroy.argtypes = [C.c_void_p]
@@ -105,7 +106,8 @@ def exported_symbols() -> List[str]:
             "clexmc_last_run_info", "clexmc_specialization_log", "clexmc_lds_probe",
             "clexmc_set_lanes_per_site", "clexmc_last_run_lanes", "clexmc_set_slabs", "clexmc_last_run_slabs",
             "clexmc_set_fused_launch", "clexmc_last_run_fused", "clexmc_nfold_run_moments",
-            "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube"]
+            "clexmc_nfold_run_time", "clexmc_set_register_cube", "clexmc_last_run_cube",
+                     "clexmc_set_nfold_block_size"]
 
 
 # layout of clexmc_nfold_run_moments' records
@@ -361,6 +363,11 @@ class Engine:
 
     def set_block_size(self, threads: int):
         self._check(lib().clexmc_set_block_size(self.h, C.c_int(threads)), "clexmc_set_block_size")
+
+    def set_nfold_block_size(self, threads: int):
+        """Threads per chain of the N-fold kernel: 64, 128, 256, or 0 (default)
+        for the smallest multiple of 64 covering the neighbour list."""
+        self._check(lib().clexmc_set_nfold_block_size(self.h, C.c_int(threads)), "clexmc_set_nfold_block_size")
 
     def read_samples(self):
         ns = C.c_int(0)

@@ -242,6 +242,14 @@ clexmc_status clexmc_last_run_ms(clexmc_handle *h, double *ms);
  * library caps the block at 512). */
 clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
 
+/* Block size of the N-fold kernel (clexmc_nfold_run*): 64, 128 or 256
+ * threads per chain, or 0 (default) for the smallest multiple of 64 that
+ * covers a site's interaction-neighbour list, at most 256.  An event's
+ * select and flip run in the first wavefront, its neighbours' rates one per
+ * thread; more wavefronts share only the launch's image load and initial
+ * rates. */
+clexmc_status clexmc_set_nfold_block_size(clexmc_handle *h, int threads);
+
 /* Slab decomposition of large chains (no reference counterpart: the
  * reference runs one chain on one thread, occupation_metropolis.hh:72-123).
  * A chain split into S slabs along the third cell axis runs on S workgroups

@@ -235,6 +235,45 @@ def test_nfold_specialized_matches_generic(system):
 
 
 @pytest.mark.parametrize("system", ["fcc_binary", "hcp_ternary"])
+def test_nfold_block_size_bit_identical(system):
+    """The N-fold kernel's block size (clexmc_set_nfold_block_size) changes
+    only how many threads share the image load, the initial rates and the
+    neighbour rates of an event: blocks, occupations and energies are
+    bit-identical for 64, 128 and 256 threads and the automatic choice."""
+    _print_case(f"nfold block sizes {system}")
+    nch = 16
+    T = np.linspace(300.0, 900.0, nch)
+    if system == "fcc_binary":
+        flat = FlatCE(fcc_binary_ce())
+        sc = Supercell(flat.ce.prim, (12, 12, 12))
+        occs = [random_occupation(sc.n_sites, 0.5, 700 + c, exact=False) for c in range(nch)]
+        chem = [[0.0, m] for m in np.linspace(-0.1, 0.1, nch)]
+    else:
+        flat = _hcp_ternary_small()
+        sc = Supercell(flat.ce.prim, (4, 4, 3))
+        rng = np.random.default_rng(23)
+        occs = [rng.integers(0, 3, sc.n_sites).astype(np.int8) for _ in range(nch)]
+        chem = [[0.0, 0.02, -0.03] * flat.n_sublat] * nch
+    out = {}
+    for thr in (0, 64, 128, 256):
+        eng = Engine(flat)
+        eng.set_nfold_block_size(thr)
+        sid = eng.add_supercell(sc, n_pair_phases=0)
+        eng.load_chains([sid] * nch, occs, T, 5151 + np.arange(nch), chem=chem)
+        blocks = eng.nfold_run(6000, 500)
+        out[thr] = (blocks, [eng.read_state(c) for c in range(nch)])
+        eng.close()
+    for thr in (64, 128, 256):
+        assert np.array_equal(out[thr][0], out[0][0]), thr
+        for a, b in zip(out[thr][1], out[0][1]):
+            assert np.array_equal(a["occ"], b["occ"]) and a["E"] == b["E"], thr
+    eng = Engine(flat)
+    with pytest.raises(RuntimeError):
+        eng.set_nfold_block_size(96)
+    eng.close()
+
+
+@pytest.mark.parametrize("system", ["fcc_binary", "hcp_ternary"])
 def test_nfold_time_sampling_exact_enumeration(system):
     """Sampling by N-fold time (clexmc_nfold_run_time): the states occupied
     at times 0, dt, 2 dt, ... are Boltzmann-distributed with equal weights,
