@@ -149,12 +149,15 @@ struct PadWalk {
 
 // the cell coordinate of padded coordinate q (halo h, period L): q - h in [0, L)
 // (q in [0, L + 2h), so for h <= L one period either way is enough: two
-// selects; h and L are the same in every lane)
+// selects; h and L are the same in every lane).  Bytes past L + 2h in a row
+// (the unused part of a padded row pitch) are clamped to a valid cell: their
+// image value is never read, but the global read of it must stay in bounds.
 __device__ __forceinline__ int pad_wrap(int q, int h, int L) {
   int x = q - h;
   if (h <= L) {
     x = x < 0 ? x + L : x;
-    return x >= L ? x - L : x;
+    x = x >= L ? x - L : x;
+    return x < L ? x : L - 1;
   }
   x %= L;
   return x < 0 ? x + L : x;

@@ -227,6 +227,40 @@ def test_lds_row_pitch_does_not_change_results(which, specialize):
         assert e == out[0][3]
 
 
+@pytest.mark.parametrize("packing", ["int8", "packed2"])
+def test_wide_row_pitch_small_supercell(packing):
+    """A row pitch far wider than the rows (64 unused bytes after each
+    10-byte row of a 6x6x6 image): the unused bytes are filled from a clamped
+    valid cell (the image load's global reads stay inside the chain), and the
+    Metropolis and N-fold runs end exactly as with the natural pitch."""
+    _print_case(f"wide row pitch 6^3 {packing}")
+    ce = fcc_binary_ce()
+    flat = FlatCE(ce)
+    sc = Supercell(ce.prim, (6, 6, 6))
+    n = 8
+    occs = [random_occupation(sc.n_sites, 0.5, 60 + c, exact=False) for c in range(n)]
+    chem = [[0.0, mu] for mu in np.linspace(-0.1, 0.1, n)]
+    temps = np.linspace(300, 1500, n)
+    out = []
+    for pitch in (0, 64):
+        eng = Engine(flat)
+        eng.set_lds_row_pitch(pitch)
+        eng.set_image_packing(packing)
+        sid = eng.add_supercell(sc, n_pair_phases=0, seed=3)
+        eng.load_chains([sid] * n, occs, temps, 70 + np.arange(n), chem=chem)
+        eng.run_passes("semigrand_canonical", 12, 1)
+        E, cnt = eng.read_samples()
+        blocks = eng.nfold_run(2000, 500)
+        states = [eng.read_state(c) for c in range(n)]
+        out.append((E, cnt, blocks, [s["occ"] for s in states], [s["E"] for s in states]))
+        eng.close()
+    E, cnt, blocks, occ, e = out[1]
+    assert np.array_equal(E, out[0][0]) and np.array_equal(cnt, out[0][1])
+    assert np.array_equal(blocks, out[0][2])
+    assert all(np.array_equal(a, b) for a, b in zip(occ, out[0][3]))
+    assert e == out[0][4]
+
+
 @pytest.mark.parametrize("dims,n_chains", [((24, 24, 24), 256), ((12, 12, 12), 8), ((6, 24, 24), 4)])
 def test_register_cube_bit_identical_to_class_loop(dims, n_chains):
     """The register-cube sweep of the specialised semi-grand canonical
