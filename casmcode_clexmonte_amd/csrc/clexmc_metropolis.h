@@ -1225,31 +1225,33 @@ __device__ __forceinline__ void metropolis_body(const RunArgs &args, const Block
     const bool record_due = n_slabs > 1 && (due || pass == args.n_passes - 1);
     const bool flush = sample_due || record_due || ((pass & 31) == 31) || (pass == args.n_passes - 1);
     if (flush) {
+      // every wave sum first (independent DPP chains the compiler can
+      // interleave), then lane q stores this wave's total of slot q
+      int vals[kMaxSlots + kMaxOcc + 2];
       int s = 0;
       if (BIN) {
 #pragma unroll
         for (int a = 0; a < kMaxSlots; ++a) {
-          const int v = wave_sum(accC[a]);
-          if (lane == 0) redI[(s + a) * n_waves + wave] = v;
+          vals[s + a] = wave_sum(accC[a]);
           accC[a] = 0;
         }
         s += kMaxSlots;
       }
 #pragma unroll
       for (int o = 0; o < kMaxOcc; ++o) {
-        const int v = wave_sum(accN[o]);
-        if (lane == 0) redI[(s + o) * n_waves + wave] = v;
+        vals[s + o] = wave_sum(accN[o]);
         accN[o] = 0;
       }
       s += kMaxOcc;
+      vals[s] = wave_sum(accAtt);
+      vals[s + 1] = wave_sum(accAcc);
+      accAtt = accAcc = 0;
       {
-        const int v0 = wave_sum(accAtt);
-        const int v1 = wave_sum(accAcc);
-        if (lane == 0) {
-          redI[s * n_waves + wave] = v0;
-          redI[(s + 1) * n_waves + wave] = v1;
-        }
-        accAtt = accAcc = 0;
+        int mine = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxSlots + kMaxOcc + 2; ++q)
+          if (q < n_islot) mine = lane == q ? vals[q] : mine;
+        if (lane < n_islot) redI[lane * n_waves + wave] = mine;
       }
       if (!BIN) {
         const double v = wave_sum(accE);
