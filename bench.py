@@ -48,14 +48,14 @@ VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # VALU lane-ops per attempt of the specialised kernel on this workload (the
 # register-cube sweep; SQ_INSTS_VALU x 64 / attempts,
 # profiles/metropolis_specialized_fcc24_sgc256.md)
-VALU_OPS_PER_ATTEMPT = 282
+VALU_OPS_PER_ATTEMPT = 283
 # HBM bytes per launch of that kernel, measured with rocprofv3 on the 20-pass
-# launch of this workload: FETCH_SIZE 2,605 KiB x 2.000 + WRITE_SIZE 3,652 KiB
+# launch of this workload: FETCH_SIZE 2,617 KiB x 2.000 + WRITE_SIZE 3,652 KiB
 # x 1.000, the factors from the same counters on a plain 256 MiB copy
 # (profiles/metropolis_specialized_fcc24_sgc256.md).  About 9 MB whatever the
 # number of passes: each chain's occupation read once into LDS and written
 # back once, plus state, tables and samples.
-HBM_BYTES_PER_LAUNCH = (2605 * 2 + 3652) * 1024
+HBM_BYTES_PER_LAUNCH = (2617 * 2 + 3652) * 1024
 
 
 def tau_int(x: np.ndarray, c: float = 5.0) -> float:
