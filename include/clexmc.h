@@ -247,7 +247,9 @@ clexmc_status clexmc_set_block_size(clexmc_handle *h, int threads);
  * covers a site's interaction-neighbour list, at most 256.  An event's
  * select and flip run in the first wavefront, its neighbours' rates one per
  * thread; more wavefronts share only the launch's image load and initial
- * rates. */
+ * rates.  Results do not depend on it.  No counterpart in the reference
+ * (its nfold loop, include/casm/clexmonte/nfold/nfold_impl.hh:18-98, runs
+ * on one CPU thread). */
 clexmc_status clexmc_set_nfold_block_size(clexmc_handle *h, int threads);
 
 /* Slab decomposition of large chains (no reference counterpart: the
@@ -288,7 +290,8 @@ clexmc_status clexmc_last_run_fused(clexmc_handle *h, int *fused);
  * binary kernel (one sublattice, classes = the residues of the cell
  * coordinates mod 3, one lane per 3x3x3 block of cells; automatic where the
  * supercell has that layout): each lane keeps the (3 + 2h)^3 cells around
- * its block in registers (one byte per cell), computes every site's field
+ * its block in registers (one bit per cell: 11 registers for h = 2),
+ * computes every site's field
  * from them and, after each class, reads back only the cells of that class
  * that other lanes may have changed, instead of one LDS read per neighbour
  * and a halo copy per class.  The proposals, random draws and results are
