@@ -318,3 +318,33 @@ def states_from_run_params(run_params, system, method: str, seed: int = 0):
             raise ValueError(f"unknown method {method!r}")
         states.append(MonteCarloState(dims, occ, cond))
     return states
+
+
+def write_flat_from_casm(prim_path, eci_path, out_path: str, sparse_eci_path=None) -> None:
+    """The flattened tables of a CASM project's cluster expansion, written
+    for the C++ calculators (``CASM::clexmonte::gpu::System::from_flat_file``):
+    the route a C++ host takes from ``prim.json`` and ``*_eci.json`` to the
+    GPU path (the reference builds its System from the same files,
+    src/casm/clexmonte/system/io/json/System_json_io.cc:453-539)."""
+    from .system import FlatCE, save_flat
+    save_flat(FlatCE(load_casm_clex(prim_path, eci_path, sparse_eci_path)), out_path)
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(
+        prog="python -m casmcode_clexmonte_amd.casm_io",
+        description="Write the flat cluster-expansion file the C++ calculators read "
+                    "(System::from_flat_file) from a CASM prim.json and ECI file.")
+    ap.add_argument("--prim", required=True, help="CASM prim.json")
+    ap.add_argument("--eci", required=True, help="CASM ECI file (bspecs and orbits with cluster functions)")
+    ap.add_argument("--sparse-eci", default=None, help="optional [[linear_function_index, eci], ...] list")
+    ap.add_argument("--out", required=True, help="flat file to write")
+    a = ap.parse_args(argv)
+    write_flat_from_casm(a.prim, a.eci, a.out, a.sparse_eci)
+    print(f"wrote {a.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

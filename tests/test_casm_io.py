@@ -216,3 +216,23 @@ def test_canonical_state_from_param_composition(zro_ce):
     (st,) = states_from_run_params(rp, system, "canonical", seed=2)
     np.testing.assert_allclose(st.conditions["mol_composition"], [2.0, 1.0, 1.0])
     np.testing.assert_allclose(st.conditions["param_composition"], [0.5])
+
+
+def test_flat_file_cli_matches_save_flat(tmp_path):
+    """The command-line route from CASM files to the C++ calculators' flat
+    file writes what save_flat writes for the same expansion."""
+    import subprocess
+    import sys
+    from casmcode_clexmonte_amd.system import save_flat
+    prim = tmp_path / "prim.json"
+    eci = tmp_path / "formation_energy_eci.json"
+    prim.write_text(json.dumps(ZRO["prim"]))
+    eci.write_text(json.dumps(ZRO["formation_energy_eci"]))
+    out = tmp_path / "system.flat"
+    r = subprocess.run([sys.executable, "-m", "casmcode_clexmonte_amd.casm_io", "--prim", str(prim),
+                        "--eci", str(eci), "--out", str(out)], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    ref = tmp_path / "ref.flat"
+    save_flat(FlatCE(clex_from_casm(ZRO["prim"], ZRO["formation_energy_eci"])), str(ref))
+    assert out.read_text() == ref.read_text()
+    assert "n_sublat" in out.read_text() and "flower_tables" in out.read_text()
