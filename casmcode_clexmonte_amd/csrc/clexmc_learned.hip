@@ -17,11 +17,13 @@
 // probabilities sigmoid(-beta e_s) of sites of plain-sampler configurations.
 //
 // Proposal (exact Metropolis-Hastings): a class of an independent-set
-// partition is split into groups of 64 sites (one per lane; the grouping is a
-// random rotation chosen independently of the occupation).  Within a group a
-// wavefront makes sequential proposals: canonical, an A site a and a B site b
-// with probabilities w_a / Z_A and w_b / Z_B; semi-grand canonical, one site
-// s with probability w_s / Z.  Weights (WEIGHTS template argument):
+// partition is split into groups of 16 sites (one per lane, a DPP row of a
+// wavefront: four groups per wavefront; the grouping is a random rotation
+// chosen independently of the occupation).  Within a group the rounds make
+// sequential proposals, the four groups of a wavefront side by side:
+// canonical, an A site a and a B site b with probabilities w_a / Z_A and
+// w_b / Z_B; semi-grand canonical, one site s with probability w_s / Z.
+// Weights (WEIGHTS template argument):
 //   0 uniform  w = 1;
 //   1 learned  w = eps + sigmoid(t z(env));
 //   2 informed w = eps + sigmoid(-beta e_s), the exact Barker weight of the
@@ -185,6 +187,36 @@ __device__ __forceinline__ int wave_pick(float v, float scan, float t) {
   return __ffsll((long long)m) - 1;
 }
 
+// Inclusive prefix sum within each row of 16 lanes (DPP row shifts by 1, 2,
+// 4, 8; lanes without a source add 0).
+__device__ __forceinline__ float row_scan_incl(float x) {
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xF, 0xF, false));
+  x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xF, 0xF, false));
+  return x;
+}
+
+// The row's total (the inclusive sum of its last lane) in every lane of the
+// row; rb is the row's first lane.
+__device__ __forceinline__ float row_total(float scan, int rb) {
+  const float z0 = lane_f(scan, 15), z1 = lane_f(scan, 31), z2 = lane_f(scan, 47), z3 = lane_f(scan, 63);
+  return rb == 0 ? z0 : (rb == 16 ? z1 : (rb == 32 ? z2 : z3));
+}
+
+// Weighted choice within a row (lanes rb .. rb + 15): v >= 0 per lane, scan
+// = row_scan_incl(v), t = u * the row's total.  The first lane of the row
+// whose inclusive sum exceeds t among lanes with v > 0 (the last such lane
+// if rounding leaves none), -1 if all v of the row are 0; the same in every
+// lane of the row.  Every lane must take part (two ballots).
+__device__ __forceinline__ int row_pick(float v, float scan, float t, int rb) {
+  const unsigned long long m = __ballot(scan > t && v > 0.f);
+  const unsigned long long m2 = __ballot(v > 0.f);
+  const uint32_t mr = (uint32_t)(m >> rb) & 0xFFFFu, mr2 = (uint32_t)(m2 >> rb) & 0xFFFFu;
+  if (mr) return rb + __ffs((int)mr) - 1;
+  return mr2 ? rb + 31 - __clz((int)mr2) : -1;
+}
+
 // binary field and exact integer cluster-product counts (same as the plain kernel)
 __device__ __forceinline__ float binary_field_l(const uint8_t *img, int pa, int o, cptr<BinSub> bs, cptr<int> off,
                                                 int n_slots, int (&K)[kMaxSlots]) {
@@ -293,8 +325,8 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
   int accC[kMaxSlots];
 #pragma unroll
   for (int a = 0; a < kMaxSlots; ++a) accC[a] = 0;
-  int accN1 = 0, accAtt = 0, accAcc = 0;  // per lane (pair phases, composition)
-  int rAtt = 0, rAcc = 0;                  // per wavefront (grouped proposals; wave-uniform)
+  int accN1 = 0, accAtt = 0, accAcc = 0;  // per lane (pair phases, composition; a row's first lane
+                                           // counts the row's grouped proposals)
 
   const unsigned long long pass0 = cs->pass_base;
   cptr<int> bin_off = cst(sc->bin_off);
@@ -348,42 +380,52 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
           w = 0.0f;
           wn = 0.0f;
         }
+        // The wavefront's 64 sites are four independent groups of 16, its
+        // DPP rows: every round, each row with positive weights makes one
+        // proposal among its own sites (the rows' sites share no cluster, so
+        // their proposals are exact and concurrent).  Uniforms of 16 rounds
+        // are drawn lane-parallel (lane 16 r + k holds row r's round k) and
+        // moved to the row's lanes per round.
+        const int rb = lane & 48;  // first lane of this lane's row
         uint4 ul = make_uint4(0u, 0u, 0u, 0u);
         for (int rd = 0; rd < R; ++rd) {
-          // uniforms of 64 rounds are drawn lane-parallel (lane l: round l's
-          // Philox block) and read out per round
-          const int slot = rd & 63;
+          const int slot = rd & 15;
           if (slot == 0) {
             const unsigned long long dl = draw + (unsigned long long)lane;
             ul = philox4x32_10(make_uint4(0xFFFF0000u | (uint32_t)wave_u, (uint32_t)dl, (uint32_t)(dl >> 32),
                                           (uint32_t)chain), k0, k1);
             draw += 64ull;
           }
-          const uint32_t x1 = (uint32_t)__builtin_amdgcn_readlane((int)ul.x, slot);
-          const uint32_t x2 = (uint32_t)__builtin_amdgcn_readlane((int)ul.y, slot);
-          const uint32_t x3 = (uint32_t)__builtin_amdgcn_readlane((int)ul.z, slot);
+          const int src = rb | slot;
+          const uint32_t x1 = (uint32_t)__shfl((int)ul.x, src, 64);
+          const uint32_t x2 = (uint32_t)__shfl((int)ul.y, src, 64);
+          const uint32_t x3 = (uint32_t)__shfl((int)ul.z, src, 64);
           // the fourth word refines an acceptance test whose 32-bit word ties
-          auto more = [&]() { return (uint32_t)__builtin_amdgcn_readlane((int)ul.w, slot); };
+          const uint32_t x4 = (uint32_t)__shfl((int)ul.w, src, 64);
+          auto more = [&]() { return x4; };
           const float u1 = u01(x1);
           if (KIND == 0) {
             const float vA = (o == 0) ? w : 0.0f;
             const float vB = (o == 1) ? w : 0.0f;
-            const float sA = wave_scan_incl(vA), sB = wave_scan_incl(vB);
-            const float ZA = lane_f(sA, 63), ZB = lane_f(sB, 63);
-            if (!(ZA > 0.0f) || !(ZB > 0.0f)) break;
-            const int la_ = wave_pick(vA, sA, u1 * ZA);
-            const int lb_ = wave_pick(vB, sB, u01(x2) * ZB);
-            if (la_ < 0 || lb_ < 0) break;
-            const float wa = lane_f(w, la_), wna = lane_f(wn, la_), ea = lane_f(e, la_);
-            const float wb = lane_f(w, lb_), wnb = lane_f(wn, lb_), eb = lane_f(e, lb_);
-            const float dE = ea + eb;
-            const float ZA2 = ZA - wa + wnb, ZB2 = ZB - wb + wna;
-            const float ratio = __fdividef(wnb * wna * ZA * ZB, wa * wb * ZA2 * ZB2);
-            const bool acc = accept_prob(__expf(-beta * dE) * ratio, x3, more);
-            rAtt += 1;
-            if (acc) {
-              rAcc += 1;
-              if (lane == la_ || lane == lb_) {
+            const float sA = row_scan_incl(vA), sB = row_scan_incl(vB);
+            const float ZA = row_total(sA, rb), ZB = row_total(sB, rb);
+            const bool live = ZA > 0.0f && ZB > 0.0f;
+            const int la_ = row_pick(vA, sA, u1 * ZA, rb), lb_ = row_pick(vB, sB, u01(x2) * ZB, rb);
+            const bool prop = live && la_ >= 0 && lb_ >= 0;
+            // the picked lanes' values (every lane takes part in the shuffles)
+            const int pa_ = prop ? la_ : lane, pb_ = prop ? lb_ : lane;
+            const float wa = __shfl(w, pa_, 64), wna = __shfl(wn, pa_, 64), ea = __shfl(e, pa_, 64);
+            const float wb = __shfl(w, pb_, 64), wnb = __shfl(wn, pb_, 64), eb = __shfl(e, pb_, 64);
+            if (prop) {
+              const float dE = ea + eb;
+              const float ZA2 = ZA - wa + wnb, ZB2 = ZB - wb + wna;
+              const float ratio = __fdividef(wnb * wna * ZA * ZB, wa * wb * ZA2 * ZB2);
+              const bool acc = accept_prob(__expf(-beta * dE) * ratio, x3, more);
+              if (lane == rb) {
+                accAtt += 1;
+                accAcc += acc ? 1 : 0;
+              }
+              if (acc && (lane == la_ || lane == lb_)) {
                 o ^= 1;
                 e = -e;
                 const float t = w;
@@ -392,19 +434,21 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
               }
             }
           } else {
-            const float sv = wave_scan_incl(w);
-            const float Z = lane_f(sv, 63);
-            if (!(Z > 0.0f)) break;
-            const int ls = wave_pick(w, sv, u1 * Z);
-            if (ls < 0) break;
-            const float ws = lane_f(w, ls), wns = lane_f(wn, ls), es = lane_f(e, ls);
-            const float Z2 = Z - ws + wns;
-            const float ratio = __fdividef(wns * Z, ws * Z2);
-            const bool acc = accept_prob(__expf(-beta * es) * ratio, x2, more);
-            rAtt += 1;
-            if (acc) {
-              rAcc += 1;
-              if (lane == ls) {
+            const float sv = row_scan_incl(w);
+            const float Z = row_total(sv, rb);
+            const int ls = row_pick(w, sv, u1 * Z, rb);
+            const bool prop = Z > 0.0f && ls >= 0;
+            const int ps = prop ? ls : lane;
+            const float ws = __shfl(w, ps, 64), wns = __shfl(wn, ps, 64), es = __shfl(e, ps, 64);
+            if (prop) {
+              const float Z2 = Z - ws + wns;
+              const float ratio = __fdividef(wns * Z, ws * Z2);
+              const bool acc = accept_prob(__expf(-beta * es) * ratio, x2, more);
+              if (lane == rb) {
+                accAtt += 1;
+                accAcc += acc ? 1 : 0;
+              }
+              if (acc && lane == ls) {
                 o ^= 1;
                 e = -e;
                 const float t = w;
@@ -549,14 +593,13 @@ __global__ __launch_bounds__(512) void learned_kernel(RunArgs args, LearnArgs la
         accC[a] = 0;
       }
       {
-        const int v0 = wave_sum_i(accN1), v1 = wave_sum_i(accAtt) + rAtt, v2 = wave_sum_i(accAcc) + rAcc;
+        const int v0 = wave_sum_i(accN1), v1 = wave_sum_i(accAtt), v2 = wave_sum_i(accAcc);
         if (lane == 0) {
           redI[(kMaxSlots + 0) * 16 + wave] = v0;
           redI[(kMaxSlots + 1) * 16 + wave] = v1;
           redI[(kMaxSlots + 2) * 16 + wave] = v2;
         }
         accN1 = accAtt = accAcc = 0;
-        rAtt = rAcc = 0;
       }
       __syncthreads();
       if (tid == 0) {
