@@ -432,7 +432,7 @@ def main():
     ap.add_argument("--c4-replicas", type=int, default=8)
     ap.add_argument("--c4-equil", type=int, default=2000)
     ap.add_argument("--c4-passes", type=int, default=12000)
-    ap.add_argument("--c4-rounds", type=int, default=32)
+    ap.add_argument("--c4-rounds", type=int, default=16)
     ap.add_argument("--c4-train-samples", type=int, default=512)
     ap.add_argument("--c4-train-steps", type=int, default=200)
     ap.add_argument("--hw-queues", type=int, default=0,

@@ -418,7 +418,8 @@ clexmc_status clexmc_set_image_packing(clexmc_handle *h, int mode);
 
 /* Proposal used by clexmc_run_passes: 0 plain (independent-set sweeps and
  * random matchings), 1 learned (MLP-weighted sequential proposals within
- * groups of 64 independent sites), 2 the same grouped proposals with uniform
+ * groups of 16 independent sites, four groups side by side per wavefront),
+ * 2 the same grouped proposals with uniform
  * weights, 3 the same with informed weights eps + sigmoid(-beta e_s) of the
  * exact flip energy (no network).  rounds_per_group: proposals per group and
  * class phase.  Binary cluster expansions only. */
